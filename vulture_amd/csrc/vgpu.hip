@@ -1,0 +1,927 @@
+// C-ABI of the MI355X FDTD core (include/vgpu.h): context management,
+// uploads from the reference's k-fastest layout to the device x-fastest
+// layout, construction of the surface work lists (Debye sites, SIBC faces and
+// gathered H corrections, source and TF/SF correction rows, observers) and
+// dispatch to the per-precision step driver.
+//
+// This translation unit is compiled with -ffp-contract=off: the TF/SF
+// geometry (distances, interpolation weights) is computed here on the host
+// and must round exactly as the reference's set-up does.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include "vgpu_internal.h"
+
+namespace vgpu {
+
+int set_error(Ctx *c, int code, const char *fmt, ...)
+{
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+void *dev_alloc(Ctx *c, size_t bytes)
+{
+  if (bytes == 0) bytes = 16;
+  void *p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+  if (hipMemset(p, 0, bytes) != hipSuccess) {
+    hipFree(p);
+    return nullptr;
+  }
+  c->allocs.push_back(p);
+  c->bytes += bytes;
+  return p;
+}
+
+template <typename V> static void *upload_vec(Ctx *c, const std::vector<V> &v)
+{
+  void *p = dev_alloc(c, v.size() * sizeof(V));
+  if (p && !v.empty()) hipMemcpy(p, v.data(), v.size() * sizeof(V), hipMemcpyHostToDevice);
+  return p;
+}
+
+// Upload a host k-fastest [NX][NY][NZ] double array into an x-fastest device
+// array of the context's precision.
+static int upload_field(Ctx *c, void *dst, const double *src)
+{
+  const size_t n = c->ncell;
+  std::vector<unsigned char> buf(n * c->esz, 0);
+  const long long NY = c->NY, NZ = c->NZ;
+  for (int k = 0; k < c->NZ; ++k)
+    for (int j = 0; j < c->NY; ++j) {
+      const size_t row = ((size_t)k * c->NY + j) * c->PX;
+      for (int i = 0; i < c->NX; ++i) {
+        const double v = src[((size_t)i * NY + j) * NZ + k];
+        if (c->esz == 8) ((double *)buf.data())[row + i] = v;
+        else ((float *)buf.data())[row + i] = (float)v;
+      }
+    }
+  if (hipMemcpy(dst, buf.data(), n * c->esz, hipMemcpyHostToDevice) != hipSuccess)
+    return set_error(c, VGPU_EDEVICE, "upload failed");
+  return VGPU_OK;
+}
+
+static void *upload_line(Ctx *c, const double *src, int n)
+{
+  void *p = dev_alloc(c, (size_t)n * c->esz);
+  if (!p) return nullptr;
+  if (c->esz == 8) {
+    hipMemcpy(p, src, (size_t)n * 8, hipMemcpyHostToDevice);
+  } else {
+    std::vector<float> f(n);
+    for (int i = 0; i < n; ++i) f[i] = (float)src[i];
+    hipMemcpy(p, f.data(), (size_t)n * 4, hipMemcpyHostToDevice);
+  }
+  return p;
+}
+
+static void *upload_reals(Ctx *c, const std::vector<double> &v)
+{
+  return upload_line(c, v.data(), (int)v.size());
+}
+
+static bool box_empty(const int *b) { return b[1] < b[0] || b[3] < b[2] || b[5] < b[4]; }
+
+static long long box_cells(const int *b)
+{
+  if (box_empty(b)) return 0;
+  return (long long)(b[1] - b[0] + 1) * (b[3] - b[2] + 1) * (b[5] - b[4] + 1);
+}
+
+// ---- TF/SF geometry (planewave.c:557-573, 1375-1419; grid.c:1397-1445) ----
+
+static void field_index_location(double r[3], int field, int i, int j, int k)
+{
+  r[0] = i; r[1] = j; r[2] = k;
+  switch (field) {
+  case 0: r[0] = i + 0.5; break;
+  case 1: r[1] = j + 0.5; break;
+  case 2: r[2] = k + 0.5; break;
+  case 3: r[1] = j + 0.5; r[2] = k + 0.5; break;
+  case 4: r[0] = i + 0.5; r[2] = k + 0.5; break;
+  case 5: r[0] = i + 0.5; r[1] = j + 0.5; break;
+  }
+}
+
+static void field_physical_location(const Ctx *c, double r[3], int field, int i, int j, int k)
+{
+  const int im = i - c->grid.gibox[0], jm = j - c->grid.gibox[2], km = k - c->grid.gibox[4];
+  r[0] = c->xl[im]; r[1] = c->yl[jm]; r[2] = c->zl[km];
+  switch (field) {
+  case 0: r[0] = c->xl[im] + 0.5 * c->dex[i]; break;
+  case 1: r[1] = c->yl[jm] + 0.5 * c->dey[j]; break;
+  case 2: r[2] = c->zl[km] + 0.5 * c->dez[k]; break;
+  case 3: r[1] = c->yl[jm] + 0.5 * c->dey[j]; r[2] = c->zl[km] + 0.5 * c->dez[k]; break;
+  case 4: r[0] = c->xl[im] + 0.5 * c->dex[i]; r[2] = c->zl[km] + 0.5 * c->dez[k]; break;
+  case 5: r[0] = c->xl[im] + 0.5 * c->dex[i]; r[1] = c->yl[jm] + 0.5 * c->dey[j]; break;
+  }
+}
+
+struct RowEntry { long long key; long long seq; CorrEntry e; };
+
+}  // namespace vgpu
+
+using namespace vgpu;
+
+struct vgpu_ctx : public Ctx {};
+
+extern "C" {
+
+int vgpu_create(const vgpu_grid_desc *g, vgpu_prec prec, int device, vgpu_ctx **out)
+{
+  if (!g || !out) return VGPU_EINVAL;
+  *out = nullptr;
+  if (g->nx < 3 || g->ny < 3 || g->nz < 3) return VGPU_EINVAL;
+  if (hipSetDevice(device) != hipSuccess) return VGPU_EDEVICE;
+  vgpu_ctx *c = new vgpu_ctx();
+  c->prec = prec;
+  c->device = device;
+  c->esz = prec == VGPU_FP64 ? 8 : 4;
+  c->grid = *g;
+  c->NX = g->nx; c->NY = g->ny; c->NZ = g->nz;
+  const int align = (int)(128 / c->esz);
+  c->PX = (c->NX + align - 1) / align * align;
+  c->plane = (long long)c->PX * c->NY;
+  c->ncell = (size_t)c->plane * c->NZ;
+  auto cp = [](std::vector<double> &v, const double *p, int n) { v.assign(p, p + n); };
+  cp(c->xl, g->xlines, g->nlines[0]); cp(c->yl, g->ylines, g->nlines[1]); cp(c->zl, g->zlines, g->nlines[2]);
+  cp(c->dex, g->dex, c->NX); cp(c->dey, g->dey, c->NY); cp(c->dez, g->dez, c->NZ);
+  cp(c->idex, g->idex, c->NX); cp(c->idey, g->idey, c->NY); cp(c->idez, g->idez, c->NZ);
+  cp(c->idhx, g->idhx, c->NX); cp(c->idhy, g->idhy, c->NY); cp(c->idhz, g->idhz, c->NZ);
+  c->grid.xlines = c->grid.ylines = c->grid.zlines = nullptr;
+  c->grid.dex = c->grid.dey = c->grid.dez = nullptr;
+  c->grid.idex = c->grid.idey = c->grid.idez = nullptr;
+  c->grid.idhx = c->grid.idhy = c->grid.idhz = nullptr;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return VGPU_EDEVICE;
+  }
+  hipEventCreate(&c->ev0);
+  hipEventCreate(&c->ev1);
+  for (int d = 0; d < 6; ++d) {
+    c->F[d] = dev_alloc(c, c->ncell * c->esz);
+    if (!c->F[d]) {
+      vgpu_destroy(c);
+      return VGPU_ENOMEM;
+    }
+  }
+  for (int d = 0; d < 3; ++d) {
+    c->alpha[d] = dev_alloc(c, c->ncell * c->esz);
+    c->beta[d] = dev_alloc(c, c->ncell * c->esz);
+    c->gamma[d] = dev_alloc(c, c->ncell * c->esz);
+    if (!c->alpha[d] || !c->beta[d] || !c->gamma[d]) {
+      vgpu_destroy(c);
+      return VGPU_ENOMEM;
+    }
+  }
+  c->ide[0] = upload_reals(c, c->idex); c->ide[1] = upload_reals(c, c->idey); c->ide[2] = upload_reals(c, c->idez);
+  c->idh[0] = upload_reals(c, c->idhx); c->idh[1] = upload_reals(c, c->idhy); c->idh[2] = upload_reals(c, c->idhz);
+  for (int r = 0; r < 6; ++r)
+    for (int f = 0; f < 6; ++f)
+      for (int q = 0; q < 6; ++q) { c->fplim[r][f][q] = 0; c->fmlim[r][f][q] = 0; }
+  const char *ev = getenv("VGPU_KERNEL_EVENTS");
+  int nev = ev ? atoi(ev) : 0;
+  for (int q = 0; q < 2 * nev; ++q) {
+    hipEvent_t e1, e2;
+    hipEventCreate(&e1);
+    hipEventCreate(&e2);
+    c->evE.push_back(e1);
+    c->evH.push_back(e2);
+  }
+  *out = c;
+  return VGPU_OK;
+}
+
+int vgpu_set_media(vgpu_ctx *c, const double *alpha[3], const double *beta[3], const double *gamma[3])
+{
+  if (!c || c->finalized) return VGPU_ESTATE;
+  for (int d = 0; d < 3; ++d) {
+    int r;
+    if ((r = upload_field(c, c->alpha[d], alpha[d])) || (r = upload_field(c, c->beta[d], beta[d])) ||
+        (r = upload_field(c, c->gamma[d], gamma[d])))
+      return r;
+  }
+  c->media_set = true;
+  return VGPU_OK;
+}
+
+int vgpu_set_pml(vgpu_ctx *c, const vgpu_pml_desc *p)
+{
+  if (!c || c->finalized || !p) return VGPU_ESTATE;
+  c->pmlmask = p->region_mask & 0x3Fu;
+  memcpy(c->fplim, p->fplim, sizeof(c->fplim));
+  for (int r = 0; r < 6; ++r) {
+    if (!((c->pmlmask >> r) & 1u)) continue;
+    for (int d = 0; d < 3; ++d) {
+      const long long ne = box_cells(c->fplim[r][d]), nh = box_cells(c->fplim[r][3 + d]);
+      c->aux1[r][d] = dev_alloc(c, (size_t)ne * c->esz);
+      c->aux2[r][d] = dev_alloc(c, (size_t)ne * c->esz);
+      c->auxB[r][d] = dev_alloc(c, (size_t)nh * c->esz);
+      if (!c->aux1[r][d] || !c->aux2[r][d] || !c->auxB[r][d]) return set_error(c, VGPU_ENOMEM, "UPML allocation");
+    }
+  }
+  const int n[3] = {c->NX, c->NY, c->NZ};
+  for (int d = 0; d < 3; ++d) {
+    const double *src[6] = {p->ad[d], p->bd[d], p->ah[d], p->bh[d], p->ibd[d], p->ibh[d]};
+    for (int q = 0; q < 6; ++q) {
+      if (!src[q]) return set_error(c, VGPU_EINVAL, "missing UPML profile array");
+      c->prof[q][d] = upload_line(c, src[q], n[d]);
+    }
+  }
+  return VGPU_OK;
+}
+
+int vgpu_set_mur(vgpu_ctx *c, unsigned face_mask, const double zeta[6], const int fmlim[6][6][6])
+{
+  if (!c || c->finalized) return VGPU_ESTATE;
+  c->murmask = face_mask & 0x3Fu;
+  for (int f = 0; f < 6; ++f) c->zeta[f] = zeta[f];
+  memcpy(c->fmlim, fmlim, sizeof(c->fmlim));
+  return VGPU_OK;
+}
+
+int vgpu_set_ghosts(vgpu_ctx *c, const int face_type[6])
+{
+  if (!c || c->finalized) return VGPU_ESTATE;
+  for (int f = 0; f < 6; ++f) {
+    if (face_type[f] < 0 || face_type[f] > 3) return set_error(c, VGPU_EINVAL, "bad ghost type");
+    c->ghost[f] = face_type[f];
+  }
+  return VGPU_OK;
+}
+
+int vgpu_add_debye_block(vgpu_ctx *c, const int flim[3][6], int npoles, const double *dalpha, const double *dbeta)
+{
+  if (!c || c->finalized) return VGPU_ESTATE;
+  if (npoles < 0 || (npoles > 0 && (!dalpha || !dbeta))) return set_error(c, VGPU_EINVAL, "bad Debye block");
+  Ctx::DebyeBlock b;
+  memcpy(b.flim, flim, sizeof(b.flim));
+  b.np = npoles;
+  b.da.assign(dalpha, dalpha + 2 * npoles);
+  b.db.assign(dbeta, dbeta + 2 * npoles);
+  c->debye.push_back(b);
+  return VGPU_OK;
+}
+
+int vgpu_add_sibc_surface(vgpu_ctx *c, const vgpu_sibc_desc *d)
+{
+  if (!c || c->finalized || !d) return VGPU_ESTATE;
+  Ctx::SibcSurf s;
+  s.d = *d;
+  const long long nf = (long long)(d->gbbox[1] - d->gbbox[0]) * (d->gbbox[3] - d->gbbox[2]) * (d->gbbox[5] - d->gbbox[4]);
+  if (nf <= 0 || d->normal < 0 || d->normal > 2) return set_error(c, VGPU_EINVAL, "bad SIBC surface");
+  s.adjA.assign(d->is_adj_a, d->is_adj_a + 4 * nf);
+  s.adjB.assign(d->is_adj_b, d->is_adj_b + 4 * nf);
+  for (int f = 0; f < 16; ++f) {
+    const int np = d->npoles[f];
+    if (np < 0) return set_error(c, VGPU_EINVAL, "bad SIBC filter");
+    if (np > 0) {
+      s.q0[f].assign(d->q0[f], d->q0[f] + 2 * np);
+      s.q1[f].assign(d->q1[f], d->q1[f] + 2 * np);
+      s.q2[f].assign(d->q2[f], d->q2[f] + 2 * np);
+    }
+    c->sibcMaxPoles = std::max(c->sibcMaxPoles, np);
+  }
+  s.d.is_adj_a = s.d.is_adj_b = nullptr;
+  for (int f = 0; f < 16; ++f) s.d.q0[f] = s.d.q1[f] = s.d.q2[f] = nullptr;
+  c->sibc.push_back(std::move(s));
+  c->sibcFaces += nf;
+  return VGPU_OK;
+}
+
+int vgpu_add_waveform(vgpu_ctx *c, int type, double size, double delay, double width, double frequency,
+                      const double *table, unsigned long n)
+{
+  if (!c || c->finalized) return VGPU_ESTATE;
+  if (type < 0 || type > 9) return set_error(c, VGPU_EINVAL, "bad waveform type %d", type);
+  WaveformDev w{};
+  w.type = type; w.size = size; w.delay = delay; w.width = width; w.frequency = frequency;
+  w.n = 0;
+  c->wftab.emplace_back();
+  if (type == 9) {
+    if (!table || n < 2) return set_error(c, VGPU_EINVAL, "external waveform needs a table");
+    c->wftab.back().assign(table, table + 3 * n);
+    w.n = n;
+  }
+  c->wf.push_back(w);
+  return (int)c->wf.size() - 1;
+}
+
+int vgpu_add_source(vgpu_ctx *c, int field, int kind, int soft, double size, int waveform, double delay,
+                    const int flim[6])
+{
+  if (!c || c->finalized) return VGPU_ESTATE;
+  if (field < 0 || field > 5 || kind < 0 || kind > 1 || waveform < 0 || waveform >= (int)c->wf.size())
+    return set_error(c, VGPU_EINVAL, "bad source");
+  Ctx::Src s{};
+  s.field = field; s.kind = kind; s.soft = soft ? 1 : 0; s.size = size; s.waveform = waveform; s.delay = delay;
+  memcpy(s.flim, flim, sizeof(s.flim));
+  c->src.push_back(s);
+  return (int)c->src.size() - 1;
+}
+
+int vgpu_add_planewave(vgpu_ctx *c, const vgpu_pw_desc *d)
+{
+  if (!c || c->finalized || !d) return VGPU_ESTATE;
+  if (d->waveform < 0 || d->waveform >= (int)c->wf.size()) return set_error(c, VGPU_EINVAL, "bad plane-wave waveform");
+  Ctx::Pw p{};
+  p.d = *d;
+  if (d->use_aux) {
+    if (d->aux_nx < 4 || d->aux_xb < 1 || d->aux_xb > d->aux_nx || d->aux_nx - d->aux_xb > 10)
+      return set_error(c, VGPU_EINVAL, "bad auxiliary grid");
+    const double *pr[4] = {d->aux_adx, d->aux_bdx, d->aux_ahx, d->aux_bhx};
+    for (int q = 0; q < 4; ++q)
+      for (int i = 0; i < 10; ++i) p.prof[q][i] = pr[q] ? pr[q][i] : 0.0;
+  }
+  p.d.aux_adx = p.d.aux_bdx = p.d.aux_ahx = p.d.aux_bhx = nullptr;
+  c->pw.push_back(p);
+  return (int)c->pw.size() - 1;
+}
+
+int vgpu_add_point_observer(vgpu_ctx *c, int i, int j, int k, uint64_t first, uint64_t last, int nfreq,
+                            const double *omega)
+{
+  if (!c || c->finalized) return VGPU_ESTATE;
+  if (i < 0 || j < 0 || k < 0 || i >= c->NX || j >= c->NY || k >= c->NZ || nfreq < 0 || (nfreq && !omega))
+    return set_error(c, VGPU_EINVAL, "bad point observer");
+  Ctx::PObs o;
+  o.i = i; o.j = j; o.k = k; o.first = first; o.last = last;
+  if (nfreq) o.omega.assign(omega, omega + nfreq);
+  c->pobs.push_back(o);
+  return (int)c->pobs.size() - 1;
+}
+
+int vgpu_add_volume_observer(vgpu_ctx *c, const int gbbox[6], const int step[3], uint64_t first, uint64_t last)
+{
+  if (!c || c->finalized) return VGPU_ESTATE;
+  for (int d = 0; d < 3; ++d)
+    if (step[d] <= 0 || gbbox[2 * d] < 0 || gbbox[2 * d + 1] >= (d == 0 ? c->NX : d == 1 ? c->NY : c->NZ) ||
+        gbbox[2 * d] > gbbox[2 * d + 1])
+      return set_error(c, VGPU_EINVAL, "bad volume observer");
+  Ctx::VObs o{};
+  memcpy(o.box, gbbox, sizeof(o.box));
+  memcpy(o.step, step, sizeof(o.step));
+  o.first = first; o.last = last;
+  o.nodes = 1;
+  for (int d = 0; d < 3; ++d) o.nodes *= 1 + (gbbox[2 * d + 1] - gbbox[2 * d]) / step[d];
+  c->vobs.push_back(o);
+  return (int)c->vobs.size() - 1;
+}
+
+// ---------------------------------------------------------------------------
+// finalize: build device lists
+// ---------------------------------------------------------------------------
+
+static int build_debye(Ctx *c)
+{
+  if (c->debye.empty()) return VGPU_OK;
+  // Debye media table: one slot per block (blocks share media in the
+  // reference, but the coefficient table is tiny).
+  int maxP = 1, cplx = 0;
+  for (auto &b : c->debye) {
+    maxP = std::max(maxP, b.np);
+    for (int p = 0; p < b.np; ++p)
+      if (b.da[2 * p + 1] != 0.0 || b.db[2 * p + 1] != 0.0) cplx = 1;
+  }
+  c->debyeMaxPoles = maxP;
+  c->debyeCplx = cplx;
+  std::vector<double> da((size_t)c->debye.size() * maxP * 2, 0.0), db(da.size(), 0.0);
+  std::vector<int> np(c->debye.size());
+  for (size_t b = 0; b < c->debye.size(); ++b) {
+    np[b] = c->debye[b].np;
+    for (int p = 0; p < c->debye[b].np; ++p)
+      for (int q = 0; q < 2; ++q) {
+        da[(b * maxP + p) * 2 + q] = c->debye[b].da[2 * p + q];
+        db[(b * maxP + p) * 2 + q] = c->debye[b].db[2 * p + q];
+      }
+  }
+  // Sites in block order; a cell already covered by an earlier block goes to
+  // the next layer so corrections apply in block order.
+  struct Site { int layer, comp; long long cell; int med; };
+  std::vector<Site> sites;
+  std::map<long long, int> seen;
+  int layers = 1;
+  for (size_t b = 0; b < c->debye.size(); ++b)
+    for (int f = 0; f < 3; ++f) {
+      const int *l = c->debye[b].flim[f];
+      for (int i = l[0]; i <= l[1]; ++i)
+        for (int j = l[2]; j <= l[3]; ++j)
+          for (int k = l[4]; k <= l[5]; ++k) {
+            if (i < 0 || j < 0 || k < 0 || i >= c->NX || j >= c->NY || k >= c->NZ)
+              return set_error(c, VGPU_EINVAL, "Debye block outside the grid");
+            const long long cell = c->lin(i, j, k);
+            const long long key = cell * 3 + f;
+            int &cnt = seen[key];
+            const int layer = cnt++;
+            layers = std::max(layers, layer + 1);
+            sites.push_back({layer, f, cell, (int)b});
+          }
+    }
+  std::stable_sort(sites.begin(), sites.end(), [](const Site &a, const Site &b) {
+    if (a.layer != b.layer) return a.layer < b.layer;
+    if (a.comp != b.comp) return a.comp < b.comp;
+    return a.cell < b.cell;
+  });
+  const long long n = (long long)sites.size();
+  std::vector<int> comp(n), med(n);
+  std::vector<long long> cell(n);
+  c->debyeLayerStart.assign(layers + 1, n);
+  for (long long s = n - 1; s >= 0; --s) c->debyeLayerStart[sites[s].layer] = s;
+  for (int l = layers - 1; l >= 0; --l)
+    if (c->debyeLayerStart[l] > c->debyeLayerStart[l + 1]) c->debyeLayerStart[l] = c->debyeLayerStart[l + 1];
+  for (long long s = 0; s < n; ++s) { comp[s] = sites[s].comp; cell[s] = sites[s].cell; med[s] = sites[s].med; }
+  c->debyeLayers = layers;
+  c->debyeSites = n;
+  c->d_dcomp = upload_vec(c, comp);
+  c->d_dcell = upload_vec(c, cell);
+  c->d_dmed = upload_vec(c, med);
+  c->d_dlast = dev_alloc(c, (size_t)n * c->esz);
+  c->d_dJ = dev_alloc(c, (size_t)n * maxP * (cplx ? 2 : 1) * c->esz);
+  c->d_dalpha = upload_vec(c, da);
+  c->d_dbeta = upload_vec(c, db);
+  c->d_dnp = upload_vec(c, np);
+  if (!c->d_dlast || !c->d_dJ) return set_error(c, VGPU_ENOMEM, "Debye state allocation");
+  return VGPU_OK;
+}
+
+static int build_sibc(Ctx *c)
+{
+  if (c->sibc.empty()) return VGPU_OK;
+  const int maxP = c->sibcMaxPoles;
+  const long long NF = c->sibcFaces;
+  const int ns = (int)c->sibc.size();
+  std::vector<int> ijk(3 * NF), surf(NF), normal(ns), set(ns), npl(16 * ns);
+  std::vector<unsigned char> adjA(4 * NF), adjB(4 * NF);
+  std::vector<double> A(16 * ns), B(16 * ns), asy(16 * ns), q((size_t)16 * ns * maxP * 6, 0.0);
+  // Gathered H corrections: (cell key, sequence) -> entry.
+  struct HE { long long key; long long seq; SibcHEntry e; };
+  std::vector<HE> hents;
+  std::vector<long long> zeroKeys;
+  long long f = 0, seq = 0;
+  const std::vector<double> *ide[3] = {&c->idex, &c->idey, &c->idez};
+  for (int s = 0; s < ns; ++s) {
+    const vgpu_sibc_desc &d = c->sibc[s].d;
+    normal[s] = d.normal;
+    set[s] = s;
+    for (int p = 0; p < 16; ++p) {
+      A[16 * s + p] = d.A[p / 4][p % 4];
+      B[16 * s + p] = d.B[p / 4][p % 4];
+      npl[16 * s + p] = d.npoles[p];
+      asy[16 * s + p] = d.asymp[p];
+      for (int kp = 0; kp < d.npoles[p]; ++kp)
+        for (int r = 0; r < 2; ++r) {
+          const size_t base = (((size_t)16 * s + p) * maxP + kp) * 6;
+          q[base + 0 + r] = c->sibc[s].q0[p][2 * kp + r];
+          q[base + 2 + r] = c->sibc[s].q1[p][2 * kp + r];
+          q[base + 4 + r] = c->sibc[s].q2[p][2 * kp + r];
+        }
+    }
+    auto pmc = [&](int dir, int idx) {
+      return (d.pmc_index[dir][0] >= 0 && idx == d.pmc_index[dir][0]) ||
+             (d.pmc_index[dir][1] >= 0 && idx == d.pmc_index[dir][1]);
+    };
+    long long lf = 0;
+    for (int i = d.gbbox[0]; i < d.gbbox[1]; ++i)
+      for (int j = d.gbbox[2]; j < d.gbbox[3]; ++j)
+        for (int k = d.gbbox[4]; k < d.gbbox[5]; ++k, ++lf, ++f) {
+          if (i < 1 || j < 1 || k < 1 || i + 1 >= c->NX || j + 1 >= c->NY || k + 1 >= c->NZ)
+            return set_error(c, VGPU_EINVAL, "SIBC face too close to the grid edge");
+          ijk[3 * f] = i; ijk[3 * f + 1] = j; ijk[3 * f + 2] = k;
+          surf[f] = s;
+          const unsigned char *a = &c->sibc[s].adjA[4 * lf], *b = &c->sibc[s].adjB[4 * lf];
+          for (int r = 0; r < 4; ++r) { adjA[4 * f + r] = a[r]; adjB[4 * f + r] = b[r]; }
+          // terms in sibc.c:638-757 order: (comp, di, dj, dk, sign, weight dir/index, Etan slot, id index)
+          auto edge = [&](int comp, int ci, int cj, int ck, double sign, double w, int slot, int idaxis, int idi) {
+            SibcHEntry e{};
+            e.face = (int)f; e.kind = 0; e.ea = slot; e.eb = slot; e.sign = sign; e.w = w;
+            e.ida = (*ide[idaxis])[idi]; e.idb = 0.0; e.na = e.nb = 0.0;
+            hents.push_back({c->lin(ci, cj, ck) * 6 + comp, seq++, e});
+          };
+          auto adj = [&](int comp, int ci, int cj, int ck, double sign, double w, int sa, int sb, int idaxis,
+                         int ia, int ib, int r) {
+            SibcHEntry e{};
+            e.face = (int)f; e.kind = 1; e.ea = sa; e.eb = sb; e.sign = sign; e.w = w;
+            e.ida = (*ide[idaxis])[ia]; e.idb = (*ide[idaxis])[ib];
+            e.na = a[r] ? 0.0 : 1.0; e.nb = b[r] ? 0.0 : 1.0;
+            hents.push_back({c->lin(ci, cj, ck) * 6 + comp, seq++, e});
+          };
+          if (d.normal == 0) {
+            const double wyl = 0.5 * (1 + pmc(2, k)), wyh = 0.5 * (1 + pmc(2, k + 1));
+            const double wzl = 0.5 * (1 + pmc(1, j)), wzh = 0.5 * (1 + pmc(1, j + 1));
+            zeroKeys.push_back(c->lin(i, j, k) * 6 + 3);
+            edge(4, i - 1, j, k, +1, wzl, 2, 0, i - 1);
+            edge(4, i - 1, j + 1, k, +1, wzh, 2, 0, i - 1);
+            edge(4, i, j, k, -1, wzl, 3, 0, i);
+            edge(4, i, j + 1, k, -1, wzh, 3, 0, i);
+            edge(5, i - 1, j, k, -1, wyl, 0, 0, i - 1);
+            edge(5, i - 1, j, k + 1, -1, wyh, 0, 0, i - 1);
+            edge(5, i, j, k, +1, wyl, 1, 0, i);
+            edge(5, i, j, k + 1, +1, wyh, 1, 0, i);
+            adj(3, i, j + 1, k, -1, wzh, 2, 3, 0, i - 1, i, 0);
+            adj(3, i, j - 1, k, +1, wzl, 2, 3, 0, i - 1, i, 1);
+            adj(3, i, j, k + 1, -1, wyh, 0, 1, 0, i - 1, i, 2);
+            adj(3, i, j, k - 1, +1, wyl, 0, 1, 0, i - 1, i, 3);
+          } else if (d.normal == 1) {
+            const double wzl = 0.5 * (1 + pmc(0, i)), wzh = 0.5 * (1 + pmc(0, i + 1));
+            const double wxl = 0.5 * (1 + pmc(2, k)), wxh = 0.5 * (1 + pmc(2, k + 1));
+            zeroKeys.push_back(c->lin(i, j, k) * 6 + 4);
+            edge(5, i, j - 1, k, +1, wxl, 2, 1, j - 1);
+            edge(5, i, j - 1, k + 1, +1, wxh, 2, 1, j - 1);
+            edge(5, i, j, k, -1, wxl, 3, 1, j);
+            edge(5, i, j, k + 1, -1, wxh, 3, 1, j);
+            edge(3, i, j - 1, k, -1, wzl, 0, 1, j - 1);
+            edge(3, i + 1, j - 1, k, -1, wzh, 0, 1, j - 1);
+            edge(3, i, j, k, +1, wzl, 1, 1, j);
+            edge(3, i + 1, j, k, +1, wzh, 1, 1, j);
+            // The B-side term of the first two uses the Ex slot (sibc.c:689-690).
+            adj(4, i, j, k + 1, +1, wzh, 0, 3, 1, j - 1, j, 0);
+            adj(4, i, j, k - 1, -1, wzl, 0, 3, 1, j - 1, j, 1);
+            adj(4, i + 1, j, k, +1, wxh, 2, 3, 1, j - 1, j, 2);
+            adj(4, i - 1, j, k, -1, wxl, 2, 3, 1, j - 1, j, 3);
+          } else {
+            const double wxl = 0.5 * (1 + pmc(1, j)), wxh = 0.5 * (1 + pmc(1, j + 1));
+            const double wyl = 0.5 * (1 + pmc(0, i)), wyh = 0.5 * (1 + pmc(0, i + 1));
+            zeroKeys.push_back(c->lin(i, j, k) * 6 + 5);
+            edge(3, i, j, k - 1, +1, wyl, 2, 2, k - 1);
+            edge(3, i + 1, j, k - 1, +1, wyh, 2, 2, k - 1);
+            edge(3, i, j, k, -1, wyl, 3, 2, k);
+            edge(3, i + 1, j, k, -1, wyh, 3, 2, k);
+            edge(4, i, j, k - 1, -1, wxl, 0, 2, k - 1);
+            edge(4, i, j + 1, k - 1, -1, wxh, 0, 2, k - 1);
+            edge(4, i, j, k, +1, wxl, 1, 2, k);
+            edge(4, i, j + 1, k, +1, wxh, 1, 2, k);
+            adj(5, i + 1, j, k, +1, wyh, 2, 3, 2, k - 1, k, 0);
+            adj(5, i - 1, j, k, -1, wyl, 2, 3, 2, k - 1, k, 1);
+            adj(5, i, j + 1, k, +1, wxh, 0, 1, 2, k - 1, k, 2);
+            adj(5, i, j - 1, k, -1, wxl, 0, 1, 2, k - 1, k, 3);
+          }
+        }
+  }
+  std::stable_sort(hents.begin(), hents.end(), [](const HE &a, const HE &b) { return a.key < b.key; });
+  std::sort(zeroKeys.begin(), zeroKeys.end());
+  zeroKeys.erase(std::unique(zeroKeys.begin(), zeroKeys.end()), zeroKeys.end());
+  std::vector<int> rcomp, rstart, rzero;
+  std::vector<long long> rcell;
+  std::vector<SibcHEntry> ents;
+  size_t e = 0, zq = 0;
+  while (e < hents.size() || zq < zeroKeys.size()) {
+    long long key;
+    if (zq < zeroKeys.size() && (e >= hents.size() || zeroKeys[zq] <= hents[e].key)) key = zeroKeys[zq];
+    else key = hents[e].key;
+    const bool isZero = zq < zeroKeys.size() && zeroKeys[zq] == key;
+    rcomp.push_back((int)(key % 6) - 3);
+    rcell.push_back(key / 6);
+    rzero.push_back(isZero ? 1 : 0);
+    rstart.push_back((int)ents.size());
+    while (e < hents.size() && hents[e].key == key) {
+      if (!isZero) ents.push_back(hents[e].e);
+      ++e;
+    }
+    if (isZero) ++zq;
+  }
+  rstart.push_back((int)ents.size());
+  c->sibcHRows = (long long)rcomp.size();
+  c->d_sijk = upload_vec(c, ijk); c->d_ssurf = upload_vec(c, surf);
+  c->d_sadjA = upload_vec(c, adjA); c->d_sadjB = upload_vec(c, adjB);
+  c->d_snormal = upload_vec(c, normal); c->d_sA = upload_vec(c, A); c->d_sB = upload_vec(c, B);
+  c->d_sset = upload_vec(c, set); c->d_snp = upload_vec(c, npl); c->d_sasymp = upload_vec(c, asy);
+  c->d_sq = upload_vec(c, q);
+  c->d_szeta = dev_alloc(c, (size_t)16 * maxP * NF * 2 * sizeof(double));
+  c->d_sold = dev_alloc(c, (size_t)16 * NF * sizeof(double));
+  c->d_setan = dev_alloc(c, (size_t)4 * NF * c->esz);
+  c->d_shcomp = upload_vec(c, rcomp); c->d_shcell = upload_vec(c, rcell);
+  c->d_shstart = upload_vec(c, rstart); c->d_shzero = upload_vec(c, rzero);
+  c->d_shent = upload_vec(c, ents);
+  if (!c->d_szeta || !c->d_sold || !c->d_setan) return set_error(c, VGPU_ENOMEM, "SIBC state allocation");
+  return VGPU_OK;
+}
+
+// TF/SF face table: face, component, incident component, offset, sign
+// (planewave.c:596-1003, eqns 5.48-5.59).
+static const int TFSF_E[12][5] = {{2, 0, 5, -1, -1}, {2, 2, 3, -1, +1}, {3, 0, 5, 0, +1}, {3, 2, 3, 0, -1},
+                                  {4, 0, 4, -1, +1}, {4, 1, 3, -1, -1}, {5, 0, 4, 0, -1}, {5, 1, 3, 0, +1},
+                                  {0, 1, 5, -1, +1}, {0, 2, 4, -1, -1}, {1, 1, 5, 0, -1}, {1, 2, 4, 0, +1}};
+static const int TFSF_H[12][5] = {{2, 5, 0, +1, -1}, {2, 3, 2, +1, +1}, {3, 5, 0, 0, +1}, {3, 3, 2, 0, -1},
+                                  {4, 4, 0, +1, +1}, {4, 3, 1, +1, -1}, {5, 4, 0, 0, -1}, {5, 3, 1, 0, +1},
+                                  {0, 5, 1, +1, +1}, {0, 4, 2, +1, -1}, {1, 5, 1, 0, -1}, {1, 4, 2, 0, +1}};
+
+static int build_corrections(Ctx *c)
+{
+  std::vector<RowEntry> rows[2];
+  long long seq = 0;
+  const bool cubic = c->grid.grid_type == 0;
+  // sources first (source.c:602-753)
+  for (size_t s = 0; s < c->src.size(); ++s) {
+    const Ctx::Src &src = c->src[s];
+    const int e = src.field < 3 ? 0 : 1;
+    const int comp = src.field % 3;
+    const int *l = src.flim;
+    for (int i = l[0]; i <= l[1]; ++i)
+      for (int j = l[2]; j <= l[3]; ++j)
+        for (int k = l[4]; k <= l[5]; ++k) {
+          if (i < 0 || j < 0 || k < 0 || i >= c->NX || j >= c->NY || k >= c->NZ)
+            return set_error(c, VGPU_EINVAL, "source outside the grid");
+          CorrEntry ce{};
+          ce.kind = src.kind;
+          ce.idx = (int)s;
+          rows[e].push_back({c->lin(i, j, k) * 3 + comp, seq++, ce});
+        }
+  }
+  // then plane waves, face by face
+  const std::vector<double> *idh[3] = {&c->idhx, &c->idhy, &c->idhz};
+  const std::vector<double> *ide[3] = {&c->idex, &c->idey, &c->idez};
+  for (int e = 0; e < 2; ++e)
+    for (size_t p = 0; p < c->pw.size(); ++p) {
+      const vgpu_pw_desc &d = c->pw[p].d;
+      for (int q = 0; q < 12; ++q) {
+        const int *t = e == 0 ? TFSF_E[q] : TFSF_H[q];
+        const int face = t[0], comp = t[1], inc = t[2], off = t[3];
+        if (!d.active[face]) continue;
+        const int axis = face >> 1;
+        const int *l = d.flim[face][comp];
+        int lo[3] = {l[0], l[2], l[4]}, hi[3] = {l[1], l[3], l[5]};
+        lo[axis] = hi[axis] = (face & 1) ? l[2 * axis + 1] : l[2 * axis];
+        for (int i = lo[0]; i <= hi[0]; ++i)
+          for (int j = lo[1]; j <= hi[1]; ++j)
+            for (int k = lo[2]; k <= hi[2]; ++k) {
+              if (i < 1 || j < 1 || k < 1 || i + 1 >= c->NX || j + 1 >= c->NY || k + 1 >= c->NZ)
+                return set_error(c, VGPU_EINVAL, "plane-wave face outside the grid");
+              const int n3[3] = {i, j, k};
+              int s3[3] = {i, j, k};
+              s3[axis] += off;
+              CorrEntry ce{};
+              ce.idx = (int)p;
+              ce.inc = inc;
+              ce.sign = t[4];
+              ce.il = e == 0 ? (*idh[axis])[n3[axis]] : (*ide[axis])[n3[axis]];
+              double r[3], dd;
+              if (cubic && d.use_aux) {
+                field_index_location(r, inc, s3[0], s3[1], s3[2]);
+                dd = d.kinc[0] * (r[0] - d.ijk0[0]) + d.kinc[1] * (r[1] - d.ijk0[1]) + d.kinc[2] * (r[2] - d.ijk0[2]);
+                ce.kind = 2;
+                int id;
+                double dp;
+                if (inc < 3) {
+                  id = (int)floor(dd);
+                  dp = dd - id;
+                  if (2 + id < 0 || 2 + id + 1 > d.aux_nx) return set_error(c, VGPU_EINVAL, "aux grid too short");
+                } else {
+                  id = (int)floor(dd + 0.5);
+                  dp = dd + 0.5 - id;
+                  if (1 + id < 0 || 2 + id > d.aux_nx) return set_error(c, VGPU_EINVAL, "aux grid too short");
+                }
+                ce.auxid = id;
+                ce.geo = dp;
+              } else {
+                field_physical_location(c, r, inc, s3[0], s3[1], s3[2]);
+                dd = d.kinc[0] * (r[0] - d.r0[0]) + d.kinc[1] * (r[1] - d.r0[1]) + d.kinc[2] * (r[2] - d.r0[2]);
+                ce.kind = 3;
+                ce.geo = dd / d.phase_velocity;
+              }
+              rows[e].push_back({c->lin(i, j, k) * 3 + comp, seq++, ce});
+            }
+      }
+    }
+  for (int e = 0; e < 2; ++e) {
+    std::stable_sort(rows[e].begin(), rows[e].end(), [](const RowEntry &a, const RowEntry &b) { return a.key < b.key; });
+    std::vector<int> rcomp, rstart;
+    std::vector<long long> rcell;
+    std::vector<CorrEntry> ents;
+    for (size_t q = 0; q < rows[e].size(); ++q) {
+      if (q == 0 || rows[e][q].key != rows[e][q - 1].key) {
+        rcomp.push_back((int)(rows[e][q].key % 3));
+        rcell.push_back(rows[e][q].key / 3);
+        rstart.push_back((int)ents.size());
+      }
+      ents.push_back(rows[e][q].e);
+    }
+    rstart.push_back((int)ents.size());
+    c->corrRows[e] = (long long)rcomp.size();
+    if (c->corrRows[e]) {
+      c->d_ccomp[e] = upload_vec(c, rcomp);
+      c->d_ccell[e] = upload_vec(c, rcell);
+      c->d_cstart[e] = upload_vec(c, rstart);
+      c->d_cent[e] = upload_vec(c, ents);
+    }
+  }
+  // waveform, source and plane-wave tables
+  std::vector<WaveformDev> wf = c->wf;
+  for (size_t w = 0; w < wf.size(); ++w)
+    if (wf[w].type == 9) {
+      void *t = upload_vec(c, c->wftab[w]);
+      wf[w].table = (const double *)t;
+    }
+  c->d_wf = upload_vec(c, wf);
+  std::vector<SourceDev> sd;
+  for (auto &s : c->src) sd.push_back({s.kind, s.soft, s.waveform, s.size, s.delay});
+  c->d_src = upload_vec(c, sd);
+  std::vector<PwDev> pd;
+  int auxmax = 1;
+  for (auto &p : c->pw) {
+    PwDev x{};
+    for (int q = 0; q < 6; ++q) x.Finc[q] = p.d.Finc[q];
+    x.waveform = p.d.waveform; x.delay = p.d.delay;
+    x.use_aux = cubic && p.d.use_aux; x.nx = p.d.aux_nx; x.xb = p.d.aux_xb;
+    x.beta = p.d.beta_eyi; x.gamma = p.d.gamma_hzi;
+    pd.push_back(x);
+    auxmax = std::max(auxmax, p.d.aux_nx + 1);
+  }
+  c->auxmax = auxmax;
+  c->d_pw = upload_vec(c, pd);
+  if (!c->pw.empty()) {
+    c->d_aux = dev_alloc(c, c->pw.size() * 2 * (size_t)auxmax * c->esz);
+    c->d_auxpml = dev_alloc(c, c->pw.size() * 30 * c->esz);
+    std::vector<double> prof;
+    for (auto &p : c->pw)
+      for (int q = 0; q < 4; ++q)
+        for (int i = 0; i < 10; ++i) prof.push_back(p.prof[q][i]);
+    c->d_auxprof = upload_reals(c, prof);
+  }
+  return VGPU_OK;
+}
+
+static int build_observers(Ctx *c)
+{
+  if (!c->pobs.empty()) {
+    std::vector<long long> cell, off;
+    std::vector<unsigned long long> first, last;
+    std::vector<int> nf;
+    std::vector<double> om;
+    long long o6 = 0;
+    for (auto &o : c->pobs) {
+      cell.push_back(c->lin(o.i, o.j, o.k));
+      first.push_back(o.first);
+      last.push_back(o.last);
+      nf.push_back((int)o.omega.size());
+      off.push_back(o6);
+      o6 += 6 * (long long)o.omega.size();
+      om.insert(om.end(), o.omega.begin(), o.omega.end());
+    }
+    c->d_ocell = upload_vec(c, cell); c->d_ofirst = upload_vec(c, first); c->d_olast = upload_vec(c, last);
+    c->d_onf = upload_vec(c, nf); c->d_ooff = upload_vec(c, off); c->d_oomega = upload_vec(c, om);
+    c->d_otd = dev_alloc(c, c->pobs.size() * (size_t)std::max<uint64_t>(c->totalSteps, 1) * 6 * sizeof(double));
+    c->d_odre = dev_alloc(c, (size_t)std::max<long long>(o6, 1) * sizeof(double));
+    c->d_odim = dev_alloc(c, (size_t)std::max<long long>(o6, 1) * sizeof(double));
+    if (!c->d_otd || !c->d_odre || !c->d_odim) return set_error(c, VGPU_ENOMEM, "observer buffers");
+  }
+  if (!c->vobs.empty()) {
+    std::vector<long long> cell;
+    std::vector<int> obs;
+    std::vector<unsigned long long> first, last;
+    for (size_t v = 0; v < c->vobs.size(); ++v) {
+      const Ctx::VObs &o = c->vobs[v];
+      c->vobsNodeStart.push_back((long long)cell.size());
+      for (int k = o.box[4]; k <= o.box[5]; k += o.step[2])
+        for (int j = o.box[2]; j <= o.box[3]; j += o.step[1])
+          for (int i = o.box[0]; i <= o.box[1]; i += o.step[0]) {
+            cell.push_back(c->lin(i, j, k));
+            obs.push_back((int)v);
+          }
+      first.push_back(o.first);
+      last.push_back(o.last);
+    }
+    c->vobsNodeStart.push_back((long long)cell.size());
+    c->vnodes = (long long)cell.size();
+    c->d_vcell = upload_vec(c, cell); c->d_vobs = upload_vec(c, obs);
+    c->d_vfirst = upload_vec(c, first); c->d_vlast = upload_vec(c, last);
+    c->d_vstage = dev_alloc(c, (size_t)c->vnodes * 6 * std::max<uint64_t>(c->maxRun, 1) * sizeof(float));
+    if (!c->d_vstage) return set_error(c, VGPU_ENOMEM, "volume observer staging");
+  }
+  return VGPU_OK;
+}
+
+int vgpu_finalize(vgpu_ctx *c, uint64_t total_steps, uint64_t max_steps_per_run)
+{
+  if (!c || c->finalized) return VGPU_ESTATE;
+  if (!c->media_set) return set_error(c, VGPU_ESTATE, "vgpu_set_media was not called");
+  c->totalSteps = total_steps;
+  c->maxRun = max_steps_per_run ? max_steps_per_run : total_steps;
+  int r;
+  if ((r = build_debye(c)) || (r = build_sibc(c)) || (r = build_corrections(c)) || (r = build_observers(c))) return r;
+  if (hipDeviceSynchronize() != hipSuccess) return set_error(c, VGPU_EDEVICE, "device error during set-up");
+  c->finalized = true;
+  return VGPU_OK;
+}
+
+int vgpu_run(vgpu_ctx *c, uint64_t first, uint64_t nsteps, void *stream)
+{
+  if (!c || !c->finalized) return VGPU_ESTATE;
+  if (nsteps > c->maxRun && c->vnodes) return set_error(c, VGPU_EINVAL, "run longer than the observer staging");
+  if (!c->pobs.empty() && first + nsteps > c->totalSteps) return set_error(c, VGPU_EINVAL, "run past total steps");
+  hipSetDevice(c->device);
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  hipEventRecord(c->ev0, s);
+  int r = c->prec == VGPU_FP64 ? run_steps<double>(c, first, nsteps, s) : run_steps<float>(c, first, nsteps, s);
+  hipEventRecord(c->ev1, s);
+  c->lastRunFirst = first;
+  c->lastRunSteps = nsteps;
+  return r;
+}
+
+int vgpu_synchronize(vgpu_ctx *c)
+{
+  if (!c) return VGPU_EINVAL;
+  if (hipEventSynchronize(c->ev1) != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
+    return set_error(c, VGPU_EDEVICE, "device error: %s", hipGetErrorString(hipGetLastError()));
+  float ms = 0;
+  hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  c->last_total_ms = ms;
+  double se = 0, sh = 0;
+  int ne = 0;
+  for (size_t q = 0; q + 1 < c->evE.size(); q += 2) {
+    float a = 0, b = 0;
+    if (hipEventElapsedTime(&a, c->evE[q], c->evE[q + 1]) == hipSuccess &&
+        hipEventElapsedTime(&b, c->evH[q], c->evH[q + 1]) == hipSuccess) {
+      se += a; sh += b; ++ne;
+    }
+  }
+  c->last_e_ms = ne ? se / ne : 0;
+  c->last_h_ms = ne ? sh / ne : 0;
+  return VGPU_OK;
+}
+
+int vgpu_read_point_observer(vgpu_ctx *c, int id, double *td, uint64_t nsteps, double *dft_re, double *dft_im)
+{
+  if (!c || !c->finalized || id < 0 || id >= (int)c->pobs.size()) return VGPU_EINVAL;
+  hipStreamSynchronize(c->stream);
+  if (td) {
+    const uint64_t n = std::min<uint64_t>(nsteps, c->totalSteps);
+    hipMemcpy(td, (char *)c->d_otd + (size_t)id * c->totalSteps * 6 * sizeof(double), n * 6 * sizeof(double),
+              hipMemcpyDeviceToHost);
+  }
+  long long off = 0;
+  for (int o = 0; o < id; ++o) off += 6 * (long long)c->pobs[o].omega.size();
+  const size_t nb = 6 * c->pobs[id].omega.size() * sizeof(double);
+  if (nb && dft_re) hipMemcpy(dft_re, (char *)c->d_odre + off * sizeof(double), nb, hipMemcpyDeviceToHost);
+  if (nb && dft_im) hipMemcpy(dft_im, (char *)c->d_odim + off * sizeof(double), nb, hipMemcpyDeviceToHost);
+  return hipGetLastError() == hipSuccess ? VGPU_OK : VGPU_EDEVICE;
+}
+
+int vgpu_read_volume_observer(vgpu_ctx *c, int id, float *out, uint64_t nsteps, uint64_t *nodes)
+{
+  if (!c || !c->finalized || id < 0 || id >= (int)c->vobs.size()) return VGPU_EINVAL;
+  hipStreamSynchronize(c->stream);
+  const long long n0 = c->vobsNodeStart[id], n1 = c->vobsNodeStart[id + 1];
+  if (nodes) *nodes = (uint64_t)(n1 - n0);
+  if (!out) return VGPU_OK;
+  const uint64_t ns = std::min<uint64_t>(nsteps, c->lastRunSteps);
+  for (uint64_t s = 0; s < ns; ++s)
+    hipMemcpy(out + s * (n1 - n0) * 6, (float *)c->d_vstage + ((size_t)s * c->vnodes + n0) * 6,
+              (size_t)(n1 - n0) * 6 * sizeof(float), hipMemcpyDeviceToHost);
+  return VGPU_OK;
+}
+
+int vgpu_download_field(vgpu_ctx *c, int component, double *host)
+{
+  if (!c || component < 0 || component > 5 || !host) return VGPU_EINVAL;
+  hipStreamSynchronize(c->stream);
+  std::vector<unsigned char> buf(c->ncell * c->esz);
+  if (hipMemcpy(buf.data(), c->F[component], buf.size(), hipMemcpyDeviceToHost) != hipSuccess)
+    return set_error(c, VGPU_EDEVICE, "download failed");
+  for (int i = 0; i < c->NX; ++i)
+    for (int j = 0; j < c->NY; ++j)
+      for (int k = 0; k < c->NZ; ++k) {
+        const size_t d = ((size_t)k * c->NY + j) * c->PX + i;
+        host[((size_t)i * c->NY + j) * c->NZ + k] =
+            c->esz == 8 ? ((double *)buf.data())[d] : (double)((float *)buf.data())[d];
+      }
+  return VGPU_OK;
+}
+
+size_t vgpu_footprint_bytes(const vgpu_ctx *c) { return c ? c->bytes : 0; }
+
+int vgpu_last_run_timing(const vgpu_ctx *c, double *total_ms, double *volume_e_ms, double *volume_h_ms)
+{
+  if (!c) return VGPU_EINVAL;
+  if (total_ms) *total_ms = c->last_total_ms;
+  if (volume_e_ms) *volume_e_ms = c->last_e_ms;
+  if (volume_h_ms) *volume_h_ms = c->last_h_ms;
+  return VGPU_OK;
+}
+
+const char *vgpu_last_error(const vgpu_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+void vgpu_destroy(vgpu_ctx *c)
+{
+  if (!c) return;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (void *p : c->allocs) hipFree(p);
+  for (auto e : c->evE) hipEventDestroy(e);
+  for (auto e : c->evH) hipEventDestroy(e);
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+}  // extern "C"

@@ -1,0 +1,326 @@
+// Internal declarations of the MI355X FDTD core library (not part of the C-ABI).
+//
+// Device layout: every 3-D array is x-fastest, index (k*NY + j)*PX + i, with
+// PX the x extent rounded up to 128 bytes so rows start on cache-line
+// boundaries.  Extents NX,NY,NZ are the reference's allocation extents
+// (gobox plus one ghost layer each side), so grid indices i,j,k are the
+// reference's own.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <string>
+#include <vector>
+#include "vgpu.h"
+
+namespace vgpu {
+
+// ---- kernel argument blocks (passed by value) ----
+
+template <typename T> struct EArgs {
+  int NX, NY, NZ, PX;
+  long long plane;
+  int k0, kchunk;
+  T *E[3];
+  const T *H[3];
+  const T *alpha[3], *beta[3];
+  const T *idh[3];
+  int in[3][6];          // inner-grid limits per E component
+  unsigned pmlmask;
+  int pl[6][3][6];       // UPML slab limits [region][component][face]
+  T *PP[6][3], *P[6][3]; // UPML auxiliaries, x-fastest within each slab box
+  const T *ad[3], *bd[3], *ah[3], *ibh[3];
+};
+
+template <typename T> struct HArgs {
+  int NX, NY, NZ, PX;
+  long long plane;
+  int k0, kchunk;
+  T *H[3];
+  const T *E[3];
+  const T *gamma[3];
+  const T *ide[3];
+  int in[3][3][6];       // ordinary-update boxes per H component (inner grid, two Mur faces)
+  unsigned pmlmask;
+  int pl[6][3][6];
+  T *B[6][3];
+  const T *ad[3], *ah[3], *bh[3], *ibd[3];
+};
+
+// Mur tangential-E faces (mur.c:135-395): one box per (face, component).
+template <typename T> struct MurArgs {
+  int NX, NY, PX;
+  long long plane;
+  T *E[3];
+  const T *H[3];
+  const T *alpha[3], *beta[3];
+  const T *idh[3];
+  int box[12][6];        // limits of face f component slot s at [2f+s]
+  int comp[12];          // component of each slot (-1 unused)
+  T zeta[6];
+};
+
+// Debye sites of one layer (debye.c:243-307): a layer holds at most one site
+// per E cell, layers run in block order.
+template <typename T> struct DebyeArgs {
+  long long n;           // sites in this layer
+  const int *comp;       // component of each site
+  const long long *cell; // device linear index
+  const int *medium;     // Debye medium slot of each site
+  T *E[3];
+  const T *beta[3];
+  T *lastE;              // per site
+  T *J;                  // [pole][site] (real poles) or [pole][site][2]
+  long long stride;      // sites in the whole state block (J pole stride)
+  long long base;        // first site of this layer in the state block
+  const double *dalpha, *dbeta;  // [medium][maxPoles][2]
+  const int *npoles;     // [medium]
+  int maxPoles;
+  int cplx;
+};
+
+// SIBC faces (sibc.c:443-586): per-face geometry, per-surface constants.
+template <typename T> struct SibcEArgs {
+  long long nfaces;
+  const int *ijk;        // [face][3]
+  const int *surf;       // [face]
+  const unsigned char *adjA, *adjB;  // [face][4]
+  const int *normal;     // [surface]
+  const double *A, *B;   // [surface][16]
+  const int *filtset;    // [surface] -> filter set
+  const int *npoles;     // [set][16]
+  const double *asymp;   // [set][16]
+  const double *q;       // [set][16][maxPoles][3][2]
+  int maxPoles;
+  double *zeta;          // [(filter*maxPoles + pole)][face][2]
+  double *old;           // [filter][face]
+  T *Etan;               // [face][4]
+  T *E[3];
+  const T *H[3];
+  int NY, PX;
+  long long plane;
+};
+
+// Gathered SIBC H corrections (sibc.c:616-776): one row per touched H cell,
+// entries in the reference's application order.
+struct SibcHEntry {
+  int face;
+  int kind;              // 0 edge term, 1 parallel-adjacent term
+  int ea, eb;            // Etan slots
+  double sign;           // +1 / -1
+  double w;              // edge weight 0.5*(1 + PMC edge)
+  double ida, idb;       // inverse lengths
+  double na, nb;         // !isAdjA, !isAdjB (adjacent term)
+};
+
+template <typename T> struct SibcHArgs {
+  long long nrows;
+  const int *rowcomp;
+  const long long *rowcell;
+  const int *rowstart;   // [nrows+1]
+  const int *rowzero;    // 1: normal H of a SIBC face, set to zero
+  const SibcHEntry *ent;
+  const T *Etan;
+  T *H[3];
+  const T *gamma[3];
+};
+
+// Sources and TF/SF corrections applied after the volume update, one row
+// per touched cell, entries in reference order (sources first, then plane
+// waves face by face, planewave.c:576-1007, source.c:602-753).
+struct CorrEntry {
+  int kind;              // 0 field source, 1 density source, 2 TF/SF aux grid, 3 TF/SF analytic
+  int idx;               // source or plane-wave index
+  int inc;               // incident field component (TF/SF)
+  int auxid;             // floor(d) (aux grid)
+  double sign;           // TF/SF sign
+  double il;             // inverse edge length (TF/SF)
+  double geo;            // dp (aux grid) or d/v (analytic)
+};
+
+struct WaveformDev {
+  int type;
+  double size, delay, width, frequency;
+  const double *table;   // [n][3]
+  unsigned long n;
+};
+
+struct SourceDev {
+  int kind, soft, waveform;
+  double size, delay;
+};
+
+struct PwDev {
+  double Finc[6];
+  int waveform;
+  double delay;
+  int use_aux, nx, xb;
+  double beta, gamma;
+};
+
+template <typename T> struct CorrArgs {
+  long long nrows;
+  const int *rowcomp;
+  const long long *rowcell;
+  const int *rowstart;
+  const CorrEntry *ent;
+  T *F[3];               // E or H components
+  const T *alpha[3];     // E only
+  const T *coef[3];      // beta (E) or gamma (H)
+  int electric;
+  double t;              // timeE or timeH
+  const WaveformDev *wf;
+  const SourceDev *src;
+  const PwDev *pw;
+  const T *aux;          // aux grids [pw][2][auxmax]: Eyi then Hzi
+  int auxmax;
+  double eta0;
+};
+
+template <typename T> struct AuxArgs {
+  int npw;
+  const PwDev *pw;
+  T *aux;                // [pw][2][auxmax]
+  T *pml;                // [pw][3][10]: PPyi, Pyi, Bzi
+  const T *prof;         // [pw][4][10]: adx, bdx, ahx, bhx
+  int auxmax;
+  const WaveformDev *wf;
+  double t;
+};
+
+// Point observers and DFT (observer.c:622-650, 857-899).
+template <typename T> struct ObsArgs {
+  int nobs;
+  const long long *cell; // [obs]
+  const unsigned long long *first, *last;
+  const int *nfreq;
+  const long long *dftoff;   // offset of the observer's [6][nfreq] block
+  const double *omega;       // concatenated per observer
+  double *td;                // [obs][nsteps][6]
+  unsigned long long tdsteps;
+  double *dre, *dim;
+  const T *F[6];
+  unsigned long long n;
+  double t;
+};
+
+template <typename T> struct VolObsArgs {
+  const long long *nodecell; // [node] for all volume observers
+  const int *nodeobs;
+  long long nnodes;
+  const unsigned long long *first, *last;
+  float *stage;              // [run step][node][6]
+  unsigned long long slot;
+  const T *F[6];
+  unsigned long long n;
+};
+
+template <typename T> struct GhostArgs {
+  int NX, NY, NZ, PX;
+  long long plane;
+  T *F;
+  int axis, dst, src;
+  T sign;
+  int lo[3], hi[3];
+};
+
+// ---- context ----
+
+struct Region { int lim[6]; };
+
+struct Ctx {
+  std::string err;
+  vgpu_prec prec;
+  int device;
+  size_t esz;            // bytes per stored real
+  int NX, NY, NZ, PX;
+  long long plane;
+  size_t ncell;          // PX*NY*NZ
+  vgpu_grid_desc grid;
+  std::vector<double> xl, yl, zl, dex, dey, dez, idex, idey, idez, idhx, idhy, idhz;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<hipEvent_t> evE, evH;
+  double last_total_ms = 0, last_e_ms = 0, last_h_ms = 0;
+  bool finalized = false;
+  size_t bytes = 0;
+
+  // device allocations
+  std::vector<void *> allocs;
+  void *F[6] = {};
+  void *alpha[3] = {}, *beta[3] = {}, *gamma[3] = {};
+  void *ide[3] = {}, *idh[3] = {};
+  // host copies kept for building surface lists
+  std::vector<double> h_beta[3], h_gamma[3], h_alpha[3];
+  bool media_set = false;
+
+  // UPML
+  unsigned pmlmask = 0;
+  int fplim[6][6][6];
+  void *aux1[6][3] = {}, *aux2[6][3] = {}, *auxB[6][3] = {};
+  void *prof[6][3] = {};   // ad, bd, ah, bh, ibd, ibh per axis
+
+  // Mur
+  unsigned murmask = 0;
+  double zeta[6] = {};
+  int fmlim[6][6][6];
+
+  // ghosts
+  int ghost[6] = {};
+
+  // Debye blocks (host side until finalize)
+  struct DebyeBlock { int flim[3][6]; int np; std::vector<double> da, db; };
+  std::vector<DebyeBlock> debye;
+  int debyeLayers = 0;
+  std::vector<long long> debyeLayerStart;
+  long long debyeSites = 0;
+  int debyeMaxPoles = 0, debyeCplx = 0;
+  void *d_dcomp = nullptr, *d_dcell = nullptr, *d_dmed = nullptr, *d_dlast = nullptr, *d_dJ = nullptr;
+  void *d_dalpha = nullptr, *d_dbeta = nullptr, *d_dnp = nullptr;
+
+  // SIBC
+  struct SibcSurf { vgpu_sibc_desc d; std::vector<unsigned char> adjA, adjB;
+                    std::vector<double> q0[16], q1[16], q2[16]; };
+  std::vector<SibcSurf> sibc;
+  long long sibcFaces = 0;
+  int sibcMaxPoles = 1;
+  void *d_sijk = nullptr, *d_ssurf = nullptr, *d_sadjA = nullptr, *d_sadjB = nullptr, *d_snormal = nullptr;
+  void *d_sA = nullptr, *d_sB = nullptr, *d_sset = nullptr, *d_snp = nullptr, *d_sasymp = nullptr, *d_sq = nullptr;
+  void *d_szeta = nullptr, *d_sold = nullptr, *d_setan = nullptr;
+  long long sibcHRows = 0;
+  void *d_shcomp = nullptr, *d_shcell = nullptr, *d_shstart = nullptr, *d_shzero = nullptr, *d_shent = nullptr;
+
+  // waveforms, sources, plane waves
+  std::vector<WaveformDev> wf;
+  std::vector<std::vector<double>> wftab;
+  struct Src { int field, kind, soft, waveform; double size, delay; int flim[6]; };
+  std::vector<Src> src;
+  struct Pw { vgpu_pw_desc d; double prof[4][10]; };
+  std::vector<Pw> pw;
+  void *d_wf = nullptr, *d_src = nullptr, *d_pw = nullptr, *d_aux = nullptr, *d_auxpml = nullptr, *d_auxprof = nullptr;
+  int auxmax = 0;
+  long long corrRows[2] = {0, 0};
+  void *d_ccomp[2] = {}, *d_ccell[2] = {}, *d_cstart[2] = {}, *d_cent[2] = {};
+
+  // observers
+  struct PObs { int i, j, k; uint64_t first, last; std::vector<double> omega; };
+  std::vector<PObs> pobs;
+  struct VObs { int box[6]; int step[3]; uint64_t first, last; long long nodes; };
+  std::vector<VObs> vobs;
+  uint64_t totalSteps = 0, maxRun = 0, lastRunSteps = 0, lastRunFirst = 0;
+  void *d_ocell = nullptr, *d_ofirst = nullptr, *d_olast = nullptr, *d_onf = nullptr, *d_ooff = nullptr;
+  void *d_oomega = nullptr, *d_otd = nullptr, *d_odre = nullptr, *d_odim = nullptr;
+  long long vnodes = 0;
+  void *d_vcell = nullptr, *d_vobs = nullptr, *d_vfirst = nullptr, *d_vlast = nullptr, *d_vstage = nullptr;
+  std::vector<long long> vobsNodeStart;
+
+  long long lin(int i, int j, int k) const { return ((long long)k * NY + j) * PX + i; }
+};
+
+int set_error(Ctx *c, int code, const char *fmt, ...);
+void *dev_alloc(Ctx *c, size_t bytes);
+
+// Per-precision step driver, instantiated for double and float.
+template <typename T> int run_steps(Ctx *c, uint64_t first, uint64_t nsteps, hipStream_t s);
+
+}  // namespace vgpu

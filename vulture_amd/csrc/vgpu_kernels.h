@@ -1,0 +1,624 @@
+// Kernel templates of the FDTD core, instantiated for double (compiled with
+// -ffp-contract=off so every update performs the reference's operations in
+// the reference's order) and for float (contraction allowed).
+//
+// Volume kernels: a 64x4 thread tile covers 64 x-cells by 4 y-rows and
+// marches along z over a chunk of planes, keeping the previous (E update) or
+// next (H update) plane's values in registers, so each field word is read
+// from HBM once per sweep; x- and y-neighbours come from the same tile's
+// loads through L1/L2.  Every cell of the allocation is visited once and
+// classified per component into the inner grid, a UPML slab or nothing.
+#pragma once
+#include "vgpu_internal.h"
+
+namespace vgpu {
+
+#define VG_DEV __device__ __forceinline__
+
+VG_DEV bool in_box(const int *b, int i, int j, int k)
+{
+  return i >= b[0] && i <= b[1] && j >= b[2] && j <= b[3] && k >= b[4] && k <= b[5];
+}
+
+template <typename T> VG_DEV long long slab_index(const int *b, int i, int j, int k)
+{
+  long long ni = b[1] - b[0] + 1, nj = b[3] - b[2] + 1;
+  return ((long long)(k - b[4]) * nj + (j - b[2])) * ni + (i - b[0]);
+}
+
+// ---------------------------------------------------------------------------
+// E volume sweep: inner grid (grid.c:709-797) and UPML slabs (pml.c:859-1023)
+// ---------------------------------------------------------------------------
+template <typename T> __global__ __launch_bounds__(256) void k_volume_e(EArgs<T> a)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= a.NX || j >= a.NY) return;
+  const int kb = a.k0 + blockIdx.z * a.kchunk;
+  const int ke = min(kb + a.kchunk, a.NZ);
+  if (kb >= ke) return;
+  const long long P = a.plane, SJ = a.PX;
+  const T *__restrict__ Hx = a.H[0];
+  const T *__restrict__ Hy = a.H[1];
+  const T *__restrict__ Hz = a.H[2];
+  const T idhx = a.idh[0][i], idhy = a.idh[1][j];
+  long long id = ((long long)kb * a.NY + j) * SJ + i;
+  // H at the plane below, carried between iterations.
+  T hxm = (kb > 0) ? Hx[id - P] : T(0), hym = (kb > 0) ? Hy[id - P] : T(0);
+  for (int k = kb; k < ke; ++k, id += P) {
+    const T hx = Hx[id], hy = Hy[id], hz = Hz[id];
+    const T idhz = a.idh[2][k];
+    // ---- Ex ----
+    {
+      const int c = 0;
+      int reg = in_box(a.in[c], i, j, k) ? 6 : -1;
+      if (reg < 0 && a.pmlmask)
+        for (int r = 0; r < 6; ++r)
+          if (((a.pmlmask >> r) & 1u) && in_box(a.pl[r][c], i, j, k)) { reg = r; break; }
+      if (reg >= 0) {
+        const T curl = idhy * (hz - Hz[id - SJ]) + idhz * (hym - hy);
+        T *__restrict__ E = a.E[c];
+        if (reg == 6) {
+          E[id] = a.alpha[c][id] * E[id] + a.beta[c][id] * curl;
+        } else {
+          const long long p = slab_index<T>(a.pl[reg][c], i, j, k);
+          T *PP = a.PP[reg][c], *PA = a.P[reg][c];
+          const T oldPP = PP[p];
+          const T pp = a.alpha[c][id] * oldPP + a.beta[c][id] * curl;
+          PP[p] = pp;
+          const T oldP = PA[p];
+          const T pn = a.ad[1][j] * oldP + a.bd[1][j] * (pp - oldPP);
+          PA[p] = pn;
+          E[id] = a.ad[2][k] * E[id] + a.bd[2][k] * a.ibh[0][i] * (pn - a.ah[0][i] * oldP);
+        }
+      }
+    }
+    // ---- Ey ----
+    {
+      const int c = 1;
+      int reg = in_box(a.in[c], i, j, k) ? 6 : -1;
+      if (reg < 0 && a.pmlmask)
+        for (int r = 0; r < 6; ++r)
+          if (((a.pmlmask >> r) & 1u) && in_box(a.pl[r][c], i, j, k)) { reg = r; break; }
+      if (reg >= 0) {
+        const T curl = idhz * (hx - hxm) + idhx * (Hz[id - 1] - hz);
+        T *__restrict__ E = a.E[c];
+        if (reg == 6) {
+          E[id] = a.alpha[c][id] * E[id] + a.beta[c][id] * curl;
+        } else {
+          const long long p = slab_index<T>(a.pl[reg][c], i, j, k);
+          T *PP = a.PP[reg][c], *PA = a.P[reg][c];
+          const T oldPP = PP[p];
+          const T pp = a.alpha[c][id] * oldPP + a.beta[c][id] * curl;
+          PP[p] = pp;
+          const T oldP = PA[p];
+          const T pn = a.ad[2][k] * oldP + a.bd[2][k] * (pp - oldPP);
+          PA[p] = pn;
+          E[id] = a.ad[0][i] * E[id] + a.bd[0][i] * a.ibh[1][j] * (pn - a.ah[1][j] * oldP);
+        }
+      }
+    }
+    // ---- Ez ----
+    {
+      const int c = 2;
+      int reg = in_box(a.in[c], i, j, k) ? 6 : -1;
+      if (reg < 0 && a.pmlmask)
+        for (int r = 0; r < 6; ++r)
+          if (((a.pmlmask >> r) & 1u) && in_box(a.pl[r][c], i, j, k)) { reg = r; break; }
+      if (reg >= 0) {
+        const T curl = idhx * (hy - Hy[id - 1]) + idhy * (Hx[id - SJ] - hx);
+        T *__restrict__ E = a.E[c];
+        if (reg == 6) {
+          E[id] = a.alpha[c][id] * E[id] + a.beta[c][id] * curl;
+        } else {
+          const long long p = slab_index<T>(a.pl[reg][c], i, j, k);
+          T *PP = a.PP[reg][c], *PA = a.P[reg][c];
+          const T oldPP = PP[p];
+          const T pp = a.alpha[c][id] * oldPP + a.beta[c][id] * curl;
+          PP[p] = pp;
+          const T oldP = PA[p];
+          const T pn = a.ad[0][i] * oldP + a.bd[0][i] * (pp - oldPP);
+          PA[p] = pn;
+          E[id] = a.ad[1][j] * E[id] + a.bd[1][j] * a.ibh[2][k] * (pn - a.ah[2][k] * oldP);
+        }
+      }
+    }
+    hxm = hx;
+    hym = hy;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// H volume sweep: inner grid (grid.c:800-888), Mur normal H (mur.c:401-555)
+// and UPML slabs (pml.c:1026-1176)
+// ---------------------------------------------------------------------------
+template <typename T> __global__ __launch_bounds__(256) void k_volume_h(HArgs<T> a)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= a.NX || j >= a.NY) return;
+  const int kb = a.k0 + blockIdx.z * a.kchunk;
+  const int ke = min(kb + a.kchunk, a.NZ);
+  if (kb >= ke) return;
+  const long long P = a.plane, SJ = a.PX;
+  const T *__restrict__ Ex = a.E[0];
+  const T *__restrict__ Ey = a.E[1];
+  const T *__restrict__ Ez = a.E[2];
+  const bool xin = i + 1 < a.NX, yin = j + 1 < a.NY;
+  const T idex = a.ide[0][i], idey = a.ide[1][j];
+  long long id = ((long long)kb * a.NY + j) * SJ + i;
+  T ex = Ex[id], ey = Ey[id];
+  for (int k = kb; k < ke; ++k, id += P) {
+    const bool zin = k + 1 < a.NZ;
+    const T exp1 = zin ? Ex[id + P] : T(0), eyp1 = zin ? Ey[id + P] : T(0);
+    const T ez = Ez[id];
+    const T idez = a.ide[2][k];
+    // ---- Hx ----
+    {
+      const int c = 0;
+      int reg = -1;
+      for (int b = 0; b < 3; ++b)
+        if (in_box(a.in[c][b], i, j, k)) { reg = 6; break; }
+      if (reg < 0 && a.pmlmask)
+        for (int r = 0; r < 6; ++r)
+          if (((a.pmlmask >> r) & 1u) && in_box(a.pl[r][c], i, j, k)) { reg = r; break; }
+      if (reg >= 0) {
+        const T ezp = yin ? Ez[id + SJ] : T(0);
+        const T curl = idez * (eyp1 - ey) + idey * (ez - ezp);
+        T *__restrict__ H = a.H[c];
+        if (reg == 6) {
+          H[id] = H[id] + a.gamma[c][id] * curl;
+        } else {
+          const long long p = slab_index<T>(a.pl[reg][c], i, j, k);
+          T *Bp = a.B[reg][c];
+          const T oldB = Bp[p];
+          const T bn = a.ah[1][j] * oldB + a.gamma[c][id] * a.bh[1][j] * curl;
+          Bp[p] = bn;
+          H[id] = a.ah[2][k] * H[id] + a.bh[2][k] * a.ibd[0][i] * (bn - a.ad[0][i] * oldB);
+        }
+      }
+    }
+    // ---- Hy ----
+    {
+      const int c = 1;
+      int reg = -1;
+      for (int b = 0; b < 3; ++b)
+        if (in_box(a.in[c][b], i, j, k)) { reg = 6; break; }
+      if (reg < 0 && a.pmlmask)
+        for (int r = 0; r < 6; ++r)
+          if (((a.pmlmask >> r) & 1u) && in_box(a.pl[r][c], i, j, k)) { reg = r; break; }
+      if (reg >= 0) {
+        const T ezx = xin ? Ez[id + 1] : T(0);
+        const T curl = idex * (ezx - ez) + idez * (ex - exp1);
+        T *__restrict__ H = a.H[c];
+        if (reg == 6) {
+          H[id] = H[id] + a.gamma[c][id] * curl;
+        } else {
+          const long long p = slab_index<T>(a.pl[reg][c], i, j, k);
+          T *Bp = a.B[reg][c];
+          const T oldB = Bp[p];
+          const T bn = a.ah[2][k] * oldB + a.gamma[c][id] * a.bh[2][k] * curl;
+          Bp[p] = bn;
+          H[id] = a.ah[0][i] * H[id] + a.bh[0][i] * a.ibd[1][j] * (bn - a.ad[1][j] * oldB);
+        }
+      }
+    }
+    // ---- Hz ----
+    {
+      const int c = 2;
+      int reg = -1;
+      for (int b = 0; b < 3; ++b)
+        if (in_box(a.in[c][b], i, j, k)) { reg = 6; break; }
+      if (reg < 0 && a.pmlmask)
+        for (int r = 0; r < 6; ++r)
+          if (((a.pmlmask >> r) & 1u) && in_box(a.pl[r][c], i, j, k)) { reg = r; break; }
+      if (reg >= 0) {
+        const T exy = yin ? Ex[id + SJ] : T(0);
+        const T eyx = xin ? Ey[id + 1] : T(0);
+        const T curl = idey * (exy - ex) + idex * (ey - eyx);
+        T *__restrict__ H = a.H[c];
+        if (reg == 6) {
+          H[id] = H[id] + a.gamma[c][id] * curl;
+        } else {
+          const long long p = slab_index<T>(a.pl[reg][c], i, j, k);
+          T *Bp = a.B[reg][c];
+          const T oldB = Bp[p];
+          const T bn = a.ah[0][i] * oldB + a.gamma[c][id] * a.bh[0][i] * curl;
+          Bp[p] = bn;
+          H[id] = a.ah[1][j] * H[id] + a.bh[1][j] * a.ibd[2][k] * (bn - a.ad[2][k] * oldB);
+        }
+      }
+    }
+    ex = exp1;
+    ey = eyp1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Mur tangential E (mur.c:135-395), before the E sweep.  blockIdx.z selects
+// the (face, component) slot; x/y of the launch span the face.
+// ---------------------------------------------------------------------------
+template <typename T> __global__ void k_mur_e(MurArgs<T> a)
+{
+  const int slot = blockIdx.z;
+  const int c = a.comp[slot];
+  if (c < 0) return;
+  const int face = slot >> 1, axis = face >> 1;
+  const int *b = a.box[slot];
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  const int v = blockIdx.y * blockDim.y + threadIdx.y;
+  // (u,v) run over the two axes other than the normal, lowest axis first.
+  const int ax1 = axis == 0 ? 1 : 0, ax2 = axis == 2 ? 1 : 2;
+  int ijk[3];
+  ijk[axis] = (face & 1) ? b[2 * axis + 1] : b[2 * axis];
+  ijk[ax1] = b[2 * ax1] + u;
+  ijk[ax2] = b[2 * ax2] + v;
+  if (ijk[ax1] > b[2 * ax1 + 1] || ijk[ax2] > b[2 * ax2 + 1]) return;
+  const int step = (face & 1) ? -1 : 1;
+  int n3[3] = {ijk[0], ijk[1], ijk[2]};
+  n3[axis] += step;
+  const long long SJ = a.PX, P = a.plane;
+  const long long id = ((long long)ijk[2] * a.NY + ijk[1]) * SJ + ijk[0];
+  const long long nb = ((long long)n3[2] * a.NY + n3[1]) * SJ + n3[0];
+  const int i = n3[0], j = n3[1], k = n3[2];
+  const T *Hx = a.H[0], *Hy = a.H[1], *Hz = a.H[2];
+  T curl;
+  if (c == 0) curl = a.idh[1][j] * (Hz[nb] - Hz[nb - SJ]) + a.idh[2][k] * (Hy[nb - P] - Hy[nb]);
+  else if (c == 1) curl = a.idh[2][k] * (Hx[nb] - Hx[nb - P]) + a.idh[0][i] * (Hz[nb - 1] - Hz[nb]);
+  else curl = a.idh[0][i] * (Hy[nb] - Hy[nb - 1]) + a.idh[1][j] * (Hx[nb - SJ] - Hx[nb]);
+  T *E = a.E[c];
+  const T tmp = a.alpha[c][nb] * E[nb] + a.beta[c][nb] * curl;
+  E[id] = E[nb] + a.zeta[face] * (tmp - E[id]);
+}
+
+// ---------------------------------------------------------------------------
+// Debye ADE correction of one layer of sites (debye.c:243-307)
+// ---------------------------------------------------------------------------
+template <typename T> __global__ void k_debye(DebyeArgs<T> a)
+{
+  const long long s = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (s >= a.n) return;
+  const long long g = a.base + s;
+  const int c = a.comp[g];
+  const long long id = a.cell[g];
+  const int m = a.medium[g];
+  const int np = a.npoles[m];
+  const double *da = a.dalpha + (long long)m * a.maxPoles * 2;
+  const double *db = a.dbeta + (long long)m * a.maxPoles * 2;
+  T *E = a.E[c];
+  T e = E[id];
+  const T beta = a.beta[c][id];
+  if (!a.cplx) {
+    T jsum = T(0);
+    for (int p = 0; p < np; ++p) jsum = jsum + (T(1) + T(da[2 * p])) * a.J[p * a.stride + g];
+    e = e - beta * jsum;
+    const T de = e - a.lastE[g];
+    for (int p = 0; p < np; ++p) {
+      T *Jp = &a.J[p * a.stride + g];
+      *Jp = T(da[2 * p]) * *Jp + T(db[2 * p]) * de;
+    }
+  } else {
+    T sr = T(0), si = T(0);
+    for (int p = 0; p < np; ++p) {
+      const T ar = T(1) + T(da[2 * p]), ai = T(da[2 * p + 1]);
+      const T jr = a.J[(p * a.stride + g) * 2], ji = a.J[(p * a.stride + g) * 2 + 1];
+      sr = sr + (ar * jr - ai * ji);
+      si = si + (ar * ji + ai * jr);
+    }
+    (void)si;
+    e = e - beta * sr;
+    const T de = e - a.lastE[g];
+    for (int p = 0; p < np; ++p) {
+      T *Jp = &a.J[(p * a.stride + g) * 2];
+      const T jr = Jp[0], ji = Jp[1];
+      const T dar = T(da[2 * p]), dai = T(da[2 * p + 1]), dbr = T(db[2 * p]), dbi = T(db[2 * p + 1]);
+      Jp[0] = (dar * jr - dai * ji) + dbr * de;
+      Jp[1] = (dar * ji + dai * jr) + dbi * de;
+    }
+  }
+  E[id] = e;
+  a.lastE[g] = e;
+}
+
+// ---------------------------------------------------------------------------
+// SIBC: E pass per face (sibc.c:443-586) with FP64 recursive convolution
+// (filter.c:309-327).
+// ---------------------------------------------------------------------------
+template <typename T> __global__ void k_sibc_e(SibcEArgs<T> a)
+{
+  const long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (f >= a.nfaces) return;
+  const int i = a.ijk[3 * f], j = a.ijk[3 * f + 1], k = a.ijk[3 * f + 2];
+  const int s = a.surf[f];
+  const int nrm = a.normal[s];
+  const long long SJ = a.PX, P = a.plane;
+  const long long id = ((long long)k * a.NY + j) * SJ + i;
+  const unsigned char *A = a.adjA + 4 * f, *B = a.adjB + 4 * f;
+  T Htan[4];
+  const T *Hx = a.H[0], *Hy = a.H[1], *Hz = a.H[2];
+  T *Ex = a.E[0], *Ey = a.E[1], *Ez = a.E[2];
+  if (nrm == 0) {
+    Ey[id] = T(0); Ey[id + P] = T(0); Ez[id] = T(0); Ez[id + SJ] = T(0);
+    Htan[0] = T(0.5 * (1 + A[0] + A[1]) * (double)(Hy[id - 1] + Hy[id - 1 + SJ]));
+    Htan[1] = T(0.5 * (1 + B[0] + B[1]) * (double)(Hy[id] + Hy[id + SJ]));
+    Htan[2] = T(0.5 * (1 + A[2] + A[3]) * (double)(Hz[id - 1] + Hz[id - 1 + P]));
+    Htan[3] = T(0.5 * (1 + B[2] + B[3]) * (double)(Hz[id] + Hz[id + P]));
+  } else if (nrm == 1) {
+    Ez[id] = T(0); Ez[id + 1] = T(0); Ex[id] = T(0); Ex[id + P] = T(0);
+    Htan[0] = T(0.5 * (1 + A[0] + A[1]) * (double)(Hz[id - SJ] + Hz[id - SJ + P]));
+    Htan[1] = T(0.5 * (1 + B[0] + B[1]) * (double)(Hz[id] + Hz[id + P]));
+    Htan[2] = T(0.5 * (1 + A[2] + A[3]) * (double)(Hx[id - SJ] + Hx[id - SJ + 1]));
+    Htan[3] = T(0.5 * (1 + B[2] + B[3]) * (double)(Hx[id] + Hx[id + 1]));
+  } else {
+    Ex[id] = T(0); Ex[id + SJ] = T(0); Ey[id] = T(0); Ey[id + 1] = T(0);
+    Htan[0] = T(0.5 * (1 + A[0] + A[1]) * (double)(Hx[id - P] + Hx[id - P + 1]));
+    Htan[1] = T(0.5 * (1 + B[0] + B[1]) * (double)(Hx[id] + Hx[id + 1]));
+    Htan[2] = T(0.5 * (1 + A[2] + A[3]) * (double)(Hy[id - P] + Hy[id - P + SJ]));
+    Htan[3] = T(0.5 * (1 + B[2] + B[3]) * (double)(Hy[id] + Hy[id + SJ]));
+  }
+  const double *Am = a.A + 16 * s, *Bm = a.B + 16 * s;
+  T Hin[4], Eout[4];
+  for (int p = 0; p < 4; ++p) {
+    Hin[p] = T(0);
+    for (int q = 0; q < 4; ++q) Hin[p] = Hin[p] + T(Am[4 * p + q]) * Htan[q];
+  }
+  const int set = a.filtset[s];
+  const long long NF = a.nfaces;
+  for (int p = 0; p < 4; ++p) {
+    Eout[p] = T(0);
+    for (int q = 0; q < 4; ++q) {
+      const int flt = 4 * p + q;
+      const int np = a.npoles[16 * set + flt];
+      const double asy = a.asymp[16 * set + flt];
+      if (np == 0 && asy == 0.0) continue;
+      const double x = (double)Hin[q];
+      double y = asy * x;
+      if (np > 0) {
+        double *old = &a.old[flt * NF + f];
+        const double xo = *old;
+        const double *qq = a.q + ((long long)(16 * set + flt) * a.maxPoles) * 6;
+        for (int kp = 0; kp < np; ++kp) {
+          double *z = &a.zeta[(((long long)flt * a.maxPoles + kp) * NF + f) * 2];
+          const double q0r = qq[6 * kp], q0i = qq[6 * kp + 1], q1r = qq[6 * kp + 2], q1i = qq[6 * kp + 3];
+          const double q2r = qq[6 * kp + 4], q2i = qq[6 * kp + 5];
+          const double zr = z[0], zi = z[1];
+          const double nr = ((q0r * zr - q0i * zi) + q1r * xo) + q2r * x;
+          const double ni = ((q0r * zi + q0i * zr) + q1i * xo) + q2i * x;
+          z[0] = nr;
+          z[1] = ni;
+          y = y + nr;
+        }
+        *old = x;
+      }
+      Eout[p] = T((double)Eout[p] + y);
+    }
+  }
+  T *Et = a.Etan + 4 * f;
+  for (int p = 0; p < 4; ++p) {
+    T v = T(0);
+    for (int q = 0; q < 4; ++q) v = v + T(Bm[4 * p + q]) * Eout[q];
+    Et[p] = v;
+  }
+}
+
+// SIBC: gathered H corrections (sibc.c:616-776).
+template <typename T> __global__ void k_sibc_h(SibcHArgs<T> a)
+{
+  const long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (r >= a.nrows) return;
+  T *H = a.H[a.rowcomp[r]];
+  const long long id = a.rowcell[r];
+  if (a.rowzero[r]) {
+    H[id] = T(0);
+    return;
+  }
+  T h = H[id];
+  const T g = a.gamma[a.rowcomp[r]][id];
+  for (int e = a.rowstart[r]; e < a.rowstart[r + 1]; ++e) {
+    const SibcHEntry &en = a.ent[e];
+    const T *Et = a.Etan + 4 * (long long)en.face;
+    T c;
+    if (en.kind == 0) c = (g * T(en.w)) * (T(en.ida) * Et[en.ea]);
+    else c = ((g * T(0.5)) * T(en.w)) * (T(en.na) * (T(en.ida) * Et[en.ea]) + T(en.nb) * (T(en.idb) * Et[en.eb]));
+    h = h + T(en.sign) * c;
+  }
+  H[id] = h;
+}
+
+// ---------------------------------------------------------------------------
+// Waveforms (waveform.c:413-638) on the device, in double.
+// ---------------------------------------------------------------------------
+VG_DEV double wf_compact(double pi, double time, double width, double span)
+{
+  if (time <= 0.0) return 0.0;
+  if (time < span * width)
+    return 1.0 / 32.0 * (10.0 - 15.0 * cos(pi / width * time) + 6.0 * cos(2.0 * pi / width * time) -
+                         cos(3.0 * pi / width * time));
+  return span < 1.5 ? 1.0 : 0.0;
+}
+
+VG_DEV double wf_value(const WaveformDev &w, double t, double delay, double pi)
+{
+  const double tt = t - delay - w.delay;
+  const double width = w.width;
+  switch (w.type) {
+  case 0:
+  case 1: return w.size * exp(-0.5 * (pow(tt / width, 2.0)));
+  case 2: return w.size * (-tt / width * exp(-0.5 * (pow(tt / width, 2.0))));
+  case 3: return w.size * ((1.0 - pow(tt / width, 2)) * exp(-0.5 * (pow(tt / width, 2.0))));
+  case 4: return w.size * (exp(-0.5 * (pow(tt / width, 2.0))) * sin(2.0 * pi * w.frequency * tt));
+  case 5: return w.size * wf_compact(pi, tt, width, 2.0);
+  case 6:
+    if (tt <= 0.0 || tt >= 2.0 * width) return 0.0;
+    return w.size * (1.0 / 32.0 * (15.0 * sin(pi / width * tt) - 12.0 * sin(2.0 * pi / width * tt) +
+                                   3.0 * sin(3.0 * pi / width * tt)));
+  case 7: return w.size * (wf_compact(pi, tt, width, 2.0) * sin(2.0 * pi * w.frequency * tt));
+  case 8: return w.size * (wf_compact(pi, tt, width, 1.0) * sin(2.0 * pi * w.frequency * tt));
+  case 9: {
+    const double *T3 = w.table;
+    const unsigned long n = w.n;
+    if (n < 2 || tt < T3[0] || tt > T3[3 * (n - 1)]) return 0.0;
+    unsigned long lo = 0, hi = n - 1;
+    while (hi - lo > 1) {
+      const unsigned long m = (hi + lo) >> 1;
+      if (T3[3 * m] > tt) hi = m;
+      else lo = m;
+    }
+    const double h = T3[3 * hi] - T3[3 * lo];
+    const double aa = (T3[3 * hi] - tt) / h, bb = (tt - T3[3 * lo]) / h;
+    return w.size * (aa * T3[3 * lo + 1] + bb * T3[3 * hi + 1] +
+                     ((aa * aa * aa - aa) * T3[3 * lo + 2] + (bb * bb * bb - bb) * T3[3 * hi + 2]) * (h * h) / 6.0);
+  }
+  default: return 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 1-D auxiliary grids of the TF/SF plane waves (planewave.c:1315-1372).
+// One block per plane wave.
+// ---------------------------------------------------------------------------
+template <typename T> __global__ void k_aux_e(AuxArgs<T> a, double pi)
+{
+  const int p = blockIdx.x;
+  const PwDev &pw = a.pw[p];
+  if (!pw.use_aux) return;
+  T *Eyi = a.aux + (long long)p * 2 * a.auxmax, *Hzi = Eyi + a.auxmax;
+  T *PPy = a.pml + p * 30, *Py = PPy + 10;
+  const T *adx = a.prof + p * 40, *bdx = adx + 10;
+  const T beta = T(pw.beta);
+  for (int i = 1 + threadIdx.x; i < pw.nx; i += blockDim.x) {
+    if (i < pw.xb) {
+      Eyi[i] = Eyi[i] + beta * (Hzi[i - 1] - Hzi[i]);
+    } else {
+      const int L = i - pw.xb;
+      const T oldPP = PPy[L];
+      PPy[L] = PPy[L] + beta * (Hzi[i - 1] - Hzi[i]);
+      const T oldP = Py[L];
+      Py[L] = Py[L] + (PPy[L] - oldPP);
+      Eyi[i] = adx[L] * Eyi[i] + bdx[L] * (Py[L] - oldP);
+    }
+  }
+  if (threadIdx.x == 0) Eyi[0] = T(wf_value(a.wf[pw.waveform], a.t, pw.delay, pi));
+}
+
+template <typename T> __global__ void k_aux_h(AuxArgs<T> a)
+{
+  const int p = blockIdx.x;
+  const PwDev &pw = a.pw[p];
+  if (!pw.use_aux) return;
+  T *Eyi = a.aux + (long long)p * 2 * a.auxmax, *Hzi = Eyi + a.auxmax;
+  T *Bz = a.pml + p * 30 + 20;
+  const T *ahx = a.prof + p * 40 + 20, *bhx = ahx + 10;
+  const T gamma = T(pw.gamma);
+  for (int i = threadIdx.x; i < pw.nx; i += blockDim.x) {
+    if (i < pw.xb) {
+      Hzi[i] = Hzi[i] + gamma * (Eyi[i] - Eyi[i + 1]);
+    } else {
+      const int L = i - pw.xb;
+      const T oldB = Bz[L];
+      Bz[L] = ahx[L] * Bz[L] + gamma * bhx[L] * (-Eyi[i + 1] + Eyi[i]);
+      Hzi[i] = Hzi[i] + (Bz[L] - oldB);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Sources and TF/SF corrections, one thread per touched cell, entries in
+// reference order (source.c:602-753, planewave.c:576-1007).
+// ---------------------------------------------------------------------------
+template <typename T> __global__ void k_corr(CorrArgs<T> a, double pi)
+{
+  const long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (r >= a.nrows) return;
+  const int c = a.rowcomp[r];
+  const long long id = a.rowcell[r];
+  T *F = a.F[c];
+  T f = F[id];
+  const T coef = a.coef[c][id];
+  const T alpha = a.electric ? a.alpha[c][id] : T(1);
+  for (int e = a.rowstart[r]; e < a.rowstart[r + 1]; ++e) {
+    const CorrEntry &en = a.ent[e];
+    if (en.kind <= 1) {
+      const SourceDev &s = a.src[en.idx];
+      const T source = T(s.size * wf_value(a.wf[s.waveform], a.t, s.delay, pi));
+      if (en.kind == 0) f = T(s.soft) * f + source;
+      else f = T(s.soft) * f - coef * source;
+    } else {
+      const PwDev &pw = a.pw[en.idx];
+      T inc;
+      if (en.kind == 2) {
+        const T dp = T(en.geo);
+        const T *Eyi = a.aux + (long long)en.idx * 2 * a.auxmax, *Hzi = Eyi + a.auxmax;
+        T value;
+        if (en.inc < 3) value = (T(1) - dp) * Eyi[2 + en.auxid] + dp * Eyi[2 + en.auxid + 1];
+        else value = T(a.eta0) * ((T(1) - dp) * Hzi[1 + en.auxid] + dp * Hzi[2 + en.auxid]);
+        inc = T(pw.Finc[en.inc]) * value;
+      } else {
+        inc = T(pw.Finc[en.inc]) * T(wf_value(a.wf[pw.waveform], a.t - en.geo, pw.delay, pi));
+      }
+      const T term = coef * (T(en.il) * inc);
+      if (a.electric) f = alpha * f + T(en.sign) * term;
+      else f = f + T(en.sign) * term;
+    }
+  }
+  F[id] = f;
+}
+
+// ---------------------------------------------------------------------------
+// Observers: point samples + FP64 running DFT (observer.c:760-899) and volume
+// samples for impulse.dat (observer.c:1164-1198).  One thread per
+// (observer, frequency); thread f==0 also stores the time sample.
+// ---------------------------------------------------------------------------
+template <typename T> __global__ void k_observe(ObsArgs<T> a)
+{
+  const int o = blockIdx.y;
+  if (o >= a.nobs) return;
+  if (a.n < a.first[o] || a.n > a.last[o]) return;
+  const long long id = a.cell[o];
+  double v[6];
+  for (int c = 0; c < 6; ++c) v[c] = (double)a.F[c][id];
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f == 0) {
+    double *td = a.td + ((long long)o * a.tdsteps + (a.n - a.first[o])) * 6;
+    for (int c = 0; c < 6; ++c) td[c] = v[c];
+  }
+  const int nf = a.nfreq[o];
+  if (f >= nf) return;
+  const double ot = a.omega[a.dftoff[o] / 6 + f] * a.t;
+  const double co = cos(ot), si = sin(ot);
+  double *re = a.dre + a.dftoff[o], *im = a.dim + a.dftoff[o];
+  for (int c = 0; c < 6; ++c) {
+    re[c * nf + f] += v[c] * co;
+    im[c * nf + f] -= v[c] * si;
+  }
+}
+
+template <typename T> __global__ void k_observe_volume(VolObsArgs<T> a)
+{
+  const long long nd = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (nd >= a.nnodes) return;
+  const int o = a.nodeobs[nd];
+  if (a.n < a.first[o] || a.n > a.last[o]) return;
+  const long long id = a.nodecell[nd];
+  float *out = a.stage + ((long long)a.slot * a.nnodes + nd) * 6;
+  for (int c = 0; c < 6; ++c) out[c] = (float)a.F[c][id];
+}
+
+// Ghost plane copy for one face (surface.c:569-812).
+template <typename T> __global__ void k_ghost(GhostArgs<T> a)
+{
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  const int v = blockIdx.y * blockDim.y + threadIdx.y;
+  const int ax1 = a.axis == 0 ? 1 : 0, ax2 = a.axis == 2 ? 1 : 2;
+  int d3[3], s3[3];
+  d3[ax1] = s3[ax1] = a.lo[ax1] + u;
+  d3[ax2] = s3[ax2] = a.lo[ax2] + v;
+  if (d3[ax1] > a.hi[ax1] || d3[ax2] > a.hi[ax2]) return;
+  d3[a.axis] = a.dst;
+  s3[a.axis] = a.src;
+  const long long di = ((long long)d3[2] * a.NY + d3[1]) * a.PX + d3[0];
+  const long long si = ((long long)s3[2] * a.NY + s3[1]) * a.PX + s3[0];
+  a.F[di] = a.sign * a.F[si];
+}
+
+}  // namespace vgpu

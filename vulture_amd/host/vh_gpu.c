@@ -1,0 +1,302 @@
+/*
+ * Device driver for a set-up solve: hands every array of a vh_solve to the
+ * vgpu C-ABI, runs the time loop on the GPU and writes the observer files.
+ * This is the drop-in replacement of propagate() (simulation.c:136-230) for
+ * a host that did the reference's set-up.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include "vh_internal.h"
+#include "../../include/vgpu.h"
+
+/* SIBC orientation matrices (sibc.c:72-110). */
+static const real Acp[4][4] = {{0, 0, 1, 0}, {0, 0, 0, -1}, {-1, 0, 0, 0}, {0, 1, 0, 0}};
+static const real Acm[4][4] = {{0, 0, 0, -1}, {0, 0, 1, 0}, {0, 1, 0, 0}, {-1, 0, 0, 0}};
+static const real Asp[4][4] = {{-1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, -1, 0}, {0, 0, 0, 1}};
+static const real Asm[4][4] = {{0, 1, 0, 0}, {-1, 0, 0, 0}, {0, 0, 0, 1}, {0, 0, -1, 0}};
+static const real Bcp[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
+static const real Bcm[4][4] = {{0, 1, 0, 0}, {1, 0, 0, 0}, {0, 0, 0, 1}, {0, 0, 1, 0}};
+static const real Bsp[4][4] = {{0, 0, -1, 0}, {0, 0, 0, -1}, {1, 0, 0, 0}, {0, 1, 0, 0}};
+static const real Bsm[4][4] = {{0, 0, 0, -1}, {0, 0, -1, 0}, {0, 1, 0, 0}, {1, 0, 0, 0}};
+
+static void comb(double C[4][4], real a, const real A[4][4], real b, const real B[4][4])
+{
+  for (int p = 0; p < 4; p++)
+    for (int q = 0; q < 4; q++) {
+      real v = a * A[p][q] + b * B[p][q];
+      C[p][q] = v;
+    }
+}
+
+static double *cplx(const double complex *z, int n)
+{
+  double *v = calloc((size_t)(n > 0 ? 2 * n : 2), sizeof(double));
+  for (int i = 0; i < n; i++) { v[2 * i] = creal(z[i]); v[2 * i + 1] = cimag(z[i]); }
+  return v;
+}
+
+static int fail(vh_solve *s, vgpu_ctx *c, const char *what, int rc)
+{
+  snprintf(s->err, sizeof(s->err), "%s failed (%d): %s", what, rc, c ? vgpu_last_error(c) : "");
+  return -1;
+}
+
+/* Build a context from a set-up solve.  Returns NULL on error (message in s->err). */
+vgpu_ctx *vh_gpu_setup(vh_solve *s, int prec, int device, unsigned long chunk)
+{
+  vgpu_grid_desc g;
+  memset(&g, 0, sizeof(g));
+  g.nx = s->numCells[0]; g.ny = s->numCells[1]; g.nz = s->numCells[2];
+  memcpy(g.gibox, s->gibox, sizeof(g.gibox));
+  memcpy(g.gobox, s->gobox, sizeof(g.gobox));
+  memcpy(g.gfilim, s->gfilim, sizeof(g.gfilim));
+  g.dt = s->dt; g.c0 = s->c0; g.eps0 = s->eps0; g.mu0 = s->mu0; g.eta0 = s->eta0;
+  g.grid_type = s->gridType;
+  g.xlines = s->xlines; g.ylines = s->ylines; g.zlines = s->zlines;
+  for (int a = 0; a < 3; a++) g.nlines[a] = s->numLines[a];
+  g.dex = s->dex; g.dey = s->dey; g.dez = s->dez;
+  g.idex = s->idex; g.idey = s->idey; g.idez = s->idez;
+  g.idhx = s->idhx; g.idhy = s->idhy; g.idhz = s->idhz;
+  vgpu_ctx *c = NULL;
+  int rc = vgpu_create(&g, prec ? VGPU_FP32 : VGPU_FP64, device, &c);
+  if (rc) { fail(s, NULL, "vgpu_create", rc); return NULL; }
+  const double *A[3] = {s->alphaEx, s->alphaEy, s->alphaEz};
+  const double *B[3] = {s->betaEx, s->betaEy, s->betaEz};
+  const double *G[3] = {s->gammaHx, s->gammaHy, s->gammaHz};
+  if ((rc = vgpu_set_media(c, A, B, G))) goto bad;
+  {
+    vgpu_pml_desc p;
+    memset(&p, 0, sizeof(p));
+    for (int f = 0; f < 6; f++)
+      if (vh_outer_type(s, f) == BT_PML) p.region_mask |= 1u << f;
+    memcpy(p.fplim, s->fplim, sizeof(p.fplim));
+    const double *ad[3] = {s->adx, s->ady, s->adz}, *bd[3] = {s->bdx, s->bdy, s->bdz};
+    const double *ah[3] = {s->ahx, s->ahy, s->ahz}, *bh[3] = {s->bhx, s->bhy, s->bhz};
+    const double *ibd[3] = {s->ibdx, s->ibdy, s->ibdz}, *ibh[3] = {s->ibhx, s->ibhy, s->ibhz};
+    for (int a = 0; a < 3; a++) { p.ad[a] = ad[a]; p.bd[a] = bd[a]; p.ah[a] = ah[a]; p.bh[a] = bh[a]; p.ibd[a] = ibd[a]; p.ibh[a] = ibh[a]; }
+    if ((rc = vgpu_set_pml(c, &p))) goto bad;
+  }
+  {
+    unsigned mm = 0;
+    int ghost[6];
+    for (int f = 0; f < 6; f++) {
+      int t = vh_outer_type(s, f);
+      if (t == BT_MUR) mm |= 1u << f;
+      ghost[f] = t == BT_PEC ? 1 : t == BT_PMC ? 2 : t == BT_PERIODIC ? 3 : 0;
+    }
+    double zeta[6];
+    for (int f = 0; f < 6; f++) zeta[f] = s->zeta[f];
+    if ((rc = vgpu_set_mur(c, mm, zeta, (const int(*)[6][6])s->fmlim))) goto bad;
+    if ((rc = vgpu_set_ghosts(c, ghost))) goto bad;
+  }
+  for (int d = 0; d < s->numDebye; d++) {
+    vh_medium *m = &s->media[s->debye[d].medium];
+    double *da = cplx(m->dalpha, m->numPoles), *db = cplx(m->dbeta, m->numPoles);
+    rc = vgpu_add_debye_block(c, (const int(*)[6])s->debye[d].flim, m->numPoles, da, db);
+    free(da);
+    free(db);
+    if (rc) goto bad;
+  }
+  for (int n = 0; n < s->numSibc; n++) {
+    vh_sibc *sb = &s->sibc[n];
+    vh_boundary *bd = &s->bounds[sb->boundary];
+    vgpu_sibc_desc d;
+    memset(&d, 0, sizeof(d));
+    memcpy(d.gbbox, sb->gbbox, sizeof(d.gbbox));
+    d.normal = sb->normal;
+    if (sb->orientation == 1) { comb(d.A, sb->cosa, Acp, sb->sina, Asp); comb(d.B, sb->cosa, Bcp, sb->sina, Bsp); }
+    else { comb(d.A, sb->cosa, Acm, sb->sina, Asm); comb(d.B, sb->cosa, Bcm, sb->sina, Bsm); }
+    unsigned char *aa = malloc(sb->nfaces * 4), *ab = malloc(sb->nfaces * 4);
+    for (size_t q = 0; q < sb->nfaces * 4; q++) { aa[q] = sb->isAdjA[q]; ab[q] = sb->isAdjB[q]; }
+    d.is_adj_a = aa;
+    d.is_adj_b = ab;
+    for (int dir = 0; dir < 3; dir++) {
+      int lo = 2 * dir, hi = lo + 1;
+      int tl = vh_outer_type(s, lo), th = vh_outer_type(s, hi);
+      d.pmc_index[dir][0] = (tl == BT_PMC || tl == BT_PERIODIC) ? s->gibox[lo] : -1;
+      d.pmc_index[dir][1] = (th == BT_PMC || th == BT_PERIODIC) ? s->gibox[hi] : -1;
+    }
+    double *bufs[48] = {0};
+    for (int f = 0; f < 16; f++) {
+      vh_rcfilter *rf = &bd->rc[f];
+      d.npoles[f] = rf->numPoles;
+      d.asymp[f] = rf->asymp;
+      bufs[3 * f] = cplx(rf->q0, rf->numPoles);
+      bufs[3 * f + 1] = cplx(rf->q1, rf->numPoles);
+      bufs[3 * f + 2] = cplx(rf->q2, rf->numPoles);
+      d.q0[f] = bufs[3 * f]; d.q1[f] = bufs[3 * f + 1]; d.q2[f] = bufs[3 * f + 2];
+    }
+    rc = vgpu_add_sibc_surface(c, &d);
+    for (int q = 0; q < 48; q++) free(bufs[q]);
+    free(aa);
+    free(ab);
+    if (rc) goto bad;
+  }
+  for (int w = 0; w < s->numWaveforms; w++) {
+    vh_waveform *wf = &s->waveforms[w];
+    rc = vgpu_add_waveform(c, wf->type, wf->size, wf->delay, wf->width, wf->frequency, wf->table, wf->tableSize);
+    if (rc < 0) goto bad;
+  }
+  for (int n = 0; n < s->numSources; n++) {
+    vh_source *src = &s->sources[n];
+    int kind;
+    if (src->type == ST_EFIELD || src->type == ST_HFIELD) kind = 0;
+    else if (src->type == ST_ELEC_CURR_DENSITY || src->type == ST_MAGN_CURR_DENSITY) kind = 1;
+    else continue;
+    rc = vgpu_add_source(c, src->field, kind, src->isSoft, src->size, src->waveform, src->delay, src->flim[src->field]);
+    if (rc < 0) goto bad;
+  }
+  for (int n = 0; n < s->numPws; n++) {
+    vh_planewave *pw = &s->pws[n];
+    vgpu_pw_desc d;
+    memset(&d, 0, sizeof(d));
+    memcpy(d.flim, pw->flim, sizeof(d.flim));
+    for (int f = 0; f < 6; f++) d.active[f] = pw->isActive[f];
+    for (int a = 0; a < 3; a++) { d.kinc[a] = pw->kinc[a]; d.ijk0[a] = pw->ijk0[a]; d.r0[a] = pw->r0[a]; }
+    for (int f = 0; f < 6; f++) d.Finc[f] = pw->Finc[f];
+    d.phase_velocity = pw->phaseVelocity;
+    d.waveform = pw->waveform;
+    d.delay = pw->delay;
+    d.use_aux = s->useAuxGrid;
+    if (s->useAuxGrid) {
+      d.aux_nx = pw->nx; d.aux_xb = pw->xb;
+      d.beta_eyi = pw->betaEyi; d.gamma_hzi = pw->gammaHzi;
+      d.aux_adx = pw->adx; d.aux_bdx = pw->bdx; d.aux_ahx = pw->ahx; d.aux_bhx = pw->bhx;
+    }
+    rc = vgpu_add_planewave(c, &d);
+    if (rc < 0) goto bad;
+  }
+  for (int o = 0; o < s->numObs; o++) {
+    vh_observer *ob = &s->obs[o];
+    if (ob->quantity != OQ_EH) continue;
+    if (ob->format == OF_ASCII) {
+      int nf = ob->domain == OD_FREQ ? (int)s->numFreq : 0;
+      rc = vgpu_add_point_observer(c, ob->gbbox[XLO], ob->gbbox[YLO], ob->gbbox[ZLO], s->startTimeStep,
+                                   s->stopTimeStep, nf, s->omega);
+    } else {
+      rc = vgpu_add_volume_observer(c, ob->gbbox, ob->step, s->startTimeStep, s->stopTimeStep);
+    }
+    if (rc < 0) goto bad;
+  }
+  if ((rc = vgpu_finalize(c, s->numTimeSteps, chunk))) goto bad;
+  return c;
+bad:
+  fail(s, c, "vgpu set-up", rc);
+  vgpu_destroy(c);
+  return NULL;
+}
+
+/* Run the whole time loop on the device and write the observer files.
+ * loop_ms receives the device time of the time loop alone. */
+int vh_gpu_run(vh_solve *s, int prec, int device, unsigned long chunk, double *loop_ms)
+{
+  unsigned long NT = s->numTimeSteps;
+  if (chunk == 0 || chunk > NT) chunk = NT ? NT : 1;
+  vgpu_ctx *c = vh_gpu_setup(s, prec, device, chunk);
+  if (!c) return -1;
+  /* Map observers to device ids in the order they were added. */
+  int npt = 0, nvol = 0;
+  int *pid = calloc((size_t)s->numObs + 1, sizeof(int)), *vid = calloc((size_t)s->numObs + 1, sizeof(int));
+  for (int o = 0; o < s->numObs; o++) {
+    pid[o] = vid[o] = -1;
+    if (s->obs[o].quantity != OQ_EH) continue;
+    if (s->obs[o].format == OF_ASCII) pid[o] = npt++;
+    else vid[o] = nvol++;
+  }
+  /* Volume samples are drained after every chunk. */
+  size_t volNodes = 0;
+  uint64_t *vnodes = calloc((size_t)nvol + 1, sizeof(uint64_t));
+  for (int o = 0; o < s->numObs; o++)
+    if (vid[o] >= 0) {
+      vgpu_read_volume_observer(c, vid[o], NULL, 0, &vnodes[vid[o]]);
+      volNodes += vnodes[vid[o]];
+    }
+  float *vol = NULL;
+  if (volNodes) {
+    vol = malloc(volNodes * 6 * sizeof(float) * (NT ? NT : 1));
+    if (!vol) { snprintf(s->err, sizeof(s->err), "volume observer buffer too large"); vgpu_destroy(c); return -1; }
+  }
+  double total = 0.0;
+  for (unsigned long first = 0; first < NT; first += chunk) {
+    unsigned long n = NT - first < chunk ? NT - first : chunk;
+    int rc = vgpu_run(c, first, n, NULL);
+    if (!rc) rc = vgpu_synchronize(c);
+    if (rc) { fail(s, c, "vgpu_run", rc); vgpu_destroy(c); free(vol); return -1; }
+    double ms;
+    vgpu_last_run_timing(c, &ms, NULL, NULL);
+    total += ms;
+    if (volNodes) {
+      size_t off = 0;
+      for (int o = 0; o < s->numObs; o++)
+        if (vid[o] >= 0) {
+          float *tmp = malloc(vnodes[vid[o]] * 6 * sizeof(float) * n);
+          vgpu_read_volume_observer(c, vid[o], tmp, n, NULL);
+          for (unsigned long q = 0; q < n; q++)
+            memcpy(vol + ((first + q) * volNodes + off) * 6, tmp + q * vnodes[vid[o]] * 6,
+                   vnodes[vid[o]] * 6 * sizeof(float));
+          free(tmp);
+          off += vnodes[vid[o]];
+        }
+    }
+  }
+  if (loop_ms) *loop_ms = total;
+  /* Point observers: time series and device DFTs. */
+  unsigned long nout = s->stopTimeStep - s->startTimeStep + 1;
+  double **td = calloc((size_t)s->numObs + 1, sizeof(double *));
+  for (int o = 0; o < s->numObs; o++) {
+    if (pid[o] < 0) continue;
+    td[o] = malloc(nout * 6 * sizeof(double));
+    vh_observer *ob = &s->obs[o];
+    double *re = NULL, *im = NULL;
+    if (ob->domain == OD_FREQ) {
+      re = malloc(6 * s->numFreq * sizeof(double));
+      im = malloc(6 * s->numFreq * sizeof(double));
+    }
+    vgpu_read_point_observer(c, pid[o], td[o], nout, re, im);
+    if (re) {
+      for (size_t q = 0; q < 6 * s->numFreq; q++) { ob->dft_re[q] = re[q]; ob->dft_im[q] = im[q]; }
+      free(re);
+      free(im);
+    }
+  }
+  s->deviceDft = 1;
+  double *pv = calloc((size_t)s->numObs * 6 + 6, sizeof(double));
+  for (unsigned long n = 0; n < NT; n++) {
+    real t = n * s->dt;
+    bool ot = n >= s->startTimeStep && n <= s->stopTimeStep;
+    for (int o = 0; o < s->numObs; o++)
+      if (pid[o] >= 0 && ot)
+        for (int q = 0; q < 6; q++) pv[6 * o + q] = td[o][(n - s->startTimeStep) * 6 + q];
+    vh_observe_step(s, n, t, pv, vol ? vol + (size_t)n * volNodes * 6 : NULL);
+  }
+  for (int o = 0; o < s->numObs; o++) free(td[o]);
+  free(td);
+  free(pv);
+  free(vol);
+  free(vnodes);
+  free(pid);
+  free(vid);
+  vgpu_destroy(c);
+  return 0;
+}
+
+/* Parse, set up, run on the device and write observer files: the whole
+ * solver behind one call (the reference's main(), vulture.c:77-197, with the
+ * time loop on the GPU).  Returns 0 on success. */
+int vh_run_mesh_gpu(const char *mesh, const char *outdir, int prec, int device, int fullPrecision, long steps,
+                    unsigned long chunk, double *loop_ms, char *err, int errlen)
+{
+  vh_solve *s = vh_create();
+  if (!s) return -1;
+  vh_set_dirs(s, NULL, outdir);
+  s->fullPrecision = fullPrecision;
+  int rc = vh_read_mesh(s, mesh);
+  if (!rc && steps >= 0) s->numTimeSteps = (unsigned long)steps;
+  if (!rc) rc = vh_init(s);
+  if (!rc) rc = vh_open_observers(s);
+  if (!rc) rc = vh_gpu_run(s, prec, device, chunk, loop_ms);
+  if (rc && err) snprintf(err, (size_t)errlen, "%s", vh_error(s));
+  vh_destroy(s);
+  return rc;
+}

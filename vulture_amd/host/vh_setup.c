@@ -7,6 +7,7 @@
  * so a float build of this file reproduces the reference float build's
  * coefficients bit for bit and a double build gives the FP64 values.
  */
+#include <sys/stat.h>
 #include "vh_internal.h"
 
 /* ---------------------------------------------------------------------- */
@@ -1207,10 +1208,27 @@ vh_solve *vh_create(void)
 
 const char *vh_error(const vh_solve *s) { return s->err; }
 
+/* Create a directory and its parents (like mkdir -p). */
+static void make_dirs(const char *path)
+{
+  char buf[VH_PATH];
+  snprintf(buf, sizeof(buf), "%s", path);
+  for (char *p = buf + 1; *p; p++)
+    if (*p == '/') {
+      *p = '\0';
+      mkdir(buf, 0777);
+      *p = '/';
+    }
+  mkdir(buf, 0777);
+}
+
 void vh_set_dirs(vh_solve *s, const char *inputDir, const char *outputDir)
 {
   if (inputDir) strncpy(s->inputDir, inputDir, VH_PATH - 1);
-  if (outputDir) strncpy(s->outputDir, outputDir, VH_PATH - 1);
+  if (outputDir && outputDir[0]) {
+    strncpy(s->outputDir, outputDir, VH_PATH - 1);
+    make_dirs(s->outputDir);
+  }
 }
 
 int vh_init(vh_solve *s)
