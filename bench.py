@@ -1,0 +1,172 @@
+#!/usr/bin/env python3
+"""Throughput of the FDTD time loop on one MI355X.
+
+Workload: the Debye-sphere volume -- a 256^3 cubic mesh of 2 mm cells with an
+8-cell UPML on all faces (272^3 = 20,123,648 updated cells), 200 seeded
+single-pole Debye spheres, a TF/SF plane wave at seeded angles on the
+auxiliary-grid path and 8 point observers with 64-frequency running DFTs.
+The mesh file is generated (vulture_amd.meshgen, seed 3) and fed to the same
+parser and set-up as any Vulture mesh.
+
+A step is one full leapfrog update (every kernel of the time loop) of every
+cell.  Both storage forms run: FP64 (the plain form) and FP32 fields with
+FP64 DFT/SIBC state; `value` is the FP32 form's Mcell-updates/s and the FP64
+figures sit in `fp64`.  Fields stay resident in HBM; W untimed steps run
+first, then K steps are timed with hipEvents on the solve's stream.
+
+`roofline` prices the dominant kernel (the E or H volume sweep, whichever
+takes longer) by its own compulsory bytes per launch: E sweep 15 words per
+interior cell (read Ex..Hz 6, alpha/beta 6, write E 3) and 27 per UPML cell
+(+6 split auxiliaries read and written); H sweep 12 and 18 words.  Its average
+duration comes from hipEvents around every launch in the timed steps.
+`step_roofline` prices the whole step with the compulsory bytes of a fused
+update: 21 words per interior cell, 39 per UPML cell and 4 per Debye site and
+pole (J and lastE read and written), over ms_per_step.
+
+`cpu_baseline` times the CPU restatement of the same loop (-O3 timing build,
+OpenMP) on a bounded number of steps of the same mesh on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "Mcells/s FP64 vs FP32 at 256^3 + %HBM peak; p50/p99 sweep-solve latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def _traffic(kernel, prec):
+    """HBM bytes per launch from a committed rocprofv3 --pmc summary, or None."""
+    path = os.path.join(HERE, "profiles", f"traffic_{kernel}_debye256_{prec}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch"):
+            return float(line.split()[1])
+    return None
+
+
+def kernel_bytes(st, esz):
+    """Compulsory bytes per launch of the E and H volume sweeps."""
+    interior = st["mesh_cells"]
+    upml = st["cells"] - st["mesh_cells"]
+    e = (15 * interior + 27 * upml) * esz
+    h = (12 * interior + 18 * upml) * esz
+    return e, h
+
+
+def step_bytes(st, esz):
+    """Compulsory bytes of one fused step (21 / 39 words, +4 per Debye site and pole)."""
+    interior = st["mesh_cells"]
+    upml = st["cells"] - st["mesh_cells"]
+    return (21 * interior + 39 * upml + 4 * st["debye_sites"] * max(st["debye_poles"], 1)) * esz
+
+
+def measure(mesh, prec, steps, warmup):
+    import ctypes
+    from vulture_amd import DeviceSolve, load_vgpu
+
+    class Stats(ctypes.Structure):
+        _fields_ = [("cells", ctypes.c_uint64), ("mesh_cells", ctypes.c_uint64), ("debye_sites", ctypes.c_uint64),
+                    ("debye_poles", ctypes.c_int), ("debye_complex", ctypes.c_int), ("sibc_faces", ctypes.c_uint64),
+                    ("correction_rows", ctypes.c_uint64 * 2), ("bytes", ctypes.c_uint64)]
+
+    t0 = time.time()
+    with DeviceSolve(mesh, precision=prec, steps=warmup + steps, kernel_events=steps) as ds:
+        setup_s = time.time() - t0
+        v = load_vgpu()
+        stc = Stats()
+        v.vgpu_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stats)]
+        if v.vgpu_get_stats(ds._c, ctypes.byref(stc)) != 0:
+            raise RuntimeError("vgpu_get_stats failed")
+        st = {k: getattr(stc, k) for k, _ in Stats._fields_ if k != "correction_rows"}
+        ds.run(0, warmup)
+        ds.synchronize()
+        ds.run(warmup, steps)
+        total_ms = ds.synchronize()
+        e_ms, h_ms = ds.kernel_ms()
+    esz = 8 if prec == "fp64" else 4
+    eb, hb = kernel_bytes(st, esz)
+    if e_ms >= h_ms:
+        name, kb, kms = "k_volume_e", eb, e_ms
+    else:
+        name, kb, kms = "k_volume_h", hb, h_ms
+    ms_step = total_ms / steps
+    achieved = kb / (kms * 1e-3) / 1e9
+    sb = step_bytes(st, esz)
+    step_gbs = sb / (ms_step * 1e-3) / 1e9
+    traffic = _traffic(name, prec)
+    return {
+        "prec": prec, "setup_s": round(setup_s, 2), "stats": st,
+        "value": st["cells"] * steps / (total_ms * 1e-3) / 1e6,
+        "ms_per_step": ms_step,
+        "roofline": {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes": kb, "avg_ms": round(kms, 5),
+                     "e_ms": round(e_ms, 5), "h_ms": round(h_ms, 5)},
+        "step_roofline": {"achieved": round(step_gbs, 1), "frac": round(step_gbs / HBM_PEAK_GBS, 4),
+                          "bytes_per_step": sb, "unit": "GB/s"},
+    }
+
+
+def cpu_baseline(mesh, steps):
+    import oracle
+    threads = min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    sec, cells, n = oracle.time_mesh(mesh, steps=steps, warmup=2, threads=threads, precision="fp32",
+                                     outdir=os.path.join(os.path.dirname(mesh), "cpu"))
+    return {"value": round(cells * n / sec / 1e6, 2), "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{n} timed steps (after 2 untimed) of the same 272^3 Debye mesh, FP32 CPU restatement "
+                      f"(-O3, OpenMP, {threads} threads): {sec:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=256, help="mesh cells per axis")
+    ap.add_argument("--precision", choices=["both", "fp32", "fp64"], default="both")
+    ap.add_argument("--cpu-steps", type=int, default=30, help="timed CPU baseline steps (0 skips it)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(multi-GPU slab decomposition is a design, not built)", file=sys.stderr)
+        return 2
+    from vulture_amd import meshgen
+    work = tempfile.mkdtemp(prefix="vgbench_")
+    mesh = meshgen.write("debye", work, n=args.n, steps=args.warmup + args.steps)
+    res = {}
+    for prec in (["fp64", "fp32"] if args.precision == "both" else [args.precision]):
+        res[prec] = measure(mesh, prec, args.steps, args.warmup)
+        print(f"# {prec}: {res[prec]['value']:.1f} Mcell-updates/s, {res[prec]['ms_per_step']:.3f} ms/step, "
+              f"setup {res[prec]['setup_s']} s", file=sys.stderr, flush=True)
+    head = res.get("fp32") or res["fp64"]
+    st = head["stats"]
+    out = {
+        "metric": METRIC, "value": round(head["value"], 1), "unit": "Mcell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(head["ms_per_step"], 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": head["prec"],
+        "data": "synthetic",
+        "config": {"workload": f"debye spheres {args.n}^3 cubic 2 mm, 8-cell UPML, 200 single-pole Debye "
+                               f"spheres, aux-grid TF/SF plane wave, 8 DFT observers (seed 3)",
+                   "cells": st["cells"], "mesh_cells": st["mesh_cells"], "debye_sites": st["debye_sites"],
+                   "device_bytes": st["bytes"]},
+        "roofline": head["roofline"], "step_roofline": head["step_roofline"],
+    }
+    if "fp64" in res and "fp32" in res:
+        f64 = res["fp64"]
+        out["fp64"] = {"value": round(f64["value"], 1), "ms_per_step": round(f64["ms_per_step"], 4),
+                       "roofline": f64["roofline"], "step_roofline": f64["step_roofline"]}
+        out["fp32_over_fp64"] = round(res["fp32"]["value"] / f64["value"], 3)
+    out["cpu_baseline"] = cpu_baseline(mesh, args.cpu_steps) if args.cpu_steps > 0 else None
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

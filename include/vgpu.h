@@ -166,6 +166,21 @@ int vgpu_download_field(vgpu_ctx *c, int component, double *host);
 /* Device bytes held by the context after vgpu_finalize. */
 size_t vgpu_footprint_bytes(const vgpu_ctx *c);
 
+/* Work counts of a finalized context, for throughput and roofline
+ * accounting (no counterpart in the reference, whose timer reports only
+ * seconds per iteration per cell, timer.c:95-99). */
+typedef struct {
+  uint64_t cells;              /* gobox cells (mesh + UPML) updated per step */
+  uint64_t mesh_cells;         /* gibox cells */
+  uint64_t debye_sites;        /* electric edges carrying a Debye correction */
+  int debye_poles;             /* poles per Debye site (maximum) */
+  int debye_complex;           /* 1 if any pole is complex */
+  uint64_t sibc_faces;         /* SIBC surface faces */
+  uint64_t correction_rows[2]; /* source / TF/SF correction rows, E and H */
+  uint64_t bytes;              /* device footprint */
+} vgpu_stats;
+int vgpu_get_stats(const vgpu_ctx *c, vgpu_stats *out);
+
 /* Time of the kernels of the last vgpu_run, from events on its stream (ms),
  * and the average duration of the E and H volume kernels in it (ms). */
 int vgpu_last_run_timing(const vgpu_ctx *c, double *total_ms, double *volume_e_ms, double *volume_h_ms);

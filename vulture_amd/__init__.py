@@ -12,7 +12,7 @@ loudly when they are missing: there is no CPU fallback.
 import ctypes
 import os
 
-__all__ = ["run_mesh", "load_vgpu", "load_host", "library_path", "VgpuError"]
+__all__ = ["run_mesh", "DeviceSolve", "load_vgpu", "load_host", "library_path", "VgpuError"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "lib")
@@ -52,6 +52,25 @@ def load_host():
         f.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_long,
                       ctypes.c_ulong, ctypes.POINTER(ctypes.c_double), ctypes.c_char_p, ctypes.c_int]
         f.restype = ctypes.c_int
+        f = _host.vh_gpu_open
+        f.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_long, ctypes.c_ulong,
+                      ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p), ctypes.c_char_p, ctypes.c_int]
+        f.restype = ctypes.c_int
+        _host.vh_gpu_close.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        _host.vh_gpu_close.restype = None
+        _host.vh_solve_sizes.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)]
+        _host.vh_solve_sizes.restype = None
+        v = _vgpu
+        v.vgpu_run.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_void_p]
+        v.vgpu_run.restype = ctypes.c_int
+        v.vgpu_synchronize.argtypes = [ctypes.c_void_p]
+        v.vgpu_synchronize.restype = ctypes.c_int
+        v.vgpu_last_run_timing.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_double)] * 3
+        v.vgpu_last_run_timing.restype = ctypes.c_int
+        v.vgpu_last_error.argtypes = [ctypes.c_void_p]
+        v.vgpu_last_error.restype = ctypes.c_char_p
+        v.vgpu_footprint_bytes.argtypes = [ctypes.c_void_p]
+        v.vgpu_footprint_bytes.restype = ctypes.c_uint64
     return _host
 
 
@@ -74,3 +93,76 @@ def run_mesh(mesh, outdir, precision="fp64", full_precision=True, steps=-1, chun
     if rc != 0:
         raise VgpuError(err.value.decode(errors="replace"))
     return ms.value
+
+
+class DeviceSolve:
+    """A mesh parsed, set up and resident on the device, stepped on demand.
+
+    Wraps vh_gpu_open (parse + set-up + upload) and the vgpu C-ABI:
+    ``run(first, n)`` enqueues n time steps, ``synchronize()`` waits and
+    returns the device milliseconds of the last run (hipEvents on the run's
+    stream).  With ``kernel_events=K`` the first K steps of each run also
+    time the E and H volume kernels (``kernel_ms()``).
+    """
+
+    def __init__(self, mesh, precision="fp32", device=0, steps=-1, kernel_events=0):
+        if precision not in ("fp64", "fp32"):
+            raise ValueError("precision must be 'fp64' or 'fp32'")
+        host = load_host()
+        self._v = load_vgpu()
+        if kernel_events:
+            os.environ["VGPU_KERNEL_EVENTS"] = str(int(kernel_events))
+        self._s = ctypes.c_void_p()
+        self._c = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(1024)
+        try:
+            rc = host.vh_gpu_open(os.fsencode(mesh), 1 if precision == "fp32" else 0, device, steps, 0,
+                                  ctypes.byref(self._s), ctypes.byref(self._c), err, len(err))
+        finally:
+            if kernel_events:
+                del os.environ["VGPU_KERNEL_EVENTS"]
+        if rc != 0:
+            raise VgpuError(err.value.decode(errors="replace"))
+        sizes = (ctypes.c_longlong * 3)()
+        host.vh_solve_sizes(self._s, sizes)
+        self.cells, self.mesh_cells, self.steps = int(sizes[0]), int(sizes[1]), int(sizes[2])
+        self.precision = precision
+
+    def _check(self, rc):
+        if rc != 0:
+            raise VgpuError(self._v.vgpu_last_error(self._c).decode(errors="replace"))
+
+    def run(self, first, nsteps, stream=None):
+        self._check(self._v.vgpu_run(self._c, first, nsteps, stream))
+
+    def synchronize(self):
+        self._check(self._v.vgpu_synchronize(self._c))
+        t = ctypes.c_double()
+        self._v.vgpu_last_run_timing(self._c, ctypes.byref(t), None, None)
+        return t.value
+
+    def kernel_ms(self):
+        e, h = ctypes.c_double(), ctypes.c_double()
+        self._v.vgpu_last_run_timing(self._c, None, ctypes.byref(e), ctypes.byref(h))
+        return e.value, h.value
+
+    def footprint_bytes(self):
+        return int(self._v.vgpu_footprint_bytes(self._c))
+
+    def close(self):
+        if self._s:
+            load_host().vh_gpu_close(self._s, self._c)
+            self._s = ctypes.c_void_p()
+            self._c = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

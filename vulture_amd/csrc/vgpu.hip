@@ -653,11 +653,12 @@ static int build_corrections(Ctx *c)
         for (int i = lo[0]; i <= hi[0]; ++i)
           for (int j = lo[1]; j <= hi[1]; ++j)
             for (int k = lo[2]; k <= hi[2]; ++k) {
-              if (i < 1 || j < 1 || k < 1 || i + 1 >= c->NX || j + 1 >= c->NY || k + 1 >= c->NZ)
-                return set_error(c, VGPU_EINVAL, "plane-wave face outside the grid");
               const int n3[3] = {i, j, k};
               int s3[3] = {i, j, k};
               s3[axis] += off;
+              if (i < 0 || j < 0 || k < 0 || i >= c->NX || j >= c->NY || k >= c->NZ || s3[axis] < 0 ||
+                  s3[axis] >= (axis == 0 ? c->NX : axis == 1 ? c->NY : c->NZ))
+                return set_error(c, VGPU_EINVAL, "plane-wave face outside the grid");
               CorrEntry ce{};
               ce.idx = (int)p;
               ce.inc = inc;
@@ -682,12 +683,17 @@ static int build_corrections(Ctx *c)
                 ce.auxid = id;
                 ce.geo = dp;
               } else {
+                for (int a = 0; a < 3; ++a) {
+                  const int m = s3[a] - c->grid.gibox[2 * a];
+                  if (m < 0 || m >= (int)(a == 0 ? c->xl : a == 1 ? c->yl : c->zl).size())
+                    return set_error(c, VGPU_EINVAL, "plane-wave face outside the mesh lines");
+                }
                 field_physical_location(c, r, inc, s3[0], s3[1], s3[2]);
                 dd = d.kinc[0] * (r[0] - d.r0[0]) + d.kinc[1] * (r[1] - d.r0[1]) + d.kinc[2] * (r[2] - d.r0[2]);
                 ce.kind = 3;
                 ce.geo = dd / d.phase_velocity;
               }
-              rows[e].push_back({c->lin(i, j, k) * 3 + comp, seq++, ce});
+              rows[e].push_back({c->lin(i, j, k) * 3 + comp % 3, seq++, ce});
             }
       }
     }
@@ -898,6 +904,22 @@ int vgpu_download_field(vgpu_ctx *c, int component, double *host)
 }
 
 size_t vgpu_footprint_bytes(const vgpu_ctx *c) { return c ? c->bytes : 0; }
+
+int vgpu_get_stats(const vgpu_ctx *c, vgpu_stats *out)
+{
+  if (!c || !out || !c->finalized) return VGPU_EINVAL;
+  const int *go = c->grid.gobox, *gi = c->grid.gibox;
+  out->cells = (uint64_t)(go[1] - go[0]) * (uint64_t)(go[3] - go[2]) * (uint64_t)(go[5] - go[4]);
+  out->mesh_cells = (uint64_t)(gi[1] - gi[0]) * (uint64_t)(gi[3] - gi[2]) * (uint64_t)(gi[5] - gi[4]);
+  out->debye_sites = (uint64_t)c->debyeSites;
+  out->debye_poles = c->debye.empty() ? 0 : c->debyeMaxPoles;
+  out->debye_complex = c->debyeCplx;
+  out->sibc_faces = (uint64_t)c->sibcFaces;
+  out->correction_rows[0] = (uint64_t)c->corrRows[0];
+  out->correction_rows[1] = (uint64_t)c->corrRows[1];
+  out->bytes = c->bytes;
+  return VGPU_OK;
+}
 
 int vgpu_last_run_timing(const vgpu_ctx *c, double *total_ms, double *volume_e_ms, double *volume_h_ms)
 {

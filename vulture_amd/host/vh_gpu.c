@@ -300,3 +300,50 @@ int vh_run_mesh_gpu(const char *mesh, const char *outdir, int prec, int device, 
   vh_destroy(s);
   return rc;
 }
+
+/* Parse and set up a mesh and build its device context without running it,
+ * for callers that drive vgpu_run themselves (timing, serving).  No observer
+ * files are opened.  Returns 0 and sets *solve and *ctx on success. */
+int vh_gpu_open(const char *mesh, int prec, int device, long steps, unsigned long chunk, vh_solve **solve,
+                vgpu_ctx **ctx, char *err, int errlen)
+{
+  *solve = NULL;
+  *ctx = NULL;
+  vh_solve *s = vh_create();
+  if (!s) return -1;
+  vh_set_dirs(s, NULL, NULL);
+  int rc = vh_read_mesh(s, mesh);
+  if (!rc && steps >= 0) s->numTimeSteps = (unsigned long)steps;
+  if (!rc) rc = vh_init(s);
+  vgpu_ctx *c = NULL;
+  if (!rc) {
+    unsigned long NT = s->numTimeSteps;
+    if (chunk == 0 || chunk > NT) chunk = NT ? NT : 1;
+    c = vh_gpu_setup(s, prec, device, chunk);
+    if (!c) rc = -1;
+  }
+  if (rc) {
+    if (err) snprintf(err, (size_t)errlen, "%s", vh_error(s));
+    vh_destroy(s);
+    return rc;
+  }
+  *solve = s;
+  *ctx = c;
+  return 0;
+}
+
+void vh_gpu_close(vh_solve *s, vgpu_ctx *c)
+{
+  if (c) vgpu_destroy(c);
+  if (s) vh_destroy(s);
+}
+
+/* out[0] = gobox cells (mesh + PML), out[1] = mesh cells, out[2] = time steps. */
+void vh_solve_sizes(const vh_solve *s, long long out[3])
+{
+  out[0] = (long long)(s->gobox[XHI] - s->gobox[XLO]) * (s->gobox[YHI] - s->gobox[YLO]) *
+           (s->gobox[ZHI] - s->gobox[ZLO]);
+  out[1] = (long long)(s->gibox[XHI] - s->gibox[XLO]) * (s->gibox[YHI] - s->gibox[YLO]) *
+           (s->gibox[ZHI] - s->gibox[ZLO]);
+  out[2] = (long long)s->numTimeSteps;
+}

@@ -1,0 +1,163 @@
+"""Synthetic workloads written as Vulture mesh files.
+
+Every workload is a plain mesh file in the reference's card format
+(mesh.c:58-191, UserManual "Mesh file format"), so the same file feeds the
+CPU oracle and the device solver.  Random choices use
+``numpy.random.default_rng(seed)``.
+
+Workloads
+---------
+``freespace``  40^3 free space, 6-cell UPML, hard Ez point source, 3 point
+               observers, 1000 steps at Courant number 0.99.
+``enclosure``  200x160x120 graded-line shielded enclosure with a 50x10 mm
+               slot, 10-cell UPML, analytic TF/SF plane wave, 512-frequency
+               DFT observer, 4000 steps.
+``debye``      256^3 cubic 2 mm mesh, 8-cell UPML, 200 non-overlapping
+               single-pole Debye spheres voxelised as k-runs, TF/SF plane wave
+               at seeded angles (auxiliary-grid path), 8 observers with a
+               64-frequency DFT, 3000 steps.  ``n`` and ``spheres`` shrink it
+               for parity runs.
+"""
+import math
+import os
+
+import numpy as np
+
+C0 = 299792458.0
+
+
+def _gauss_card(name, f_20db):
+    """Gaussian pulse reaching -20 dB at f_20db; delay 5 widths."""
+    w = math.sqrt(2.0 * math.log(10.0)) / (2.0 * math.pi * f_20db)
+    return f"WF {name} GAUSSIAN_PULSE 1.0 {5 * w:.6e} {w:.6e}"
+
+
+def freespace(steps=1000, n=40, pml=6):
+    c = n // 2
+    o = c + 8
+    lines = ["VM 1.0.0", f"CE free space {n}^3, {pml}-cell UPML, hard Ez point source", f"DM {n} {n} {n}", "GS"]
+    lines += [f"BT {f} PML {pml}" for f in ("XLO", "XHI", "YLO", "YHI", "ZLO", "ZHI")]
+    lines += ["WF wf1 GAUSSIAN_PULSE 1.0 1.1385e-9 2.277e-10",
+              f"EX {c} {c} {c} {c} {c} {c + 1} src =EZ wf1 1.0",
+              f"OP {o} {o} {c} {c} {c} {c} opx TDOM_ASCII",
+              f"OP {c} {c} {o} {o} {c} {c} opy TDOM_ASCII",
+              f"OP {c} {c} {c} {c} {o} {o} opz TDOM_ASCII",
+              "GE", f"NT {steps}", "CN 0.99", "MS 0.01", "EN"]
+    return "\n".join(lines) + "\n"
+
+
+def graded_sizes(n, fine_lo, fine_len, dmin=1e-3, ratio=1.2, dmax=1e-2):
+    """Cell sizes: `fine_len` cells of dmin from `fine_lo`, growing by `ratio` outwards to dmax."""
+    d = np.empty(n)
+    for i in range(n):
+        if i < fine_lo:
+            m = fine_lo - i
+        elif i >= fine_lo + fine_len:
+            m = i - (fine_lo + fine_len) + 1
+        else:
+            m = 0
+        d[i] = min(dmin * ratio ** m, dmax)
+    return d
+
+
+def _lines_card(axis, sizes):
+    x = np.concatenate([[0.0], np.cumsum(sizes)])
+    body = "\n".join(f"{v:.9e}" for v in x)
+    return f"{axis}L\n{body}"
+
+
+def enclosure(steps=4000, nx=200, ny=160, nz=120, pml=10, nfreq=512):
+    sx = graded_sizes(nx, nx // 2 - 25, 50)
+    sy = graded_sizes(ny, 0, 4)
+    sz = graded_sizes(nz, nz // 2 - 5, 10)
+    a0, a1 = nx // 2 - 25, nx // 2 + 25
+    z0, z1 = nz // 2 - 5, nz // 2 + 5
+    lo, hx, hy, hz = 2, nx - 2, ny - 2, nz - 2
+    lines = ["VM 1.0.0", f"CE graded shielded enclosure {nx}x{ny}x{nz} with a 50x10 mm slot", f"DM {nx} {ny} {nz}", "GS"]
+    lines += [f"BT {f} PML {pml}" for f in ("XLO", "XHI", "YLO", "YHI", "ZLO", "ZHI")]
+    lines += [
+              f"TB {lo} {hx} {lo} {lo} {lo} {z0} PEC", f"TB {lo} {hx} {lo} {lo} {z1} {hz} PEC",
+              f"TB {lo} {a0} {lo} {lo} {z0} {z1} PEC", f"TB {a1} {hx} {lo} {lo} {z0} {z1} PEC",
+              f"TB {lo} {hx} {hy} {hy} {lo} {hz} PEC",
+              f"TB {lo} {hx} {lo} {hy} {lo} {lo} PEC", f"TB {lo} {hx} {lo} {hy} {hz} {hz} PEC",
+              f"TB {lo} {lo} {lo} {hy} {lo} {hz} PEC", f"TB {hx} {hx} {lo} {hy} {lo} {hz} PEC",
+              _gauss_card("wf1", 3e9),
+              f"PW 1 {nx - 1} 1 {ny - 1} 1 {nz - 1} pw1 wf1 90 90 90 111111",
+              f"OP {nx // 2} {nx // 2} {ny // 2} {ny // 2} {nz // 2} {nz // 2} centre1 TDOM_ASCII",
+              f"OP {nx // 2} {nx // 2} {ny // 2} {ny // 2} {nz // 2} {nz // 2} centre2 FDOM_ASCII",
+              "GE", f"NT {steps}", f"OF 1e8 3e9 {nfreq}", "CN 0.99",
+              _lines_card("X", sx), _lines_card("Y", sy), _lines_card("Z", sz), "EN"]
+    return "\n".join(lines) + "\n"
+
+
+def _spheres(rng, n, count, rmin, rmax, gap=2):
+    out = []
+    tries = 0
+    while len(out) < count:
+        tries += 1
+        if tries > 200000:
+            raise RuntimeError("could not place the spheres")
+        r = int(rng.integers(rmin, rmax + 1))
+        if 2 * r + 2 > n:
+            continue
+        c = rng.integers(r + 1, n - r, size=3)
+        if all(np.sum((c - q[0]) ** 2) > (r + q[1] + gap) ** 2 for q in out):
+            out.append((c, r))
+    return out
+
+
+def debye(steps=3000, n=256, spheres=200, pml=8, seed=3, rmin=4, rmax=20, observers=8, nfreq=64):
+    """Debye-sphere volume.  Each sphere is its own single-pole DEBYE medium
+    (eps_inf ~ U[2,10], delta_eps ~ U[1,40], tau ~ logU[1e-11,1e-9] s,
+    sigma ~ logU[1e-3,1] S/m; residue 0.5*delta_eps/tau, pole -1/tau) and is
+    voxelised as one MB card per (i, j) run of cells whose centres lie inside.
+    Each run carries the face mask 101011 (XLO, YLO, ZLO, ZHI included; XHI,
+    YHI left to the neighbouring run), so no electric edge belongs to two
+    Debye blocks."""
+    rng = np.random.default_rng(seed)
+    sph = _spheres(rng, n, spheres, min(rmin, n // 4), min(rmax, n // 4))
+    lines = ["VM 1.0.0", f"CE Debye spheres: {n}^3 cubic 2 mm, {len(sph)} inclusions, {pml}-cell UPML",
+             f"DM {n} {n} {n}", "GS"]
+    lines += [f"BT {f} PML {pml}" for f in ("XLO", "XHI", "YLO", "YHI", "ZLO", "ZHI")]
+    for s in range(len(sph)):
+        einf = rng.uniform(2, 10)
+        de = rng.uniform(1, 40)
+        tau = 10 ** rng.uniform(-11, -9)
+        sig = 10 ** rng.uniform(-3, 0)
+        lines.append(f"MT d{s} DEBYE {einf:.6e} {sig:.6e} 1.0 {0.5 * de / tau:.6e} {-1.0 / tau:.6e}")
+    for s, (c, r) in enumerate(sph):
+        for i in range(c[0] - r, c[0] + r + 1):
+            for j in range(c[1] - r, c[1] + r + 1):
+                h2 = r * r - (i + 0.5 - c[0]) ** 2 - (j + 0.5 - c[1]) ** 2
+                if h2 <= 0:
+                    continue
+                h = math.sqrt(h2)
+                k0 = math.ceil(c[2] - 0.5 - h)
+                k1 = math.floor(c[2] - 0.5 + h)
+                if k1 < k0:
+                    continue
+                lines.append(f"MB {i} {i + 1} {j} {j + 1} {k0} {k1 + 1} d{s} 101011")
+    theta = rng.uniform(0, 180)
+    phi = rng.uniform(0, 360)
+    eta = rng.uniform(0, 360)
+    m = max(4, n // 64)
+    lines.append(_gauss_card("wf1", 15e9))
+    lines.append(f"PW {m} {n - m} {m} {n - m} {m} {n - m} pw1 wf1 {theta:.4f} {phi:.4f} {eta:.4f} 111111")
+    for o in range(observers):
+        p = rng.integers(m + 2, n - m - 2, size=3)
+        lines.append(f"OP {p[0]} {p[0]} {p[1]} {p[1]} {p[2]} {p[2]} t{o} TDOM_ASCII")
+        lines.append(f"OP {p[0]} {p[0]} {p[1]} {p[1]} {p[2]} {p[2]} f{o} FDOM_ASCII")
+    lines += ["GE", f"NT {steps}", f"OF 1e8 15e9 {nfreq}", "MS 2e-3", "EN"]
+    return "\n".join(lines) + "\n"
+
+
+WORKLOADS = {"freespace": freespace, "enclosure": enclosure, "debye": debye}
+
+
+def write(name, outdir, **kw):
+    """Write workload `name` to `outdir`/<name>.mesh and return the path."""
+    os.makedirs(outdir, exist_ok=True)
+    path = os.path.join(outdir, f"{name}.mesh")
+    with open(path, "w") as f:
+        f.write(WORKLOADS[name](**kw))
+    return path

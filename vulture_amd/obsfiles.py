@@ -38,3 +38,59 @@ def rel_l2(x, ref):
     ref = np.asarray(ref, dtype=np.float64)
     d = np.linalg.norm(ref)
     return float(np.linalg.norm(x - ref) / d) if d > 0 else float(np.linalg.norm(x - ref))
+
+
+def waveform_spectrum(outdir):
+    """Raw waveform DFT from the single wf_*_fd.asc in `outdir` (None if absent or ambiguous)."""
+    files = observer_files(outdir, "wf_*_fd.asc")
+    if len(files) != 1:
+        return None
+    return read_fd(files[0])[1][:, 0]
+
+
+def compare_outputs(test_dir, ref_dir, band_db=60.0):
+    """Compare every eh_* observer file of `test_dir` with `ref_dir`.
+
+    Returns a dict with the worst time-series relative L2 (`td`), the worst
+    relative L2 of the raw field DFTs (`fd_raw`: the normalised spectra
+    multiplied back by the waveform DFT, so that frequencies where the
+    excitation has no energy do not amplify rounding noise), and the worst
+    magnitude error in dB over the band where the waveform spectrum lies
+    within `band_db` of its peak (`fd_db`).  Missing or misshapen files give
+    infinity.
+    """
+    res = {"td": 0.0, "fd_raw": 0.0, "fd_db": 0.0, "files": 0}
+    inf = float("inf")
+    for f in observer_files(ref_dir, "eh_*_td.asc"):
+        g = os.path.join(test_dir, os.path.basename(f))
+        if not os.path.exists(g):
+            res["td"] = inf
+            continue
+        _, _, va = read_td(g)
+        _, _, vb = read_td(f)
+        res["td"] = max(res["td"], rel_l2(va, vb) if va.shape == vb.shape else inf)
+        res["files"] += 1
+    wf = waveform_spectrum(ref_dir)
+    for f in observer_files(ref_dir, "eh_*_fd.asc"):
+        g = os.path.join(test_dir, os.path.basename(f))
+        if not os.path.exists(g):
+            res["fd_raw"] = inf
+            continue
+        _, va = read_fd(g)
+        _, vb = read_fd(f)
+        res["files"] += 1
+        if va.shape != vb.shape:
+            res["fd_raw"] = inf
+            continue
+        w = wf[:, None] if wf is not None and len(wf) == len(vb) else np.ones((len(vb), 1))
+        res["fd_raw"] = max(res["fd_raw"],
+                            float(np.linalg.norm(va * w - vb * w) / max(np.linalg.norm(vb * w), 1e-300)))
+        band = np.abs(w[:, 0]) >= np.abs(w[:, 0]).max() * 10 ** (-band_db / 20) if wf is not None else \
+            np.ones(len(vb), bool)
+        ma, mb = np.abs(va[band]), np.abs(vb[band])
+        floor = mb.max() * 1e-3 if mb.size else 0.0
+        sel = mb > floor
+        if sel.any():
+            db = np.abs(20 * np.log10(ma[sel] / mb[sel]))
+            res["fd_db"] = max(res["fd_db"], float(db.max()))
+    return res
