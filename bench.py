@@ -68,23 +68,12 @@ def step_bytes(st, esz):
 
 
 def measure(mesh, prec, steps, warmup):
-    import ctypes
-    from vulture_amd import DeviceSolve, load_vgpu
-
-    class Stats(ctypes.Structure):
-        _fields_ = [("cells", ctypes.c_uint64), ("mesh_cells", ctypes.c_uint64), ("debye_sites", ctypes.c_uint64),
-                    ("debye_poles", ctypes.c_int), ("debye_complex", ctypes.c_int), ("sibc_faces", ctypes.c_uint64),
-                    ("correction_rows", ctypes.c_uint64 * 2), ("bytes", ctypes.c_uint64)]
+    from vulture_amd import DeviceSolve
 
     t0 = time.time()
     with DeviceSolve(mesh, precision=prec, steps=warmup + steps, kernel_events=steps) as ds:
         setup_s = time.time() - t0
-        v = load_vgpu()
-        stc = Stats()
-        v.vgpu_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stats)]
-        if v.vgpu_get_stats(ds._c, ctypes.byref(stc)) != 0:
-            raise RuntimeError("vgpu_get_stats failed")
-        st = {k: getattr(stc, k) for k, _ in Stats._fields_ if k != "correction_rows"}
+        st = ds.stats()
         ds.run(0, warmup)
         ds.synchronize()
         ds.run(warmup, steps)

@@ -60,6 +60,8 @@ def load_host():
         _host.vh_gpu_close.restype = None
         _host.vh_solve_sizes.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)]
         _host.vh_solve_sizes.restype = None
+        _host.vh_solve_shape.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)]
+        _host.vh_solve_shape.restype = None
         v = _vgpu
         v.vgpu_run.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_void_p]
         v.vgpu_run.restype = ctypes.c_int
@@ -71,6 +73,10 @@ def load_host():
         v.vgpu_last_error.restype = ctypes.c_char_p
         v.vgpu_footprint_bytes.argtypes = [ctypes.c_void_p]
         v.vgpu_footprint_bytes.restype = ctypes.c_uint64
+        v.vgpu_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stats)]
+        v.vgpu_get_stats.restype = ctypes.c_int
+        v.vgpu_download_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        v.vgpu_download_field.restype = ctypes.c_int
     return _host
 
 
@@ -93,6 +99,13 @@ def run_mesh(mesh, outdir, precision="fp64", full_precision=True, steps=-1, chun
     if rc != 0:
         raise VgpuError(err.value.decode(errors="replace"))
     return ms.value
+
+
+class Stats(ctypes.Structure):
+    """vgpu_stats (include/vgpu.h)."""
+    _fields_ = [("cells", ctypes.c_uint64), ("mesh_cells", ctypes.c_uint64), ("debye_sites", ctypes.c_uint64),
+                ("debye_poles", ctypes.c_int), ("debye_complex", ctypes.c_int), ("sibc_faces", ctypes.c_uint64),
+                ("correction_rows", ctypes.c_uint64 * 2), ("bytes", ctypes.c_uint64)]
 
 
 class DeviceSolve:
@@ -148,6 +161,26 @@ class DeviceSolve:
 
     def footprint_bytes(self):
         return int(self._v.vgpu_footprint_bytes(self._c))
+
+    def stats(self):
+        st = Stats()
+        self._check(self._v.vgpu_get_stats(self._c, ctypes.byref(st)))
+        d = {k: getattr(st, k) for k, _ in Stats._fields_}
+        d["correction_rows"] = [int(x) for x in st.correction_rows]
+        return d
+
+    def download(self, component):
+        """Field component 0..5 (Ex..Hz) as float64 [nx][ny][nz] including ghost cells."""
+        import numpy as np
+        n = self.grid_shape()
+        out = np.empty(n, dtype=np.float64)
+        self._check(self._v.vgpu_download_field(self._c, component, out.ctypes.data))
+        return out
+
+    def grid_shape(self):
+        sizes = (ctypes.c_longlong * 3)()
+        load_host().vh_solve_shape(self._s, sizes)
+        return tuple(int(x) for x in sizes)
 
     def close(self):
         if self._s:

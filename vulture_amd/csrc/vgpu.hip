@@ -824,6 +824,7 @@ int vgpu_run(vgpu_ctx *c, uint64_t first, uint64_t nsteps, void *stream)
   if (nsteps > c->maxRun && c->vnodes) return set_error(c, VGPU_EINVAL, "run longer than the observer staging");
   if (!c->pobs.empty() && first + nsteps > c->totalSteps) return set_error(c, VGPU_EINVAL, "run past total steps");
   hipSetDevice(c->device);
+  (void)hipGetLastError();  // launch errors reported below belong to this run only
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
   hipEventRecord(c->ev0, s);
   int r = c->prec == VGPU_FP64 ? run_steps<double>(c, first, nsteps, s) : run_steps<float>(c, first, nsteps, s);
@@ -843,7 +844,10 @@ int vgpu_synchronize(vgpu_ctx *c)
   c->last_total_ms = ms;
   double se = 0, sh = 0;
   int ne = 0;
-  for (size_t q = 0; q + 1 < c->evE.size(); q += 2) {
+  // only the pairs the last run recorded: querying an unrecorded event fails
+  // and would leave a stale error behind for the next launch check
+  const size_t rec = 2 * std::min<size_t>(c->evE.size() / 2, (size_t)c->lastRunSteps);
+  for (size_t q = 0; q + 1 < rec; q += 2) {
     float a = 0, b = 0;
     if (hipEventElapsedTime(&a, c->evE[q], c->evE[q + 1]) == hipSuccess &&
         hipEventElapsedTime(&b, c->evH[q], c->evH[q + 1]) == hipSuccess) {

@@ -347,3 +347,9 @@ void vh_solve_sizes(const vh_solve *s, long long out[3])
            (s->gibox[ZHI] - s->gibox[ZLO]);
   out[2] = (long long)s->numTimeSteps;
 }
+
+/* Array extents numCells[3] (gobox + ghost layers) of the host layout. */
+void vh_solve_shape(const vh_solve *s, long long out[3])
+{
+  for (int a = 0; a < 3; a++) out[a] = s->numCells[a];
+}

@@ -67,11 +67,12 @@ def _lines_card(axis, sizes):
 
 
 def enclosure(steps=4000, nx=200, ny=160, nz=120, pml=10, nfreq=512):
-    sx = graded_sizes(nx, nx // 2 - 25, 50)
+    ax, az = min(25, nx // 4), min(5, nz // 8)
+    sx = graded_sizes(nx, nx // 2 - ax, 2 * ax)
     sy = graded_sizes(ny, 0, 4)
-    sz = graded_sizes(nz, nz // 2 - 5, 10)
-    a0, a1 = nx // 2 - 25, nx // 2 + 25
-    z0, z1 = nz // 2 - 5, nz // 2 + 5
+    sz = graded_sizes(nz, nz // 2 - az, 2 * az)
+    a0, a1 = nx // 2 - ax, nx // 2 + ax
+    z0, z1 = nz // 2 - az, nz // 2 + az
     lo, hx, hy, hz = 2, nx - 2, ny - 2, nz - 2
     lines = ["VM 1.0.0", f"CE graded shielded enclosure {nx}x{ny}x{nz} with a 50x10 mm slot", f"DM {nx} {ny} {nz}", "GS"]
     lines += [f"BT {f} PML {pml}" for f in ("XLO", "XHI", "YLO", "YHI", "ZLO", "ZHI")]

@@ -1,0 +1,116 @@
+"""rocprofv3 helpers for the roofline figures.
+
+  python -m vulture_amd.profiling copy [--mib N]
+      A plain elementwise copy of N MiB (default 1024), five times: the
+      calibration kernel for the FETCH_SIZE / WRITE_SIZE counters (it reads
+      and writes exactly N MiB per dispatch).
+  python -m vulture_amd.profiling stats DIR
+      Print the kernel_stats.csv that `rocprofv3 --kernel-trace --stats -d DIR`
+      wrote, as a text table.
+  python -m vulture_amd.profiling traffic --kernel NAME --fetch DIR --write DIR
+          --copy-fetch DIR --copy-write DIR [--mib N]
+      From four `rocprofv3 --pmc FETCH_SIZE|WRITE_SIZE -d DIR` runs (the
+      workload and the copy kernel), print the HBM bytes per launch of
+      kernel NAME: each counter (KiB per dispatch) scaled by what it reads
+      for the copy of known size.
+"""
+import argparse
+import csv
+import glob
+import os
+import sys
+
+
+def _rows(d, suffix):
+    files = sorted(glob.glob(os.path.join(d, "**", f"*{suffix}"), recursive=True))
+    if not files:
+        raise SystemExit(f"no *{suffix} under {d}")
+    out = []
+    for f in files:
+        with open(f, newline="") as fh:
+            out.extend(csv.DictReader(fh))
+    return out
+
+
+def counter_per_dispatch(d, kernel, counter):
+    """Mean value of `counter` over dispatches whose kernel name contains `kernel`."""
+    per = {}
+    for r in _rows(d, "counter_collection.csv"):
+        if kernel not in r.get("Kernel_Name", "") or r.get("Counter_Name") != counter:
+            continue
+        key = r.get("Dispatch_Id") or r.get("Correlation_Id")
+        per[key] = per.get(key, 0.0) + float(r["Counter_Value"])
+    if not per:
+        raise SystemExit(f"no {counter} rows for kernels matching {kernel!r} under {d}")
+    return sum(per.values()) / len(per), len(per)
+
+
+def cmd_copy(args):
+    import torch
+    n = args.mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        torch.mul(a, 1.0, out=b)  # one elementwise kernel: reads and writes N MiB
+    torch.cuda.synchronize()
+    print(f"copied {args.mib} MiB x5", flush=True)
+
+
+def cmd_stats(args):
+    rows = _rows(args.dir, "kernel_stats.csv")
+    rows.sort(key=lambda r: -float(r.get("TotalDurationNs", 0)))
+    print(f"{'calls':>7} {'avg_us':>10} {'total_ms':>10} {'pct':>6}  kernel")
+    for r in rows:
+        name = r["Name"]
+        if len(name) > 90:
+            name = name[:87] + "..."
+        print(f"{int(r['Calls']):7d} {float(r['AverageNs']) / 1e3:10.2f} {float(r['TotalDurationNs']) / 1e6:10.3f} "
+              f"{float(r['Percentage']):6.2f}  {name}")
+
+
+def cmd_traffic(args):
+    copy_bytes = args.mib * (1 << 20)
+    cf, _ = counter_per_dispatch(args.copy_fetch, args.copy_kernel, "FETCH_SIZE")
+    cw, _ = counter_per_dispatch(args.copy_write, args.copy_kernel, "WRITE_SIZE")
+    kf, nf = counter_per_dispatch(args.fetch, args.kernel, "FETCH_SIZE")
+    kw, nw = counter_per_dispatch(args.write, args.kernel, "WRITE_SIZE")
+    ff = copy_bytes / (cf * 1024.0)
+    fw = copy_bytes / (cw * 1024.0)
+    fetch = kf * 1024.0 * ff
+    write = kw * 1024.0 * fw
+    print(f"kernel {args.kernel}")
+    print(f"dispatches_fetch {nf}")
+    print(f"dispatches_write {nw}")
+    print(f"copy_FETCH_SIZE_KiB {cf:.1f}")
+    print(f"copy_WRITE_SIZE_KiB {cw:.1f}")
+    print(f"copy_bytes {copy_bytes}")
+    print(f"scale_fetch {ff:.4f}")
+    print(f"scale_write {fw:.4f}")
+    print(f"FETCH_SIZE_KiB_per_launch {kf:.1f}")
+    print(f"WRITE_SIZE_KiB_per_launch {kw:.1f}")
+    print(f"fetch_bytes_per_launch {fetch:.0f}")
+    print(f"write_bytes_per_launch {write:.0f}")
+    print(f"traffic_bytes_per_launch {fetch + write:.0f}")
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description="rocprofv3 helpers")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    p = sub.add_parser("copy")
+    p.add_argument("--mib", type=int, default=1024)
+    p = sub.add_parser("stats")
+    p.add_argument("dir")
+    p = sub.add_parser("traffic")
+    p.add_argument("--kernel", required=True)
+    p.add_argument("--fetch", required=True)
+    p.add_argument("--write", required=True)
+    p.add_argument("--copy-fetch", required=True)
+    p.add_argument("--copy-write", required=True)
+    p.add_argument("--copy-kernel", default="elementwise")
+    p.add_argument("--mib", type=int, default=1024)
+    args = ap.parse_args(argv)
+    {"copy": cmd_copy, "stats": cmd_stats, "traffic": cmd_traffic}[args.cmd](args)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])
