@@ -121,6 +121,13 @@ def main():
     ap.add_argument("--n", type=int, default=256, help="mesh cells per axis")
     ap.add_argument("--precision", choices=["both", "fp32", "fp64"], default="both")
     ap.add_argument("--cpu-steps", type=int, default=30, help="timed CPU baseline steps (0 skips it)")
+    ap.add_argument("--spheres", type=int, default=200, help="Debye spheres (workload variants for profiling)")
+    ap.add_argument("--boundary", choices=["PML", "PEC", "MUR"], default="PML")
+    ap.add_argument("--no-planewave", action="store_true", help="hard point source instead of the TF/SF plane wave")
+    ap.add_argument("--serve", type=int, default=24, help="requests in the serving sweep (0 skips it)")
+    ap.add_argument("--serve-active", type=int, default=8, help="solves in flight at once when serving")
+    ap.add_argument("--serve-nmax", type=int, default=128, help="largest cells per axis of a serving request")
+    ap.add_argument("--serve-steps", type=int, nargs=2, default=[200, 800], help="step range of serving requests")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -128,7 +135,8 @@ def main():
         return 2
     from vulture_amd import meshgen
     work = tempfile.mkdtemp(prefix="vgbench_")
-    mesh = meshgen.write("debye", work, n=args.n, steps=args.warmup + args.steps)
+    mesh = meshgen.write("debye", work, n=args.n, steps=args.warmup + args.steps, spheres=args.spheres,
+                         boundary=args.boundary, planewave=not args.no_planewave)
     res = {}
     for prec in (["fp64", "fp32"] if args.precision == "both" else [args.precision]):
         res[prec] = measure(mesh, prec, args.steps, args.warmup)
@@ -153,6 +161,19 @@ def main():
                        "roofline": f64["roofline"], "step_roofline": f64["step_roofline"]}
         out["fp32_over_fp64"] = round(res["fp32"]["value"] / f64["value"], 3)
     out["cpu_baseline"] = cpu_baseline(mesh, args.cpu_steps) if args.cpu_steps > 0 else None
+    if args.serve > 0:
+        from vulture_amd import serve
+        rep = serve.sweep(os.path.join(work, "serve"), count=args.serve, load=0.7, precision="fp32",
+                          max_active=args.serve_active, chunk=200, nmin=32, nmax=args.serve_nmax,
+                          steps_min=args.serve_steps[0], steps_max=args.serve_steps[1], nfreq=64)
+        rep["workload"] = (f"{args.serve} seeded requests: cells per axis log-uniform [32, {args.serve_nmax}], "
+                           f"graded lines, 4-24 SIMPLE/DEBYE blocks, 8-cell UPML, TF/SF at random angles, "
+                           f"{args.serve_steps[0]}-{args.serve_steps[1]} steps, 64-frequency DFT observers; "
+                           f"Poisson arrivals at 70% of isolated capacity; FP32")
+        out["serving"] = rep
+        print(f"# serving: p50 {rep['p50_latency_s']:.3f} s p99 {rep['p99_latency_s']:.3f} s, "
+              f"saturated {rep['saturated_mcells_per_s']:.0f} vs isolated {rep['isolated_mcells_per_s']:.0f} "
+              f"Mcell-updates/s", file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -152,6 +152,11 @@ int vgpu_finalize(vgpu_ctx *c, uint64_t total_steps, uint64_t max_steps_per_run)
 int vgpu_run(vgpu_ctx *c, uint64_t first, uint64_t nsteps, void *stream);
 int vgpu_synchronize(vgpu_ctx *c);
 
+/* Non-blocking completion test of the last vgpu_run: 1 done, 0 still
+ * running, <0 device error.  No counterpart in the reference, whose loop is
+ * synchronous; it lets one host thread keep many solves in flight. */
+int vgpu_query(vgpu_ctx *c);
+
 /* Point observer samples: td[n][6] for every step sampled so far (FP64),
  * dft_re/dft_im[6][nfreq] (either may be NULL). */
 int vgpu_read_point_observer(vgpu_ctx *c, int id, double *td, uint64_t nsteps, double *dft_re, double *dft_im);

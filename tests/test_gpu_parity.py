@@ -126,3 +126,13 @@ def test_full_size_fp32_tracks_fp64_and_fp64_is_linear(tmp_path):
         assert np.count_nonzero(a) > 1000
         assert np.linalg.norm(b - a) / np.linalg.norm(a) <= 1e-4
         assert np.array_equal(c, 2.0 * a)
+
+
+@pytest.mark.parametrize("name", ["parplate1D_debyepml_xyz", "closedbox_go04", "freespace_mur", "dgref_pml6"])
+def test_full_coefficient_arrays_path(name, tmp_path, monkeypatch):
+    """The sweep kernels' fallback that reads the per-cell alpha/beta/gamma
+    arrays (taken when the coefficient classes do not fit 16-bit indices)
+    gives the same results as the class-table path."""
+    monkeypatch.setenv("VGPU_NO_INDEXED_MEDIA", "1")
+    mesh = _stage(name, tmp_path / "in")
+    _check(_run_pair(mesh, tmp_path, "fp64"), "fp64")

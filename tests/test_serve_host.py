@@ -1,0 +1,70 @@
+"""The solve server's host logic (scheduler, admission, workers, output) on
+the CPU: libvhost's sources are linked against a host-only stand-in for the
+device library and run under ThreadSanitizer with meshes of every kind."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT
+from vulture_amd import meshgen
+
+HOST = os.path.join(ROOT, "vulture_amd", "host")
+NATIVE = os.path.join(ROOT, "tests", "native")
+SRC = ["vh_parse.c", "vh_setup.c", "vh_observe.c", "vh_gpu.c", "vh_serve.c"]
+
+
+@pytest.fixture(scope="module")
+def driver(tmp_path_factory):
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    d = tmp_path_factory.mktemp("serve")
+    exe = str(d / "serve_driver")
+    cmd = ["gcc", "-std=gnu99", "-O1", "-g", "-fsanitize=thread", "-DVH_REAL=double", "-I" + HOST,
+           "-Wno-stringop-truncation", "-Wno-format-truncation", "-Wno-unused-result"]
+    cmd += [os.path.join(HOST, f) for f in SRC]
+    cmd += [os.path.join(NATIVE, "vgpu_stub.c"), os.path.join(NATIVE, "serve_driver.c"), "-lm", "-lpthread", "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        pytest.fail(r.stderr[-2000:])
+    return exe
+
+
+def test_server_completes_every_request_without_races(driver, tmp_path):
+    meshes = []
+    rng = np.random.default_rng(7)
+    for q in range(6):
+        text, _, _ = meshgen.serving_request(rng, nmin=12, nmax=24, steps_min=20, steps_max=60, nfreq=8)
+        p = tmp_path / f"r{q}.mesh"
+        p.write_text(text)
+        meshes.append(str(p))
+    for name in ("closedbox_go04", "dgref_pml6", "freespace_mur"):
+        src = os.path.join(GOLDEN, "ref_inputs", name)
+        dst = tmp_path / name
+        shutil.copytree(src, dst)
+        meshes.append(str(dst / f"{name}.mesh"))
+    env = dict(os.environ, TSAN_OPTIONS="halt_on_error=1 second_deadlock_stack=1")
+    r = subprocess.run([driver, "3", str(tmp_path / "out")] + meshes, capture_output=True, text=True,
+                       timeout=600, env=env, cwd=str(tmp_path))
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
+    assert "ThreadSanitizer" not in r.stderr, r.stderr[-4000:]
+    lines = r.stdout.strip().splitlines()
+    assert len(lines) == len(meshes)
+    for q, line in enumerate(lines):
+        assert line.split()[1] == "0", line
+        out = tmp_path / "out" / str(q)
+        assert any(f.endswith(".asc") for f in os.listdir(out)), q
+
+
+def test_failed_request_is_reported_and_others_finish(driver, tmp_path):
+    good = tmp_path / "good.mesh"
+    good.write_text(meshgen.freespace(steps=30, n=16, pml=4))
+    bad = tmp_path / "bad.mesh"
+    bad.write_text("VM 1.0.0\nDM 10 10 10\nGS\nNOT A CARD\nGE\nEN\n")
+    r = subprocess.run([driver, "2", str(tmp_path / "out"), str(good), str(bad), str(good)], capture_output=True,
+                       text=True, timeout=300, cwd=str(tmp_path))
+    assert "ThreadSanitizer" not in r.stderr, r.stderr[-4000:]
+    rcs = [l.split()[1] for l in r.stdout.strip().splitlines()]
+    assert rcs == ["0", "-1", "0"], r.stdout

@@ -69,7 +69,7 @@ def build(verbose=False, force=True):
     libvgpu = os.path.join(LIB, "libvgpu.so")
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", libvgpu] + objs, verbose)
 
-    hsrc = ["vh_parse.c", "vh_setup.c", "vh_observe.c", "vh_gpu.c"]
+    hsrc = ["vh_parse.c", "vh_setup.c", "vh_observe.c", "vh_gpu.c", "vh_serve.c"]
     hobjs = []
     for src in hsrc:
         s = os.path.join(HOST, src)
@@ -77,7 +77,8 @@ def build(verbose=False, force=True):
         _run(["gcc"] + C_COMMON + ["-DVH_REAL=double", "-c", s, "-o", o], verbose)
         hobjs.append(o)
     libvhost = os.path.join(LIB, "libvhost.so")
-    _run(["gcc", "-shared", "-o", libvhost] + hobjs + ["-L" + LIB, "-lvgpu", "-Wl,-rpath,$ORIGIN", "-lm"], verbose)
+    _run(["gcc", "-shared", "-o", libvhost] + hobjs + ["-L" + LIB, "-lvgpu", "-Wl,-rpath,$ORIGIN", "-lm", "-lpthread"],
+         verbose)
     cli = os.path.join(BIN, "vgvulture")
     _run(["gcc"] + C_COMMON + [os.path.join(HOST, "vgvulture_main.c"), "-o", cli, "-L" + LIB, "-lvhost", "-lvgpu",
                                "-Wl,-rpath,$ORIGIN/../lib", "-lm"], verbose)

@@ -12,8 +12,10 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <array>
 #include <map>
-#include "vgpu_internal.h"
+#include <unordered_map>
+#include "vgpu_kernels.h"
 
 namespace vgpu {
 
@@ -68,6 +70,52 @@ static int upload_field(Ctx *c, void *dst, const double *src)
   if (hipMemcpy(dst, buf.data(), n * c->esz, hipMemcpyHostToDevice) != hipSuccess)
     return set_error(c, VGPU_EDEVICE, "upload failed");
   return VGPU_OK;
+}
+
+// Coefficient classes: the distinct N-tuples of per-cell coefficients, in
+// first-seen order over the host layout, and the class of every device cell
+// (padding cells stay in class 0).
+template <int N> struct TupleHash {
+  size_t operator()(const std::array<uint64_t, N> &k) const
+  {
+    uint64_t h = 1469598103934665603ull;
+    for (int q = 0; q < N; ++q) h = (h ^ k[q]) * 1099511628211ull + (k[q] >> 29);
+    return (size_t)h;
+  }
+};
+
+template <int N>
+static void build_classes(Ctx *c, const double *const src[N], std::vector<uint32_t> &cls, std::vector<double> &tab)
+{
+  cls.assign(c->ncell, 0);
+  tab.clear();
+  std::unordered_map<std::array<uint64_t, N>, uint32_t, TupleHash<N>> ids;
+  std::array<uint64_t, N> prev{};
+  uint32_t prevId = UINT32_MAX;
+  const size_t NY = c->NY, NZ = c->NZ;
+  for (int i = 0; i < c->NX; ++i)
+    for (int j = 0; j < c->NY; ++j)
+      for (int k = 0; k < c->NZ; ++k) {
+        const size_t h = ((size_t)i * NY + j) * NZ + k;
+        std::array<uint64_t, N> key;
+        for (int q = 0; q < N; ++q) memcpy(&key[q], &src[q][h], 8);
+        uint32_t id;
+        if (prevId != UINT32_MAX && key == prev) {
+          id = prevId;
+        } else {
+          auto it = ids.find(key);
+          if (it == ids.end()) {
+            id = (uint32_t)ids.size();
+            ids.emplace(key, id);
+            for (int q = 0; q < N; ++q) tab.push_back(src[q][h]);
+          } else {
+            id = it->second;
+          }
+          prev = key;
+          prevId = id;
+        }
+        cls[c->lin(i, j, k)] = id;
+      }
 }
 
 static void *upload_line(Ctx *c, const double *src, int n)
@@ -161,6 +209,22 @@ int vgpu_create(const vgpu_grid_desc *g, vgpu_prec prec, int device, vgpu_ctx **
   c->grid.dex = c->grid.dey = c->grid.dez = nullptr;
   c->grid.idex = c->grid.idey = c->grid.idez = nullptr;
   c->grid.idhx = c->grid.idhy = c->grid.idhz = nullptr;
+  // Fast boxes: where every E (H) component takes the ordinary update; x
+  // chunks of the sweeps start on the E fast box so whole wavefronts fall
+  // inside it.
+  for (int e = 0; e < 2; ++e) {
+    int *fb = e == 0 ? c->fastE : c->fastH;
+    for (int a = 0; a < 3; ++a) {
+      fb[2 * a] = 1;
+      fb[2 * a + 1] = (a == 0 ? c->NX : a == 1 ? c->NY : c->NZ) - 2;
+      for (int d = 0; d < 3; ++d) {
+        fb[2 * a] = std::max(fb[2 * a], g->gfilim[3 * e + d][2 * a]);
+        fb[2 * a + 1] = std::min(fb[2 * a + 1], g->gfilim[3 * e + d][2 * a + 1]);
+      }
+    }
+  }
+  c->X0 = c->fastE[0] <= c->fastE[1] ? c->fastE[0] - 64 * ((c->fastE[0] + 63) / 64) : 0;
+  c->NCH = (c->NX - c->X0 + 63) / 64;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return VGPU_EDEVICE;
@@ -210,6 +274,10 @@ int vgpu_set_media(vgpu_ctx *c, const double *alpha[3], const double *beta[3], c
         (r = upload_field(c, c->gamma[d], gamma[d])))
       return r;
   }
+  const double *e6[6] = {alpha[0], beta[0], alpha[1], beta[1], alpha[2], beta[2]};
+  build_classes<6>(c, e6, c->ecls, c->etab);
+  build_classes<3>(c, gamma, c->hcls, c->htab);
+  c->edslot.assign(c->etab.size() / 6 * 3, -1);
   c->media_set = true;
   return VGPU_OK;
 }
@@ -384,8 +452,7 @@ int vgpu_add_volume_observer(vgpu_ctx *c, const int gbbox[6], const int step[3],
 static int build_debye(Ctx *c)
 {
   if (c->debye.empty()) return VGPU_OK;
-  // Debye media table: one slot per block (blocks share media in the
-  // reference, but the coefficient table is tiny).
+  // Distinct Debye media (blocks of one medium share their coefficients).
   int maxP = 1, cplx = 0;
   for (auto &b : c->debye) {
     maxP = std::max(maxP, b.np);
@@ -394,21 +461,37 @@ static int build_debye(Ctx *c)
   }
   c->debyeMaxPoles = maxP;
   c->debyeCplx = cplx;
-  std::vector<double> da((size_t)c->debye.size() * maxP * 2, 0.0), db(da.size(), 0.0);
-  std::vector<int> np(c->debye.size());
+  std::map<std::vector<double>, int> mediumOf;
+  std::vector<int> blockMedium(c->debye.size());
+  std::vector<double> da, db;
+  std::vector<int> np;
   for (size_t b = 0; b < c->debye.size(); ++b) {
-    np[b] = c->debye[b].np;
-    for (int p = 0; p < c->debye[b].np; ++p)
-      for (int q = 0; q < 2; ++q) {
-        da[(b * maxP + p) * 2 + q] = c->debye[b].da[2 * p + q];
-        db[(b * maxP + p) * 2 + q] = c->debye[b].db[2 * p + q];
-      }
+    const auto &blk = c->debye[b];
+    std::vector<double> key;
+    key.push_back(blk.np);
+    key.insert(key.end(), blk.da.begin(), blk.da.begin() + 2 * blk.np);
+    key.insert(key.end(), blk.db.begin(), blk.db.begin() + 2 * blk.np);
+    auto it = mediumOf.find(key);
+    if (it == mediumOf.end()) {
+      const int m = (int)np.size();
+      it = mediumOf.emplace(key, m).first;
+      np.push_back(blk.np);
+      da.resize((size_t)(m + 1) * maxP * 2, 0.0);
+      db.resize(da.size(), 0.0);
+      for (int p = 0; p < blk.np; ++p)
+        for (int q = 0; q < 2; ++q) {
+          da[((size_t)m * maxP + p) * 2 + q] = blk.da[2 * p + q];
+          db[((size_t)m * maxP + p) * 2 + q] = blk.db[2 * p + q];
+        }
+    }
+    blockMedium[b] = it->second;
   }
   // Sites in block order; a cell already covered by an earlier block goes to
-  // the next layer so corrections apply in block order.
+  // the next layer so corrections apply in block order (debye.c:243-307
+  // loops block by block).
   struct Site { int layer, comp; long long cell; int med; };
   std::vector<Site> sites;
-  std::map<long long, int> seen;
+  std::unordered_map<long long, int> seen;
   int layers = 1;
   for (size_t b = 0; b < c->debye.size(); ++b)
     for (int f = 0; f < 3; ++f) {
@@ -419,14 +502,14 @@ static int build_debye(Ctx *c)
             if (i < 0 || j < 0 || k < 0 || i >= c->NX || j >= c->NY || k >= c->NZ)
               return set_error(c, VGPU_EINVAL, "Debye block outside the grid");
             const long long cell = c->lin(i, j, k);
-            const long long key = cell * 3 + f;
-            int &cnt = seen[key];
+            int &cnt = seen[cell * 3 + f];
             const int layer = cnt++;
             layers = std::max(layers, layer + 1);
-            sites.push_back({layer, f, cell, (int)b});
+            sites.push_back({layer, f, cell, blockMedium[b]});
           }
     }
-  std::stable_sort(sites.begin(), sites.end(), [](const Site &a, const Site &b) {
+  seen.clear();
+  std::sort(sites.begin(), sites.end(), [](const Site &a, const Site &b) {
     if (a.layer != b.layer) return a.layer < b.layer;
     if (a.comp != b.comp) return a.comp < b.comp;
     return a.cell < b.cell;
@@ -441,6 +524,67 @@ static int build_debye(Ctx *c)
   for (long long s = 0; s < n; ++s) { comp[s] = sites[s].comp; cell[s] = sites[s].cell; med[s] = sites[s].med; }
   c->debyeLayers = layers;
   c->debyeSites = n;
+  c->debyeL0 = c->debyeLayerStart[1];
+
+  // Refine the E coefficient classes with each component's layer-0 Debye
+  // medium, so the E sweep can apply layer 0 itself.
+  std::unordered_map<long long, std::array<int, 3>> slot;
+  for (long long s = 0; s < c->debyeL0; ++s) {
+    auto it = slot.find(cell[s]);
+    if (it == slot.end()) it = slot.emplace(cell[s], std::array<int, 3>{-1, -1, -1}).first;
+    it->second[comp[s]] = med[s];
+  }
+  std::map<std::array<int, 4>, uint32_t> refined;
+  for (auto &kv : slot) {
+    const uint32_t old = c->ecls[kv.first];
+    const std::array<int, 4> key{(int)old, kv.second[0], kv.second[1], kv.second[2]};
+    auto it = refined.find(key);
+    if (it == refined.end()) {
+      const uint32_t id = (uint32_t)(c->etab.size() / 6);
+      for (int q = 0; q < 6; ++q) c->etab.push_back(c->etab[(size_t)old * 6 + q]);
+      for (int q = 0; q < 3; ++q) c->edslot.push_back(kv.second[q]);
+      it = refined.emplace(key, id).first;
+    }
+    c->ecls[kv.first] = it->second;
+  }
+  // First layer-0 site of every (component, row, 64-cell x chunk), for the
+  // E sweep's in-wavefront site numbering.
+  const size_t nrow = (size_t)c->NZ * c->NY * c->NCH;
+  std::vector<int> dbase(3 * nrow + 1, 0);
+  for (long long s = 0; s < c->debyeL0; ++s) {
+    const long long cl = cell[s];
+    const int i = (int)(cl % c->PX);
+    const long long row = cl / c->PX;  // k*NY + j
+    const size_t q = (size_t)comp[s] * nrow + (size_t)row * c->NCH + (i - c->X0) / 64;
+    dbase[q + 1]++;
+  }
+  if (c->debyeL0 > INT32_MAX) return set_error(c, VGPU_EINVAL, "too many Debye sites");
+  for (size_t q = 1; q < dbase.size(); ++q) dbase[q] += dbase[q - 1];
+  // dbase[q] is the first site of chunk q; dbase[3*nrow] the layer-0 count.
+  // The E sweep numbers layer-0 sites by counting, in x order within each
+  // 64-cell chunk, the cells whose class carries a Debye medium for the
+  // component.  Check here that this numbering reproduces the site list
+  // exactly, so the sweep can never address a site outside it.
+  for (int f = 0; f < 3; ++f) {
+    long long g = f == 0 ? 0 : dbase[(size_t)f * nrow];
+    for (int k = 0; k < c->NZ; ++k)
+      for (int j = 0; j < c->NY; ++j)
+        for (int i = 0; i < c->NX; ++i) {
+          const long long cl = c->lin(i, j, k);
+          const size_t row = (size_t)f * nrow + ((size_t)k * c->NY + j) * c->NCH + (i - c->X0) / 64;
+          if (((i - c->X0) % 64 == 0 || i == 0) && dbase[row] != g)
+            return set_error(c, VGPU_EINVAL, "Debye site numbering (chunk start)");
+          const int ds = c->edslot[(size_t)c->ecls[cl] * 3 + f];
+          if (ds < 0) continue;
+          if (g >= c->debyeL0 || cell[g] != cl || comp[g] != f || med[g] != ds)
+            return set_error(c, VGPU_EINVAL, "Debye site numbering (site %lld)", g);
+          ++g;
+        }
+    const long long end = dbase[(size_t)(f + 1) * nrow];
+    if (g != end) return set_error(c, VGPU_EINVAL, "Debye site numbering (component %d count)", f);
+  }
+  c->d_dbase = upload_vec(c, dbase);
+
   c->d_dcomp = upload_vec(c, comp);
   c->d_dcell = upload_vec(c, cell);
   c->d_dmed = upload_vec(c, med);
@@ -449,7 +593,30 @@ static int build_debye(Ctx *c)
   c->d_dalpha = upload_vec(c, da);
   c->d_dbeta = upload_vec(c, db);
   c->d_dnp = upload_vec(c, np);
-  if (!c->d_dlast || !c->d_dJ) return set_error(c, VGPU_ENOMEM, "Debye state allocation");
+  if (!c->d_dlast || !c->d_dJ || !c->d_dbase) return set_error(c, VGPU_ENOMEM, "Debye state allocation");
+  return VGPU_OK;
+}
+
+// Upload the coefficient classes when both tables fit 16-bit indices.
+static int build_class_tables(Ctx *c)
+{
+  c->nEcls = (int)(c->etab.size() / 6);
+  c->nHcls = (int)(c->htab.size() / 3);
+  c->indexed = c->nEcls <= 65536 && c->nHcls <= 65536 && getenv("VGPU_NO_INDEXED_MEDIA") == nullptr;
+  if (c->indexed) {
+    std::vector<uint16_t> e(c->ncell), h(c->ncell);
+    for (size_t q = 0; q < c->ncell; ++q) { e[q] = (uint16_t)c->ecls[q]; h[q] = (uint16_t)c->hcls[q]; }
+    c->d_ecls = upload_vec(c, e);
+    c->d_hcls = upload_vec(c, h);
+    c->d_etab = upload_reals(c, c->etab);
+    c->d_htab = upload_reals(c, c->htab);
+    c->d_edslot = upload_vec(c, c->edslot);
+    if (!c->d_ecls || !c->d_hcls || !c->d_etab || !c->d_htab || !c->d_edslot)
+      return set_error(c, VGPU_ENOMEM, "coefficient class allocation");
+  }
+  c->debyeFused = c->indexed && c->debyeSites > 0 && getenv("VGPU_NO_DEBYE_FUSION") == nullptr;
+  std::vector<uint32_t>().swap(c->ecls);
+  std::vector<uint32_t>().swap(c->hcls);
   return VGPU_OK;
 }
 
@@ -811,11 +978,44 @@ int vgpu_finalize(vgpu_ctx *c, uint64_t total_steps, uint64_t max_steps_per_run)
   if (!c->media_set) return set_error(c, VGPU_ESTATE, "vgpu_set_media was not called");
   c->totalSteps = total_steps;
   c->maxRun = max_steps_per_run ? max_steps_per_run : total_steps;
+  c->d_step = dev_alloc(c, 64);
+  if (!c->d_step) return set_error(c, VGPU_ENOMEM, "step counter");
   int r;
-  if ((r = build_debye(c)) || (r = build_sibc(c)) || (r = build_corrections(c)) || (r = build_observers(c))) return r;
+  if ((r = build_debye(c)) || (r = build_class_tables(c)) || (r = build_sibc(c)) || (r = build_corrections(c)) ||
+      (r = build_observers(c)))
+    return r;
   if (hipDeviceSynchronize() != hipSuccess) return set_error(c, VGPU_EDEVICE, "device error during set-up");
   c->finalized = true;
   return VGPU_OK;
+}
+
+// Capture a graph of `nsteps` whole steps (plus the counter advance) on the
+// context's stream.  On any failure the context keeps launching directly.
+static void capture_graph(Ctx *c, uint64_t nsteps)
+{
+  c->graphTried = true;
+  hipStream_t s = c->stream;
+  if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    (void)hipGetLastError();
+    return;
+  }
+  const int r = c->prec == VGPU_FP64 ? run_steps<double>(c, nsteps, s, false) : run_steps<float>(c, nsteps, s, false);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(s, &g);
+  if (r != VGPU_OK || e != hipSuccess || !g) {
+    if (g) hipGraphDestroy(g);
+    (void)hipGetLastError();
+    return;
+  }
+  hipGraphExec_t x = nullptr;
+  if (hipGraphInstantiate(&x, g, nullptr, nullptr, 0) != hipSuccess || !x) {
+    hipGraphDestroy(g);
+    (void)hipGetLastError();
+    return;
+  }
+  c->graph = g;
+  c->graphExec = x;
+  c->graphSteps = nsteps;
 }
 
 int vgpu_run(vgpu_ctx *c, uint64_t first, uint64_t nsteps, void *stream)
@@ -827,8 +1027,26 @@ int vgpu_run(vgpu_ctx *c, uint64_t first, uint64_t nsteps, void *stream)
   (void)hipGetLastError();  // launch errors reported below belong to this run only
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
   hipEventRecord(c->ev0, s);
-  int r = c->prec == VGPU_FP64 ? run_steps<double>(c, first, nsteps, s) : run_steps<float>(c, first, nsteps, s);
+  if (c->stepHost != first)
+    hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, s, (unsigned long long *)c->d_step,
+                       (unsigned long long)first);
+  // Runs of the staging length are replayed from a captured graph once the
+  // solve is long enough to replay it at least twice (not when the kernels
+  // are being timed individually).
+  const bool graphable = s == c->stream && c->evE.empty() && nsteps >= 2 && nsteps == c->maxRun &&
+                         c->totalSteps >= 2 * nsteps && getenv("VGPU_NO_GRAPHS") == nullptr;
+  int r = VGPU_OK;
+  if (graphable && !c->graphExec && !c->graphTried) capture_graph(c, nsteps);
+  if (graphable && c->graphExec && c->graphSteps == nsteps) {
+    if (hipGraphLaunch(c->graphExec, s) != hipSuccess)
+      r = set_error(c, VGPU_EDEVICE, "graph launch failed: %s", hipGetErrorString(hipGetLastError()));
+    c->lastRunTimed = false;
+  } else {
+    r = c->prec == VGPU_FP64 ? run_steps<double>(c, nsteps, s, true) : run_steps<float>(c, nsteps, s, true);
+    c->lastRunTimed = true;
+  }
   hipEventRecord(c->ev1, s);
+  c->stepHost = first + nsteps;
   c->lastRunFirst = first;
   c->lastRunSteps = nsteps;
   return r;
@@ -846,7 +1064,7 @@ int vgpu_synchronize(vgpu_ctx *c)
   int ne = 0;
   // only the pairs the last run recorded: querying an unrecorded event fails
   // and would leave a stale error behind for the next launch check
-  const size_t rec = 2 * std::min<size_t>(c->evE.size() / 2, (size_t)c->lastRunSteps);
+  const size_t rec = c->lastRunTimed ? 2 * std::min<size_t>(c->evE.size() / 2, (size_t)c->lastRunSteps) : 0;
   for (size_t q = 0; q + 1 < rec; q += 2) {
     float a = 0, b = 0;
     if (hipEventElapsedTime(&a, c->evE[q], c->evE[q + 1]) == hipSuccess &&
@@ -857,6 +1075,15 @@ int vgpu_synchronize(vgpu_ctx *c)
   c->last_e_ms = ne ? se / ne : 0;
   c->last_h_ms = ne ? sh / ne : 0;
   return VGPU_OK;
+}
+
+int vgpu_query(vgpu_ctx *c)
+{
+  if (!c || !c->finalized) return VGPU_EINVAL;
+  const hipError_t e = hipEventQuery(c->ev1);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  return set_error(c, VGPU_EDEVICE, "device error: %s", hipGetErrorString(e));
 }
 
 int vgpu_read_point_observer(vgpu_ctx *c, int id, double *td, uint64_t nsteps, double *dft_re, double *dft_im)
@@ -942,6 +1169,8 @@ void vgpu_destroy(vgpu_ctx *c)
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
   for (void *p : c->allocs) hipFree(p);
+  if (c->graphExec) hipGraphExecDestroy(c->graphExec);
+  if (c->graph) hipGraphDestroy(c->graph);
   for (auto e : c->evE) hipEventDestroy(e);
   for (auto e : c->evH) hipEventDestroy(e);
   if (c->ev0) hipEventDestroy(c->ev0);

@@ -21,6 +21,8 @@ template <typename T> struct EArgs {
   int NX, NY, NZ, PX;
   long long plane;
   int k0, kchunk;
+  int x0;                // x of the first lane of x chunk 0 (<= 0): chunks start on the fast box
+  int fast[6];           // box where every component takes the ordinary update
   T *E[3];
   const T *H[3];
   const T *alpha[3], *beta[3];
@@ -30,12 +32,27 @@ template <typename T> struct EArgs {
   int pl[6][3][6];       // UPML slab limits [region][component][face]
   T *PP[6][3], *P[6][3]; // UPML auxiliaries, x-fastest within each slab box
   const T *ad[3], *bd[3], *ah[3], *ibh[3];
+  // coefficient classes: per-cell class, [class][6] alpha/beta, [class][3] Debye medium
+  const uint16_t *ecls;
+  const T *etab;
+  const int *edslot;
+  // layer-0 Debye sites applied in the sweep
+  int dfuse;
+  const int *dbase;      // [3][NZ][NY][NCH] first site of each 64-cell chunk
+  int NCH;
+  T *dlast, *dJ;
+  long long dstride;
+  const double *dalpha, *dbeta;
+  const int *dnp;
+  int dmaxP, dcplx;
 };
 
 template <typename T> struct HArgs {
   int NX, NY, NZ, PX;
   long long plane;
   int k0, kchunk;
+  int x0;
+  int fast[6];
   T *H[3];
   const T *E[3];
   const T *gamma[3];
@@ -45,6 +62,8 @@ template <typename T> struct HArgs {
   int pl[6][3][6];
   T *B[6][3];
   const T *ad[3], *ah[3], *bh[3], *ibd[3];
+  const uint16_t *hcls;  // coefficient classes: per-cell class, [class][3] gamma
+  const T *htab;
 };
 
 // Mur tangential-E faces (mur.c:135-395): one box per (face, component).
@@ -77,6 +96,8 @@ template <typename T> struct DebyeArgs {
   const int *npoles;     // [medium]
   int maxPoles;
   int cplx;
+  const uint16_t *ecls;  // coefficient classes (beta from etab when set)
+  const T *etab;
 };
 
 // SIBC faces (sibc.c:443-586): per-face geometry, per-surface constants.
@@ -168,7 +189,9 @@ template <typename T> struct CorrArgs {
   const T *alpha[3];     // E only
   const T *coef[3];      // beta (E) or gamma (H)
   int electric;
-  double t;              // timeE or timeH
+  const unsigned long long *step;  // device step counter: n = *step + soff
+  int soff;
+  double dt, half;       // t = (n + half) * dt: timeE (half 0) or timeH (half 0.5)
   const WaveformDev *wf;
   const SourceDev *src;
   const PwDev *pw;
@@ -185,7 +208,9 @@ template <typename T> struct AuxArgs {
   const T *prof;         // [pw][4][10]: adx, bdx, ahx, bhx
   int auxmax;
   const WaveformDev *wf;
-  double t;
+  const unsigned long long *step;  // t = (*step + soff) * dt (timeE)
+  int soff;
+  double dt;
 };
 
 // Point observers and DFT (observer.c:622-650, 857-899).
@@ -200,8 +225,9 @@ template <typename T> struct ObsArgs {
   unsigned long long tdsteps;
   double *dre, *dim;
   const T *F[6];
-  unsigned long long n;
-  double t;
+  const unsigned long long *step;  // n = *step + soff, t = n * dt
+  int soff;
+  double dt;
 };
 
 template <typename T> struct VolObsArgs {
@@ -210,9 +236,9 @@ template <typename T> struct VolObsArgs {
   long long nnodes;
   const unsigned long long *first, *last;
   float *stage;              // [run step][node][6]
-  unsigned long long slot;
   const T *F[6];
-  unsigned long long n;
+  const unsigned long long *step;  // n = *step + soff; staging slot soff
+  int soff;
 };
 
 template <typename T> struct GhostArgs {
@@ -250,9 +276,27 @@ struct Ctx {
   void *F[6] = {};
   void *alpha[3] = {}, *beta[3] = {}, *gamma[3] = {};
   void *ide[3] = {}, *idh[3] = {};
-  // host copies kept for building surface lists
-  std::vector<double> h_beta[3], h_gamma[3], h_alpha[3];
   bool media_set = false;
+
+  // Coefficient classes (the reference's indexed-media representation,
+  // grid.h:108-135): every cell's (alpha, beta) triple of E components and
+  // gamma triple of H components is one entry of a small table.  Built on
+  // the host in vgpu_set_media, refined with the layer-0 Debye block of each
+  // E component in vgpu_finalize, uploaded as uint16 per cell when both
+  // tables fit.
+  std::vector<uint32_t> ecls, hcls;          // host, per device cell
+  std::vector<double> etab, htab;            // [class][6] alpha/beta, [class][3] gamma
+  std::vector<int> edslot;                   // [class][3] layer-0 Debye medium or -1
+  bool indexed = false;
+  void *d_ecls = nullptr, *d_hcls = nullptr, *d_etab = nullptr, *d_htab = nullptr, *d_edslot = nullptr;
+  int nEcls = 0, nHcls = 0;
+  // layer-0 Debye sites handled inside the E sweep
+  bool debyeFused = false;
+  long long debyeL0 = 0;
+  int NCH = 0;                               // 64-cell x chunks per row
+  int X0 = 0;                                // x of the first cell of chunk 0 (<= 0)
+  int fastE[6] = {}, fastH[6] = {};          // boxes of the sweeps' ordinary-update fast path
+  void *d_dbase = nullptr;                   // [3][NZ][NY][NCH] + 1: first layer-0 site
 
   // UPML
   unsigned pmlmask = 0;
@@ -308,6 +352,15 @@ struct Ctx {
   struct VObs { int box[6]; int step[3]; uint64_t first, last; long long nodes; };
   std::vector<VObs> vobs;
   uint64_t totalSteps = 0, maxRun = 0, lastRunSteps = 0, lastRunFirst = 0;
+  // device step counter (read by the time-dependent kernels) and the host's
+  // view of its value once the enqueued work has run
+  void *d_step = nullptr;
+  uint64_t stepHost = 0;
+  // a captured graph of graphSteps whole steps (replayed by vgpu_run)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graphExec = nullptr;
+  uint64_t graphSteps = 0;
+  bool graphTried = false, lastRunTimed = false;
   void *d_ocell = nullptr, *d_ofirst = nullptr, *d_olast = nullptr, *d_onf = nullptr, *d_ooff = nullptr;
   void *d_oomega = nullptr, *d_otd = nullptr, *d_odre = nullptr, *d_odim = nullptr;
   long long vnodes = 0;
@@ -321,6 +374,8 @@ int set_error(Ctx *c, int code, const char *fmt, ...);
 void *dev_alloc(Ctx *c, size_t bytes);
 
 // Per-precision step driver, instantiated for double and float.
-template <typename T> int run_steps(Ctx *c, uint64_t first, uint64_t nsteps, hipStream_t s);
+// Enqueues nsteps steps that start at the device step counter, then advances
+// it; with timeKernels the volume sweeps of the first steps are timed.
+template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool timeKernels);
 
 }  // namespace vgpu

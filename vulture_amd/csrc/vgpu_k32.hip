@@ -1,4 +1,4 @@
 // FP32 instantiation of the step driver (fields, coefficients, UPML and Debye
 // state in single precision; SIBC state and DFT accumulators stay FP64).
 #include "vgpu_run.h"
-template int vgpu::run_steps<float>(vgpu::Ctx *, uint64_t, uint64_t, hipStream_t);
+template int vgpu::run_steps<float>(vgpu::Ctx *, uint64_t, hipStream_t, bool);

@@ -1,4 +1,4 @@
 // FP64 instantiation of the step driver.  Built with -ffp-contract=off so the
 // kernels perform the reference's double-precision operations one by one.
 #include "vgpu_run.h"
-template int vgpu::run_steps<double>(vgpu::Ctx *, uint64_t, uint64_t, hipStream_t);
+template int vgpu::run_steps<double>(vgpu::Ctx *, uint64_t, hipStream_t, bool);

@@ -26,105 +26,233 @@ template <typename T> VG_DEV long long slab_index(const int *b, int i, int j, in
   return ((long long)(k - b[4]) * nj + (j - b[2])) * ni + (i - b[0]);
 }
 
-// ---------------------------------------------------------------------------
-// E volume sweep: inner grid (grid.c:709-797) and UPML slabs (pml.c:859-1023)
-// ---------------------------------------------------------------------------
-template <typename T> __global__ __launch_bounds__(256) void k_volume_e(EArgs<T> a)
+// Region of component C at (i,j,k): 6 = ordinary update, 0..5 = UPML slab
+// (then p is the cell's offset in that slab), -1 = not updated by the sweep.
+// The loops are unrolled so every box and pointer is a compile-time field of
+// the kernel arguments (a runtime index into them would spill the argument
+// block to scratch).
+template <int NB, int C, typename A>
+VG_DEV int classify(const A &a, const int (*in)[6], int i, int j, int k, long long &p)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+#pragma unroll
+  for (int b = 0; b < NB; ++b)
+    if (in_box(in[b], i, j, k)) return 6;
+  if (a.pmlmask) {
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+      if (((a.pmlmask >> r) & 1u) && in_box(a.pl[r][C], i, j, k)) {
+        p = slab_index<int>(a.pl[r][C], i, j, k);
+        return r;
+      }
+  }
+  return -1;
+}
+
+// Slab array of region reg (0..5) for component C, selected without a runtime
+// index into the argument block.
+template <int C, typename T> VG_DEV T *slab_ptr(T *const (&arr)[6][3], int reg)
+{
+  T *q = nullptr;
+#pragma unroll
+  for (int r = 0; r < 6; ++r)
+    if (reg == r) q = arr[r][C];
+  return q;
+}
+
+// Debye ADE correction of one site (debye.c:243-307); shared by the fused E
+// sweep (layer 0) and k_debye (later layers).  g is the site, m its medium.
+template <typename T>
+VG_DEV T debye_site(T e, T beta, long long g, int m, T *__restrict__ lastE, T *__restrict__ J, long long stride,
+                    const double *__restrict__ dalpha, const double *__restrict__ dbeta, const int *__restrict__ npoles,
+                    int maxPoles, int cplx)
+{
+  const int np = npoles[m];
+  const double *da = dalpha + (long long)m * maxPoles * 2;
+  const double *db = dbeta + (long long)m * maxPoles * 2;
+  if (!cplx) {
+    T jsum = T(0);
+    for (int p = 0; p < np; ++p) jsum = jsum + (T(1) + T(da[2 * p])) * J[p * stride + g];
+    e = e - beta * jsum;
+    const T de = e - lastE[g];
+    for (int p = 0; p < np; ++p) {
+      T *Jp = &J[p * stride + g];
+      *Jp = T(da[2 * p]) * *Jp + T(db[2 * p]) * de;
+    }
+  } else {
+    T sr = T(0);
+    for (int p = 0; p < np; ++p) {
+      const T ar = T(1) + T(da[2 * p]), ai = T(da[2 * p + 1]);
+      const T jr = J[(p * stride + g) * 2], ji = J[(p * stride + g) * 2 + 1];
+      sr = sr + (ar * jr - ai * ji);
+    }
+    e = e - beta * sr;
+    const T de = e - lastE[g];
+    for (int p = 0; p < np; ++p) {
+      T *Jp = &J[(p * stride + g) * 2];
+      const T jr = Jp[0], ji = Jp[1];
+      const T dar = T(da[2 * p]), dai = T(da[2 * p + 1]), dbr = T(db[2 * p]), dbi = T(db[2 * p + 1]);
+      Jp[0] = (dar * jr - dai * ji) + dbr * de;
+      Jp[1] = (dar * ji + dai * jr) + dbi * de;
+    }
+  }
+  lastE[g] = e;
+  return e;
+}
+
+// Layer-0 Debye correction inside the E sweep.  The chunk's first site and
+// the next chunk's come from dbase (uniform over the wavefront, so rows
+// without Debye sites skip this entirely); a site's number is the chunk's
+// first site plus the count of lower lanes whose class carries a Debye
+// medium for C.  On lanes holding a site, e becomes the corrected value (from
+// E[id] when the sweep did not update the cell, have == false) and the
+// function returns true.
+template <typename T, int C>
+VG_DEV bool debye_fused(const EArgs<T> &a, const int n3[3], bool vx, long long id, unsigned cls, T be, bool have, T &e)
+{
+  const long long row = ((long long)(C * a.NZ + n3[2]) * a.NY + n3[1]) * a.NCH + blockIdx.x;
+  const int g0 = a.dbase[row];
+  if (a.dbase[row + 1] == g0) return false;
+  const int ds = vx ? a.edslot[cls * 3 + C] : -1;
+  const unsigned long long m = __ballot(ds >= 0);
+  if (ds < 0) return false;
+  const long long g = g0 + __popcll(m & ((1ull << __lane_id()) - 1ull));
+  if (!have) e = a.E[C][id];
+  e = debye_site<T>(e, be, g, ds, a.dlast, a.dJ, a.dstride, a.dalpha, a.dbeta, a.dnp, a.dmaxP, a.dcplx);
+  return true;
+}
+
+// One E component of one cell: ordinary update, or UPML split update with the
+// axis roles of component C (pml.c:859-1023): P along axis C+1, E along C+2.
+// With IDX the coefficients come from the class table; layer-0 Debye sites
+// are corrected in place (numbered by a wavefront ballot in (component, cell)
+// order, matching the site list).
+template <typename T, bool IDX, int C, typename Curl>
+VG_DEV void e_component(const EArgs<T> &a, const int n3[3], bool vx, long long id, unsigned cls, Curl curlf)
+{
+  constexpr int A1 = (C + 1) % 3, A2 = (C + 2) % 3;
+  long long p = 0;
+  const int reg = vx ? classify<1, C>(a, &a.in[C], n3[0], n3[1], n3[2], p) : -1;
+  T *__restrict__ E = a.E[C];
+  T al = T(0), be = T(0);
+  if (vx) {
+    if (IDX) {
+      al = a.etab[cls * 6 + 2 * C];
+      be = a.etab[cls * 6 + 2 * C + 1];
+    } else if (reg >= 0) {
+      al = a.alpha[C][id];
+      be = a.beta[C][id];
+    }
+  }
+  T e = T(0);
+  bool write = false;
+  if (reg == 6) {
+    e = al * E[id] + be * curlf();
+    write = true;
+  } else if (reg >= 0) {
+    const T curl = curlf();
+    T *PP = slab_ptr<C>(a.PP, reg), *PA = slab_ptr<C>(a.P, reg);
+    const T oldPP = PP[p];
+    const T pp = al * oldPP + be * curl;
+    PP[p] = pp;
+    const T oldP = PA[p];
+    const T pn = a.ad[A1][n3[A1]] * oldP + a.bd[A1][n3[A1]] * (pp - oldPP);
+    PA[p] = pn;
+    e = a.ad[A2][n3[A2]] * E[id] + a.bd[A2][n3[A2]] * a.ibh[C][n3[C]] * (pn - a.ah[C][n3[C]] * oldP);
+    write = true;
+  }
+  if (IDX && a.dfuse && debye_fused<T, C>(a, n3, vx, id, cls, be, write, e)) write = true;
+  if (write) E[id] = e;
+}
+
+// Fast path of one E component: every lane of the wavefront is inside the
+// inner grid of all three components (ordinary update only).
+template <typename T, bool IDX, int C>
+VG_DEV void e_fast(const EArgs<T> &a, const int n3[3], bool vx, long long id, unsigned cls, T curl)
+{
+  T *__restrict__ E = a.E[C];
+  T al, be;
+  if (IDX) {
+    al = a.etab[cls * 6 + 2 * C];
+    be = a.etab[cls * 6 + 2 * C + 1];
+  } else {
+    al = a.alpha[C][id];
+    be = a.beta[C][id];
+  }
+  T e = al * E[id] + be * curl;
+  if (IDX && a.dfuse) debye_fused<T, C>(a, n3, vx, id, cls, be, true, e);
+  if (vx) E[id] = e;
+}
+
+
+// ---------------------------------------------------------------------------
+// E volume sweep: inner grid (grid.c:709-797), UPML slabs (pml.c:859-1023)
+// and, with coefficient classes, the first Debye layer (debye.c:243-307).
+// One wavefront is one 64-cell x segment of one y row.
+// ---------------------------------------------------------------------------
+template <typename T, bool IDX> __global__ __launch_bounds__(256) void k_volume_e(EArgs<T> a)
+{
+  const int x0 = a.x0 + (int)blockIdx.x * 64;
+  const int i = x0 + (int)threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  if (i >= a.NX || j >= a.NY) return;
+  if (j >= a.NY) return;  // whole wavefront
+  const bool vx = i >= 0 && i < a.NX;
+  const int ic = min(max(i, 0), a.NX - 1);
   const int kb = a.k0 + blockIdx.z * a.kchunk;
   const int ke = min(kb + a.kchunk, a.NZ);
   if (kb >= ke) return;
+  // Wavefront-uniform: do the valid lanes of this row lie in the fast box?
+  const bool fastxy = max(x0, 0) >= a.fast[0] && min(x0 + 63, a.NX - 1) <= a.fast[1] && j >= a.fast[2] &&
+                      j <= a.fast[3];
   const long long P = a.plane, SJ = a.PX;
   const T *__restrict__ Hx = a.H[0];
   const T *__restrict__ Hy = a.H[1];
   const T *__restrict__ Hz = a.H[2];
-  const T idhx = a.idh[0][i], idhy = a.idh[1][j];
-  long long id = ((long long)kb * a.NY + j) * SJ + i;
+  const T idhx = a.idh[0][ic], idhy = a.idh[1][j];
+  long long id = ((long long)kb * a.NY + j) * SJ + ic;
   // H at the plane below, carried between iterations.
   T hxm = (kb > 0) ? Hx[id - P] : T(0), hym = (kb > 0) ? Hy[id - P] : T(0);
   for (int k = kb; k < ke; ++k, id += P) {
     const T hx = Hx[id], hy = Hy[id], hz = Hz[id];
     const T idhz = a.idh[2][k];
-    // ---- Ex ----
-    {
-      const int c = 0;
-      int reg = in_box(a.in[c], i, j, k) ? 6 : -1;
-      if (reg < 0 && a.pmlmask)
-        for (int r = 0; r < 6; ++r)
-          if (((a.pmlmask >> r) & 1u) && in_box(a.pl[r][c], i, j, k)) { reg = r; break; }
-      if (reg >= 0) {
-        const T curl = idhy * (hz - Hz[id - SJ]) + idhz * (hym - hy);
-        T *__restrict__ E = a.E[c];
-        if (reg == 6) {
-          E[id] = a.alpha[c][id] * E[id] + a.beta[c][id] * curl;
-        } else {
-          const long long p = slab_index<T>(a.pl[reg][c], i, j, k);
-          T *PP = a.PP[reg][c], *PA = a.P[reg][c];
-          const T oldPP = PP[p];
-          const T pp = a.alpha[c][id] * oldPP + a.beta[c][id] * curl;
-          PP[p] = pp;
-          const T oldP = PA[p];
-          const T pn = a.ad[1][j] * oldP + a.bd[1][j] * (pp - oldPP);
-          PA[p] = pn;
-          E[id] = a.ad[2][k] * E[id] + a.bd[2][k] * a.ibh[0][i] * (pn - a.ah[0][i] * oldP);
-        }
-      }
-    }
-    // ---- Ey ----
-    {
-      const int c = 1;
-      int reg = in_box(a.in[c], i, j, k) ? 6 : -1;
-      if (reg < 0 && a.pmlmask)
-        for (int r = 0; r < 6; ++r)
-          if (((a.pmlmask >> r) & 1u) && in_box(a.pl[r][c], i, j, k)) { reg = r; break; }
-      if (reg >= 0) {
-        const T curl = idhz * (hx - hxm) + idhx * (Hz[id - 1] - hz);
-        T *__restrict__ E = a.E[c];
-        if (reg == 6) {
-          E[id] = a.alpha[c][id] * E[id] + a.beta[c][id] * curl;
-        } else {
-          const long long p = slab_index<T>(a.pl[reg][c], i, j, k);
-          T *PP = a.PP[reg][c], *PA = a.P[reg][c];
-          const T oldPP = PP[p];
-          const T pp = a.alpha[c][id] * oldPP + a.beta[c][id] * curl;
-          PP[p] = pp;
-          const T oldP = PA[p];
-          const T pn = a.ad[2][k] * oldP + a.bd[2][k] * (pp - oldPP);
-          PA[p] = pn;
-          E[id] = a.ad[0][i] * E[id] + a.bd[0][i] * a.ibh[1][j] * (pn - a.ah[1][j] * oldP);
-        }
-      }
-    }
-    // ---- Ez ----
-    {
-      const int c = 2;
-      int reg = in_box(a.in[c], i, j, k) ? 6 : -1;
-      if (reg < 0 && a.pmlmask)
-        for (int r = 0; r < 6; ++r)
-          if (((a.pmlmask >> r) & 1u) && in_box(a.pl[r][c], i, j, k)) { reg = r; break; }
-      if (reg >= 0) {
-        const T curl = idhx * (hy - Hy[id - 1]) + idhy * (Hx[id - SJ] - hx);
-        T *__restrict__ E = a.E[c];
-        if (reg == 6) {
-          E[id] = a.alpha[c][id] * E[id] + a.beta[c][id] * curl;
-        } else {
-          const long long p = slab_index<T>(a.pl[reg][c], i, j, k);
-          T *PP = a.PP[reg][c], *PA = a.P[reg][c];
-          const T oldPP = PP[p];
-          const T pp = a.alpha[c][id] * oldPP + a.beta[c][id] * curl;
-          PP[p] = pp;
-          const T oldP = PA[p];
-          const T pn = a.ad[0][i] * oldP + a.bd[0][i] * (pp - oldPP);
-          PA[p] = pn;
-          E[id] = a.ad[1][j] * E[id] + a.bd[1][j] * a.ibh[2][k] * (pn - a.ah[2][k] * oldP);
-        }
-      }
+    const unsigned cls = (IDX && vx) ? (unsigned)a.ecls[id] : 0u;
+    const int n3[3] = {i, j, k};
+    if (fastxy && k >= a.fast[4] && k <= a.fast[5]) {
+      // i-1 and j-1 are inside the grid for every lane here (the fast box
+      // starts at 1 and invalid lanes read a clamped in-row cell)
+      e_fast<T, IDX, 0>(a, n3, vx, id, cls, idhy * (hz - Hz[id - SJ]) + idhz * (hym - hy));
+      e_fast<T, IDX, 1>(a, n3, vx, id, cls, idhz * (hx - hxm) + idhx * (Hz[id - 1] - hz));
+      e_fast<T, IDX, 2>(a, n3, vx, id, cls, idhx * (hy - Hy[id - 1]) + idhy * (Hx[id - SJ] - hx));
+    } else {
+      // The curls read the i-1 / j-1 neighbours, so they are evaluated only
+      // for cells the sweep updates (never on the low ghost layer).
+      e_component<T, IDX, 0>(a, n3, vx, id, cls, [&]() { return idhy * (hz - Hz[id - SJ]) + idhz * (hym - hy); });
+      e_component<T, IDX, 1>(a, n3, vx, id, cls, [&]() { return idhz * (hx - hxm) + idhx * (Hz[id - 1] - hz); });
+      e_component<T, IDX, 2>(a, n3, vx, id, cls, [&]() { return idhx * (hy - Hy[id - 1]) + idhy * (Hx[id - SJ] - hx); });
     }
     hxm = hx;
     hym = hy;
+  }
+}
+
+// One H component: ordinary update (inner grid or Mur normal H), or UPML
+// split update (pml.c:1026-1176): B along axis C+1, H along C+2.
+template <typename T, bool IDX, int C>
+VG_DEV void h_component(const HArgs<T> &a, const int n3[3], long long id, unsigned cls, T curl)
+{
+  constexpr int A1 = (C + 1) % 3, A2 = (C + 2) % 3;
+  long long p = 0;
+  const int reg = classify<3, C>(a, a.in[C], n3[0], n3[1], n3[2], p);
+  if (reg < 0) return;
+  const T g = IDX ? a.htab[cls * 3 + C] : a.gamma[C][id];
+  T *__restrict__ H = a.H[C];
+  if (reg == 6) {
+    H[id] = H[id] + g * curl;
+  } else {
+    T *Bp = slab_ptr<C>(a.B, reg);
+    const T oldB = Bp[p];
+    const T bn = a.ah[A1][n3[A1]] * oldB + g * a.bh[A1][n3[A1]] * curl;
+    Bp[p] = bn;
+    H[id] = a.ah[A2][n3[A2]] * H[id] + a.bh[A2][n3[A2]] * a.ibd[C][n3[C]] * (bn - a.ad[C][n3[C]] * oldB);
   }
 }
 
@@ -132,102 +260,58 @@ template <typename T> __global__ __launch_bounds__(256) void k_volume_e(EArgs<T>
 // H volume sweep: inner grid (grid.c:800-888), Mur normal H (mur.c:401-555)
 // and UPML slabs (pml.c:1026-1176)
 // ---------------------------------------------------------------------------
-template <typename T> __global__ __launch_bounds__(256) void k_volume_h(HArgs<T> a)
+template <typename T, bool IDX> __global__ __launch_bounds__(256) void k_volume_h(HArgs<T> a)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int x0 = a.x0 + (int)blockIdx.x * 64;
+  const int i = x0 + (int)threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  if (i >= a.NX || j >= a.NY) return;
+  if (j >= a.NY) return;  // whole wavefront
+  const bool vx = i >= 0 && i < a.NX;
+  const int ic = min(max(i, 0), a.NX - 1);
   const int kb = a.k0 + blockIdx.z * a.kchunk;
   const int ke = min(kb + a.kchunk, a.NZ);
   if (kb >= ke) return;
+  const bool fastxy = max(x0, 0) >= a.fast[0] && min(x0 + 63, a.NX - 1) <= a.fast[1] && j >= a.fast[2] &&
+                      j <= a.fast[3];
   const long long P = a.plane, SJ = a.PX;
   const T *__restrict__ Ex = a.E[0];
   const T *__restrict__ Ey = a.E[1];
   const T *__restrict__ Ez = a.E[2];
-  const bool xin = i + 1 < a.NX, yin = j + 1 < a.NY;
-  const T idex = a.ide[0][i], idey = a.ide[1][j];
-  long long id = ((long long)kb * a.NY + j) * SJ + i;
+  const bool xin = ic + 1 < a.NX, yin = j + 1 < a.NY;
+  const T idex = a.ide[0][ic], idey = a.ide[1][j];
+  long long id = ((long long)kb * a.NY + j) * SJ + ic;
   T ex = Ex[id], ey = Ey[id];
   for (int k = kb; k < ke; ++k, id += P) {
     const bool zin = k + 1 < a.NZ;
     const T exp1 = zin ? Ex[id + P] : T(0), eyp1 = zin ? Ey[id + P] : T(0);
     const T ez = Ez[id];
     const T idez = a.ide[2][k];
-    // ---- Hx ----
-    {
-      const int c = 0;
-      int reg = -1;
-      for (int b = 0; b < 3; ++b)
-        if (in_box(a.in[c][b], i, j, k)) { reg = 6; break; }
-      if (reg < 0 && a.pmlmask)
-        for (int r = 0; r < 6; ++r)
-          if (((a.pmlmask >> r) & 1u) && in_box(a.pl[r][c], i, j, k)) { reg = r; break; }
-      if (reg >= 0) {
-        const T ezp = yin ? Ez[id + SJ] : T(0);
-        const T curl = idez * (eyp1 - ey) + idey * (ez - ezp);
-        T *__restrict__ H = a.H[c];
-        if (reg == 6) {
-          H[id] = H[id] + a.gamma[c][id] * curl;
-        } else {
-          const long long p = slab_index<T>(a.pl[reg][c], i, j, k);
-          T *Bp = a.B[reg][c];
-          const T oldB = Bp[p];
-          const T bn = a.ah[1][j] * oldB + a.gamma[c][id] * a.bh[1][j] * curl;
-          Bp[p] = bn;
-          H[id] = a.ah[2][k] * H[id] + a.bh[2][k] * a.ibd[0][i] * (bn - a.ad[0][i] * oldB);
-        }
+    const unsigned cls = (IDX && vx) ? (unsigned)a.hcls[id] : 0u;
+    const T ezp = yin ? Ez[id + SJ] : T(0);
+    const T ezx = xin ? Ez[id + 1] : T(0);
+    const T exy = yin ? Ex[id + SJ] : T(0);
+    const T eyx = xin ? Ey[id + 1] : T(0);
+    const T c0 = idez * (eyp1 - ey) + idey * (ez - ezp);
+    const T c1 = idex * (ezx - ez) + idez * (ex - exp1);
+    const T c2 = idey * (exy - ex) + idex * (ey - eyx);
+    if (fastxy && k >= a.fast[4] && k <= a.fast[5]) {
+      T *__restrict__ H0 = a.H[0];
+      T *__restrict__ H1 = a.H[1];
+      T *__restrict__ H2 = a.H[2];
+      const T g0 = IDX ? a.htab[cls * 3] : a.gamma[0][id];
+      const T g1 = IDX ? a.htab[cls * 3 + 1] : a.gamma[1][id];
+      const T g2 = IDX ? a.htab[cls * 3 + 2] : a.gamma[2][id];
+      const T h0 = H0[id] + g0 * c0, h1 = H1[id] + g1 * c1, h2 = H2[id] + g2 * c2;
+      if (vx) {
+        H0[id] = h0;
+        H1[id] = h1;
+        H2[id] = h2;
       }
-    }
-    // ---- Hy ----
-    {
-      const int c = 1;
-      int reg = -1;
-      for (int b = 0; b < 3; ++b)
-        if (in_box(a.in[c][b], i, j, k)) { reg = 6; break; }
-      if (reg < 0 && a.pmlmask)
-        for (int r = 0; r < 6; ++r)
-          if (((a.pmlmask >> r) & 1u) && in_box(a.pl[r][c], i, j, k)) { reg = r; break; }
-      if (reg >= 0) {
-        const T ezx = xin ? Ez[id + 1] : T(0);
-        const T curl = idex * (ezx - ez) + idez * (ex - exp1);
-        T *__restrict__ H = a.H[c];
-        if (reg == 6) {
-          H[id] = H[id] + a.gamma[c][id] * curl;
-        } else {
-          const long long p = slab_index<T>(a.pl[reg][c], i, j, k);
-          T *Bp = a.B[reg][c];
-          const T oldB = Bp[p];
-          const T bn = a.ah[2][k] * oldB + a.gamma[c][id] * a.bh[2][k] * curl;
-          Bp[p] = bn;
-          H[id] = a.ah[0][i] * H[id] + a.bh[0][i] * a.ibd[1][j] * (bn - a.ad[1][j] * oldB);
-        }
-      }
-    }
-    // ---- Hz ----
-    {
-      const int c = 2;
-      int reg = -1;
-      for (int b = 0; b < 3; ++b)
-        if (in_box(a.in[c][b], i, j, k)) { reg = 6; break; }
-      if (reg < 0 && a.pmlmask)
-        for (int r = 0; r < 6; ++r)
-          if (((a.pmlmask >> r) & 1u) && in_box(a.pl[r][c], i, j, k)) { reg = r; break; }
-      if (reg >= 0) {
-        const T exy = yin ? Ex[id + SJ] : T(0);
-        const T eyx = xin ? Ey[id + 1] : T(0);
-        const T curl = idey * (exy - ex) + idex * (ey - eyx);
-        T *__restrict__ H = a.H[c];
-        if (reg == 6) {
-          H[id] = H[id] + a.gamma[c][id] * curl;
-        } else {
-          const long long p = slab_index<T>(a.pl[reg][c], i, j, k);
-          T *Bp = a.B[reg][c];
-          const T oldB = Bp[p];
-          const T bn = a.ah[0][i] * oldB + a.gamma[c][id] * a.bh[0][i] * curl;
-          Bp[p] = bn;
-          H[id] = a.ah[1][j] * H[id] + a.bh[1][j] * a.ibd[2][k] * (bn - a.ad[2][k] * oldB);
-        }
-      }
+    } else if (vx) {
+      const int n3[3] = {i, j, k};
+      h_component<T, IDX, 0>(a, n3, id, cls, c0);
+      h_component<T, IDX, 1>(a, n3, id, cls, c1);
+      h_component<T, IDX, 2>(a, n3, id, cls, c2);
     }
     ex = exp1;
     ey = eyp1;
@@ -281,43 +365,10 @@ template <typename T> __global__ void k_debye(DebyeArgs<T> a)
   const long long g = a.base + s;
   const int c = a.comp[g];
   const long long id = a.cell[g];
-  const int m = a.medium[g];
-  const int np = a.npoles[m];
-  const double *da = a.dalpha + (long long)m * a.maxPoles * 2;
-  const double *db = a.dbeta + (long long)m * a.maxPoles * 2;
   T *E = a.E[c];
-  T e = E[id];
-  const T beta = a.beta[c][id];
-  if (!a.cplx) {
-    T jsum = T(0);
-    for (int p = 0; p < np; ++p) jsum = jsum + (T(1) + T(da[2 * p])) * a.J[p * a.stride + g];
-    e = e - beta * jsum;
-    const T de = e - a.lastE[g];
-    for (int p = 0; p < np; ++p) {
-      T *Jp = &a.J[p * a.stride + g];
-      *Jp = T(da[2 * p]) * *Jp + T(db[2 * p]) * de;
-    }
-  } else {
-    T sr = T(0), si = T(0);
-    for (int p = 0; p < np; ++p) {
-      const T ar = T(1) + T(da[2 * p]), ai = T(da[2 * p + 1]);
-      const T jr = a.J[(p * a.stride + g) * 2], ji = a.J[(p * a.stride + g) * 2 + 1];
-      sr = sr + (ar * jr - ai * ji);
-      si = si + (ar * ji + ai * jr);
-    }
-    (void)si;
-    e = e - beta * sr;
-    const T de = e - a.lastE[g];
-    for (int p = 0; p < np; ++p) {
-      T *Jp = &a.J[(p * a.stride + g) * 2];
-      const T jr = Jp[0], ji = Jp[1];
-      const T dar = T(da[2 * p]), dai = T(da[2 * p + 1]), dbr = T(db[2 * p]), dbi = T(db[2 * p + 1]);
-      Jp[0] = (dar * jr - dai * ji) + dbr * de;
-      Jp[1] = (dar * ji + dai * jr) + dbi * de;
-    }
-  }
-  E[id] = e;
-  a.lastE[g] = e;
+  const T beta = a.etab ? a.etab[(unsigned)a.ecls[id] * 6 + 2 * c + 1] : a.beta[c][id];
+  E[id] = debye_site<T>(E[id], beta, g, a.medium[g], a.lastE, a.J, a.stride, a.dalpha, a.dbeta, a.npoles, a.maxPoles,
+                        a.cplx);
 }
 
 // ---------------------------------------------------------------------------
@@ -498,7 +549,8 @@ template <typename T> __global__ void k_aux_e(AuxArgs<T> a, double pi)
       Eyi[i] = adx[L] * Eyi[i] + bdx[L] * (Py[L] - oldP);
     }
   }
-  if (threadIdx.x == 0) Eyi[0] = T(wf_value(a.wf[pw.waveform], a.t, pw.delay, pi));
+  const double t = (double)(*a.step + (unsigned long long)a.soff) * a.dt;
+  if (threadIdx.x == 0) Eyi[0] = T(wf_value(a.wf[pw.waveform], t, pw.delay, pi));
 }
 
 template <typename T> __global__ void k_aux_h(AuxArgs<T> a)
@@ -530,6 +582,7 @@ template <typename T> __global__ void k_corr(CorrArgs<T> a, double pi)
 {
   const long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (r >= a.nrows) return;
+  const double t = ((double)(*a.step + (unsigned long long)a.soff) + a.half) * a.dt;
   const int c = a.rowcomp[r];
   const long long id = a.rowcell[r];
   T *F = a.F[c];
@@ -540,7 +593,7 @@ template <typename T> __global__ void k_corr(CorrArgs<T> a, double pi)
     const CorrEntry &en = a.ent[e];
     if (en.kind <= 1) {
       const SourceDev &s = a.src[en.idx];
-      const T source = T(s.size * wf_value(a.wf[s.waveform], a.t, s.delay, pi));
+      const T source = T(s.size * wf_value(a.wf[s.waveform], t, s.delay, pi));
       if (en.kind == 0) f = T(s.soft) * f + source;
       else f = T(s.soft) * f - coef * source;
     } else {
@@ -554,7 +607,7 @@ template <typename T> __global__ void k_corr(CorrArgs<T> a, double pi)
         else value = T(a.eta0) * ((T(1) - dp) * Hzi[1 + en.auxid] + dp * Hzi[2 + en.auxid]);
         inc = T(pw.Finc[en.inc]) * value;
       } else {
-        inc = T(pw.Finc[en.inc]) * T(wf_value(a.wf[pw.waveform], a.t - en.geo, pw.delay, pi));
+        inc = T(pw.Finc[en.inc]) * T(wf_value(a.wf[pw.waveform], t - en.geo, pw.delay, pi));
       }
       const T term = coef * (T(en.il) * inc);
       if (a.electric) f = alpha * f + T(en.sign) * term;
@@ -573,18 +626,20 @@ template <typename T> __global__ void k_observe(ObsArgs<T> a)
 {
   const int o = blockIdx.y;
   if (o >= a.nobs) return;
-  if (a.n < a.first[o] || a.n > a.last[o]) return;
+  const unsigned long long n = *a.step + (unsigned long long)a.soff;
+  if (n < a.first[o] || n > a.last[o]) return;
+  const double t = (double)n * a.dt;
   const long long id = a.cell[o];
   double v[6];
   for (int c = 0; c < 6; ++c) v[c] = (double)a.F[c][id];
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f == 0) {
-    double *td = a.td + ((long long)o * a.tdsteps + (a.n - a.first[o])) * 6;
+    double *td = a.td + ((long long)o * a.tdsteps + (n - a.first[o])) * 6;
     for (int c = 0; c < 6; ++c) td[c] = v[c];
   }
   const int nf = a.nfreq[o];
   if (f >= nf) return;
-  const double ot = a.omega[a.dftoff[o] / 6 + f] * a.t;
+  const double ot = a.omega[a.dftoff[o] / 6 + f] * t;
   const double co = cos(ot), si = sin(ot);
   double *re = a.dre + a.dftoff[o], *im = a.dim + a.dftoff[o];
   for (int c = 0; c < 6; ++c) {
@@ -598,9 +653,10 @@ template <typename T> __global__ void k_observe_volume(VolObsArgs<T> a)
   const long long nd = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (nd >= a.nnodes) return;
   const int o = a.nodeobs[nd];
-  if (a.n < a.first[o] || a.n > a.last[o]) return;
+  const unsigned long long n = *a.step + (unsigned long long)a.soff;
+  if (n < a.first[o] || n > a.last[o]) return;
   const long long id = a.nodecell[nd];
-  float *out = a.stage + ((long long)a.slot * a.nnodes + nd) * 6;
+  float *out = a.stage + ((long long)a.soff * a.nnodes + nd) * 6;
   for (int c = 0; c < 6; ++c) out[c] = (float)a.F[c][id];
 }
 
@@ -619,6 +675,18 @@ template <typename T> __global__ void k_ghost(GhostArgs<T> a)
   const long long di = ((long long)d3[2] * a.NY + d3[1]) * a.PX + d3[0];
   const long long si = ((long long)s3[2] * a.NY + s3[1]) * a.PX + s3[0];
   a.F[di] = a.sign * a.F[si];
+}
+
+// Device step counter: set before a run that does not continue the previous
+// one, advanced after every run (a one-lane vector store).
+static __global__ void k_set_step(unsigned long long *step, unsigned long long v)
+{
+  if (threadIdx.x == 0) step[threadIdx.x] = v;
+}
+
+static __global__ void k_advance_step(unsigned long long *step, unsigned long long by)
+{
+  if (threadIdx.x == 0) step[threadIdx.x] = step[threadIdx.x] + by;
 }
 
 }  // namespace vgpu

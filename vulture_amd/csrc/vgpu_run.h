@@ -19,6 +19,8 @@ template <typename T> static void fill_volume_args(Ctx *c, EArgs<T> &ea, HArgs<T
   ea.plane = ha.plane = c->plane;
   ea.k0 = ha.k0 = 0;
   ea.kchunk = ha.kchunk = kchunk;
+  ea.x0 = ha.x0 = c->X0;
+  for (int q = 0; q < 6; ++q) { ea.fast[q] = c->fastE[q]; ha.fast[q] = c->fastH[q]; }
   for (int d = 0; d < 3; ++d) {
     ea.E[d] = P<T>(c->F[d]);
     ea.H[d] = P<T>(c->F[3 + d]);
@@ -52,6 +54,15 @@ template <typename T> static void fill_volume_args(Ctx *c, EArgs<T> &ea, HArgs<T
     ha.bh[d] = P<T>(c->prof[3][d]);
     ha.ibd[d] = P<T>(c->prof[4][d]);
   }
+  ea.ecls = P<uint16_t>(c->d_ecls); ea.etab = P<T>(c->d_etab); ea.edslot = P<int>(c->d_edslot);
+  ha.hcls = P<uint16_t>(c->d_hcls); ha.htab = P<T>(c->d_htab);
+  ea.dfuse = c->debyeFused ? 1 : 0;
+  if (c->debyeFused) {
+    ea.dbase = P<int>(c->d_dbase); ea.NCH = c->NCH;
+    ea.dlast = P<T>(c->d_dlast); ea.dJ = P<T>(c->d_dJ); ea.dstride = c->debyeSites;
+    ea.dalpha = P<double>(c->d_dalpha); ea.dbeta = P<double>(c->d_dbeta); ea.dnp = P<int>(c->d_dnp);
+    ea.dmaxP = c->debyeMaxPoles; ea.dcplx = c->debyeCplx;
+  }
   ea.pmlmask = ha.pmlmask = c->pmlmask;
   for (int r = 0; r < 6; ++r)
     for (int d = 0; d < 3; ++d) {
@@ -65,16 +76,19 @@ template <typename T> static void fill_volume_args(Ctx *c, EArgs<T> &ea, HArgs<T
     }
 }
 
-template <typename T> int run_steps(Ctx *c, uint64_t first, uint64_t nsteps, hipStream_t s)
+template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool timeKernels)
 {
   const double pi = 4.0 * atan(1.0);
   const double dt = c->grid.dt;
-  // Volume launch: 64x4 tiles, z chunk sized for >= ~4k blocks.
+  timeKernels = timeKernels && !c->evE.empty();
+  // Volume launch: 64x4 tiles, x chunks aligned to the fast box.
   const dim3 vblk(64, 4, 1);
-  const int gx = (c->NX + 63) / 64, gy = (c->NY + 3) / 4;
-  int kchunk = (int)((long long)c->NZ * gx * gy / 4096);
+  const int gx = c->NCH, gy = (c->NY + 3) / 4;
+  // z planes per block: short marches (about 16k blocks at 272^3) keep many
+  // rows in flight; at least 4 planes so the register carry pays.
+  int kchunk = (int)((long long)c->NZ * gx * gy / 16384);
   if (kchunk < 4) kchunk = 4;
-  if (kchunk > 64) kchunk = 64;
+  if (kchunk > 32) kchunk = 32;
   const dim3 vgrid(gx, gy, (c->NZ + kchunk - 1) / kchunk);
   EArgs<T> ea;
   HArgs<T> ha;
@@ -114,6 +128,7 @@ template <typename T> int run_steps(Ctx *c, uint64_t first, uint64_t nsteps, hip
     da.lastE = P<T>(c->d_dlast); da.J = P<T>(c->d_dJ); da.stride = c->debyeSites;
     da.dalpha = P<double>(c->d_dalpha); da.dbeta = P<double>(c->d_dbeta); da.npoles = P<int>(c->d_dnp);
     da.maxPoles = c->debyeMaxPoles; da.cplx = c->debyeCplx;
+    if (c->indexed) { da.ecls = P<uint16_t>(c->d_ecls); da.etab = P<T>(c->d_etab); }
   }
 
   SibcEArgs<T> se{};
@@ -206,25 +221,32 @@ template <typename T> int run_steps(Ctx *c, uint64_t first, uint64_t nsteps, hip
     }
   };
 
-  const bool timeKernels = !c->evE.empty();
+  // Time-dependent kernels read the step number from the device counter
+  // (step n = counter + offset within this run), so the same launches can be
+  // captured once in a graph and replayed for later runs.
+  unsigned long long *stepp = P<unsigned long long>(c->d_step);
+  oa.step = stepp; oa.dt = dt;
+  va.step = stepp;
+  xa.step = stepp; xa.dt = dt;
+  for (int e = 0; e < 2; ++e) { ce[e].step = stepp; ce[e].dt = dt; ce[e].half = e == 0 ? 0.0 : 0.5; }
   int ne = 0, nh = 0;
-  for (uint64_t n = first; n < first + nsteps; ++n) {
-    const double tE = (double)n * dt;
-    const double tH = ((double)n + 0.5) * dt;
+  for (uint64_t q = 0; q < nsteps; ++q) {
+    const int soff = (int)q;
     if (oa.nobs) {
-      oa.n = n; oa.t = tE;
+      oa.soff = soff;
       hipLaunchKernelGGL(k_observe<T>, dim3((maxnf + 63) / 64, oa.nobs), dim3(64), 0, s, oa);
     }
     if (va.nnodes) {
-      va.n = n; va.slot = n - first;
+      va.soff = soff;
       hipLaunchKernelGGL(k_observe_volume<T>, dim3((unsigned)((va.nnodes + 255) / 256)), dim3(256), 0, s, va);
     }
     if (c->murmask)
       hipLaunchKernelGGL(k_mur_e<T>, dim3((murmax1 + 15) / 16, (murmax2 + 15) / 16, 12), dim3(16, 16), 0, s, ma);
     if (timeKernels && ne < (int)c->evE.size() / 2) hipEventRecord(c->evE[2 * ne], s);
-    hipLaunchKernelGGL(k_volume_e<T>, vgrid, vblk, 0, s, ea);
+    if (c->indexed) hipLaunchKernelGGL((k_volume_e<T, true>), vgrid, vblk, 0, s, ea);
+    else hipLaunchKernelGGL((k_volume_e<T, false>), vgrid, vblk, 0, s, ea);
     if (timeKernels && ne < (int)c->evE.size() / 2) { hipEventRecord(c->evE[2 * ne + 1], s); ++ne; }
-    for (int l = 0; l < c->debyeLayers; ++l) {
+    for (int l = c->debyeFused ? 1 : 0; l < c->debyeLayers; ++l) {
       da.base = c->debyeLayerStart[l];
       da.n = c->debyeLayerStart[l + 1] - c->debyeLayerStart[l];
       hipLaunchKernelGGL(k_debye<T>, dim3((unsigned)((da.n + 255) / 256)), dim3(256), 0, s, da);
@@ -232,26 +254,28 @@ template <typename T> int run_steps(Ctx *c, uint64_t first, uint64_t nsteps, hip
     if (c->sibcFaces)
       hipLaunchKernelGGL(k_sibc_e<T>, dim3((unsigned)((se.nfaces + 127) / 128)), dim3(128), 0, s, se);
     if (anyAux) {
-      xa.t = tE;
+      xa.soff = soff;
       hipLaunchKernelGGL(k_aux_e<T>, dim3(xa.npw), dim3(256), 0, s, xa, pi);
     }
     if (ce[0].nrows) {
-      ce[0].t = tE;
+      ce[0].soff = soff;
       hipLaunchKernelGGL(k_corr<T>, dim3((unsigned)((ce[0].nrows + 255) / 256)), dim3(256), 0, s, ce[0], pi);
     }
     ghost(1);
     if (timeKernels && nh < (int)c->evH.size() / 2) hipEventRecord(c->evH[2 * nh], s);
-    hipLaunchKernelGGL(k_volume_h<T>, vgrid, vblk, 0, s, ha);
+    if (c->indexed) hipLaunchKernelGGL((k_volume_h<T, true>), vgrid, vblk, 0, s, ha);
+    else hipLaunchKernelGGL((k_volume_h<T, false>), vgrid, vblk, 0, s, ha);
     if (timeKernels && nh < (int)c->evH.size() / 2) { hipEventRecord(c->evH[2 * nh + 1], s); ++nh; }
     if (c->sibcFaces)
       hipLaunchKernelGGL(k_sibc_h<T>, dim3((unsigned)((sh.nrows + 255) / 256)), dim3(256), 0, s, sh);
     if (anyAux) hipLaunchKernelGGL(k_aux_h<T>, dim3(xa.npw), dim3(256), 0, s, xa);
     if (ce[1].nrows) {
-      ce[1].t = tH;
+      ce[1].soff = soff;
       hipLaunchKernelGGL(k_corr<T>, dim3((unsigned)((ce[1].nrows + 255) / 256)), dim3(256), 0, s, ce[1], pi);
     }
     ghost(0);
   }
+  hipLaunchKernelGGL(k_advance_step, dim3(1), dim3(64), 0, s, stepp, (unsigned long long)nsteps);
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return set_error(c, VGPU_EDEVICE, "kernel launch failed: %s", hipGetErrorString(err));
   return VGPU_OK;

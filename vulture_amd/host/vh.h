@@ -278,6 +278,9 @@ int vh_init(vh_solve *s);
 
 /* Observer helpers shared by the CPU restatement and the device driver. */
 int vh_open_observers(vh_solve *s);
+/* Free the volume arrays (fields, coefficients, UPML and Debye state) once a
+ * device holds them; observers, waveforms and grid lines stay. */
+void vh_release_grid_arrays(vh_solve *s);
 void vh_observe_step(vh_solve *s, unsigned long n, real t, const double *pointVals, const float *binVals);
 void vh_observer_point_values(const vh_solve *s, double *vals);
 void vh_close_observers(vh_solve *s);

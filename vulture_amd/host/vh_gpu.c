@@ -8,7 +8,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include "vh_internal.h"
-#include "../../include/vgpu.h"
+#include "vh_gpu.h"
 
 /* SIBC orientation matrices (sibc.c:72-110). */
 static const real Acp[4][4] = {{0, 0, 1, 0}, {0, 0, 0, -1}, {-1, 0, 0, 0}, {0, 1, 0, 0}};
@@ -187,65 +187,133 @@ bad:
   return NULL;
 }
 
-/* Run the whole time loop on the device and write the observer files.
- * loop_ms receives the device time of the time loop alone. */
-int vh_gpu_run(vh_solve *s, int prec, int device, unsigned long chunk, double *loop_ms)
+/* A solve on the device, advanced chunk by chunk: the context, the mapping
+ * of the host's observers to device observers, and the host buffer for
+ * binary volume samples (drained after every chunk). */
+struct vh_dsolve {
+  vh_solve *s;
+  vgpu_ctx *c;
+  unsigned long NT, chunk, next, inflight;
+  int *pid, *vid;
+  uint64_t *vnodes;
+  size_t volNodes;
+  float *vol;
+  double loop_ms;
+};
+
+void vh_dsolve_close(vh_dsolve *d)
+{
+  if (!d) return;
+  if (d->c) vgpu_destroy(d->c);
+  free(d->pid);
+  free(d->vid);
+  free(d->vnodes);
+  free(d->vol);
+  free(d);
+}
+
+/* Build the device context for a set-up solve (uploads everything).
+ * Returns NULL on error (message in s->err). */
+vh_dsolve *vh_dsolve_open(vh_solve *s, int prec, int device, unsigned long chunk)
 {
   unsigned long NT = s->numTimeSteps;
   if (chunk == 0 || chunk > NT) chunk = NT ? NT : 1;
-  vgpu_ctx *c = vh_gpu_setup(s, prec, device, chunk);
-  if (!c) return -1;
+  vh_dsolve *d = calloc(1, sizeof(*d));
+  if (!d) return NULL;
+  d->s = s;
+  d->NT = NT;
+  d->chunk = chunk;
+  d->c = vh_gpu_setup(s, prec, device, chunk);
+  if (!d->c) { free(d); return NULL; }
   /* Map observers to device ids in the order they were added. */
   int npt = 0, nvol = 0;
-  int *pid = calloc((size_t)s->numObs + 1, sizeof(int)), *vid = calloc((size_t)s->numObs + 1, sizeof(int));
+  d->pid = calloc((size_t)s->numObs + 1, sizeof(int));
+  d->vid = calloc((size_t)s->numObs + 1, sizeof(int));
+  d->vnodes = calloc((size_t)s->numObs + 1, sizeof(uint64_t));
   for (int o = 0; o < s->numObs; o++) {
-    pid[o] = vid[o] = -1;
+    d->pid[o] = d->vid[o] = -1;
     if (s->obs[o].quantity != OQ_EH) continue;
-    if (s->obs[o].format == OF_ASCII) pid[o] = npt++;
-    else vid[o] = nvol++;
+    if (s->obs[o].format == OF_ASCII) d->pid[o] = npt++;
+    else d->vid[o] = nvol++;
   }
-  /* Volume samples are drained after every chunk. */
-  size_t volNodes = 0;
-  uint64_t *vnodes = calloc((size_t)nvol + 1, sizeof(uint64_t));
   for (int o = 0; o < s->numObs; o++)
-    if (vid[o] >= 0) {
-      vgpu_read_volume_observer(c, vid[o], NULL, 0, &vnodes[vid[o]]);
-      volNodes += vnodes[vid[o]];
+    if (d->vid[o] >= 0) {
+      vgpu_read_volume_observer(d->c, d->vid[o], NULL, 0, &d->vnodes[d->vid[o]]);
+      d->volNodes += d->vnodes[d->vid[o]];
     }
-  float *vol = NULL;
-  if (volNodes) {
-    vol = malloc(volNodes * 6 * sizeof(float) * (NT ? NT : 1));
-    if (!vol) { snprintf(s->err, sizeof(s->err), "volume observer buffer too large"); vgpu_destroy(c); return -1; }
-  }
-  double total = 0.0;
-  for (unsigned long first = 0; first < NT; first += chunk) {
-    unsigned long n = NT - first < chunk ? NT - first : chunk;
-    int rc = vgpu_run(c, first, n, NULL);
-    if (!rc) rc = vgpu_synchronize(c);
-    if (rc) { fail(s, c, "vgpu_run", rc); vgpu_destroy(c); free(vol); return -1; }
-    double ms;
-    vgpu_last_run_timing(c, &ms, NULL, NULL);
-    total += ms;
-    if (volNodes) {
-      size_t off = 0;
-      for (int o = 0; o < s->numObs; o++)
-        if (vid[o] >= 0) {
-          float *tmp = malloc(vnodes[vid[o]] * 6 * sizeof(float) * n);
-          vgpu_read_volume_observer(c, vid[o], tmp, n, NULL);
-          for (unsigned long q = 0; q < n; q++)
-            memcpy(vol + ((first + q) * volNodes + off) * 6, tmp + q * vnodes[vid[o]] * 6,
-                   vnodes[vid[o]] * 6 * sizeof(float));
-          free(tmp);
-          off += vnodes[vid[o]];
-        }
+  if (d->volNodes) {
+    d->vol = malloc(d->volNodes * 6 * sizeof(float) * (NT ? NT : 1));
+    if (!d->vol) {
+      snprintf(s->err, sizeof(s->err), "volume observer buffer too large");
+      vh_dsolve_close(d);
+      return NULL;
     }
   }
-  if (loop_ms) *loop_ms = total;
-  /* Point observers: time series and device DFTs. */
+  return d;
+}
+
+vgpu_ctx *vh_dsolve_context(vh_dsolve *d) { return d->c; }
+unsigned long vh_dsolve_remaining(const vh_dsolve *d) { return d->NT - d->next; }
+
+/* Enqueue the next chunk of steps on the context's stream (asynchronous).
+ * Returns the steps enqueued (0 when the loop is complete), <0 on error. */
+long vh_dsolve_enqueue(vh_dsolve *d)
+{
+  if (d->inflight || d->next >= d->NT) return 0;
+  unsigned long n = d->NT - d->next < d->chunk ? d->NT - d->next : d->chunk;
+  int rc = vgpu_run(d->c, d->next, n, NULL);
+  if (rc) return fail(d->s, d->c, "vgpu_run", rc);
+  d->inflight = n;
+  return (long)n;
+}
+
+/* 1 when the chunk in flight has finished (or none is), 0 while it runs. */
+int vh_dsolve_ready(vh_dsolve *d)
+{
+  if (!d->inflight) return 1;
+  return vgpu_query(d->c);
+}
+
+/* Complete the chunk in flight: wait for it, add its device time and drain
+ * the volume-observer staging.  Returns 0 or <0 on error. */
+int vh_dsolve_complete(vh_dsolve *d)
+{
+  if (!d->inflight) return 0;
+  vh_solve *s = d->s;
+  int rc = vgpu_synchronize(d->c);
+  if (rc) return fail(s, d->c, "vgpu_run", rc);
+  double ms;
+  vgpu_last_run_timing(d->c, &ms, NULL, NULL);
+  d->loop_ms += ms;
+  unsigned long first = d->next, n = d->inflight;
+  if (d->volNodes) {
+    size_t off = 0;
+    for (int o = 0; o < s->numObs; o++)
+      if (d->vid[o] >= 0) {
+        const uint64_t vn = d->vnodes[d->vid[o]];
+        float *tmp = malloc(vn * 6 * sizeof(float) * n);
+        if (!tmp) { snprintf(s->err, sizeof(s->err), "volume observer staging"); return -1; }
+        vgpu_read_volume_observer(d->c, d->vid[o], tmp, n, NULL);
+        for (unsigned long q = 0; q < n; q++)
+          memcpy(d->vol + ((first + q) * d->volNodes + off) * 6, tmp + q * vn * 6, vn * 6 * sizeof(float));
+        free(tmp);
+        off += vn;
+      }
+  }
+  d->next += n;
+  d->inflight = 0;
+  return 0;
+}
+
+/* After the last chunk: read the point observers (time samples and device
+ * DFTs) and free the device context, so its memory is available at once. */
+int vh_dsolve_collect(vh_dsolve *d, double ***tdOut)
+{
+  vh_solve *s = d->s;
   unsigned long nout = s->stopTimeStep - s->startTimeStep + 1;
   double **td = calloc((size_t)s->numObs + 1, sizeof(double *));
   for (int o = 0; o < s->numObs; o++) {
-    if (pid[o] < 0) continue;
+    if (d->pid[o] < 0) continue;
     td[o] = malloc(nout * 6 * sizeof(double));
     vh_observer *ob = &s->obs[o];
     double *re = NULL, *im = NULL;
@@ -253,7 +321,7 @@ int vh_gpu_run(vh_solve *s, int prec, int device, unsigned long chunk, double *l
       re = malloc(6 * s->numFreq * sizeof(double));
       im = malloc(6 * s->numFreq * sizeof(double));
     }
-    vgpu_read_point_observer(c, pid[o], td[o], nout, re, im);
+    vgpu_read_point_observer(d->c, d->pid[o], td[o], nout, re, im);
     if (re) {
       for (size_t q = 0; q < 6 * s->numFreq; q++) { ob->dft_re[q] = re[q]; ob->dft_im[q] = im[q]; }
       free(re);
@@ -261,23 +329,47 @@ int vh_gpu_run(vh_solve *s, int prec, int device, unsigned long chunk, double *l
     }
   }
   s->deviceDft = 1;
+  vgpu_destroy(d->c);
+  d->c = NULL;
+  *tdOut = td;
+  return 0;
+}
+
+/* Replay the collected samples through the host observer writers (the
+ * reference's per-step file output, observer.c:622-650) and free them. */
+void vh_dsolve_write(vh_dsolve *d, double **td)
+{
+  vh_solve *s = d->s;
   double *pv = calloc((size_t)s->numObs * 6 + 6, sizeof(double));
-  for (unsigned long n = 0; n < NT; n++) {
+  for (unsigned long n = 0; n < d->NT; n++) {
     real t = n * s->dt;
     bool ot = n >= s->startTimeStep && n <= s->stopTimeStep;
     for (int o = 0; o < s->numObs; o++)
-      if (pid[o] >= 0 && ot)
+      if (d->pid[o] >= 0 && ot)
         for (int q = 0; q < 6; q++) pv[6 * o + q] = td[o][(n - s->startTimeStep) * 6 + q];
-    vh_observe_step(s, n, t, pv, vol ? vol + (size_t)n * volNodes * 6 : NULL);
+    vh_observe_step(s, n, t, pv, d->vol ? d->vol + (size_t)n * d->volNodes * 6 : NULL);
   }
   for (int o = 0; o < s->numObs; o++) free(td[o]);
   free(td);
   free(pv);
-  free(vol);
-  free(vnodes);
-  free(pid);
-  free(vid);
-  vgpu_destroy(c);
+}
+
+double vh_dsolve_loop_ms(const vh_dsolve *d) { return d->loop_ms; }
+
+/* Run the whole time loop on the device and write the observer files.
+ * loop_ms receives the device time of the time loop alone. */
+int vh_gpu_run(vh_solve *s, int prec, int device, unsigned long chunk, double *loop_ms)
+{
+  vh_dsolve *d = vh_dsolve_open(s, prec, device, chunk);
+  if (!d) return -1;
+  while (d->next < d->NT) {
+    if (vh_dsolve_enqueue(d) < 0 || vh_dsolve_complete(d)) { vh_dsolve_close(d); return -1; }
+  }
+  if (loop_ms) *loop_ms = d->loop_ms;
+  double **td = NULL;
+  vh_dsolve_collect(d, &td);
+  vh_dsolve_write(d, td);
+  vh_dsolve_close(d);
   return 0;
 }
 

@@ -1,0 +1,36 @@
+/*
+ * vh_gpu.h - the host's device driver and solve server.
+ *
+ * The device driver hands a set-up vh_solve to the vgpu C-ABI and runs its
+ * time loop (the drop-in for propagate(), simulation.c:136-230); the server
+ * keeps many such solves resident and in flight on one GPU at once.
+ */
+#ifndef VH_GPU_H
+#define VH_GPU_H
+
+#include "vh.h"
+#include "../../include/vgpu.h"
+
+/* ---- device driver (vh_gpu.c) ---- */
+vgpu_ctx *vh_gpu_setup(vh_solve *s, int prec, int device, unsigned long chunk);
+int vh_gpu_run(vh_solve *s, int prec, int device, unsigned long chunk, double *loop_ms);
+int vh_run_mesh_gpu(const char *mesh, const char *outdir, int prec, int device, int fullPrecision, long steps,
+                    unsigned long chunk, double *loop_ms, char *err, int errlen);
+
+typedef struct vh_dsolve vh_dsolve;
+vh_dsolve *vh_dsolve_open(vh_solve *s, int prec, int device, unsigned long chunk);
+long vh_dsolve_enqueue(vh_dsolve *d);
+int vh_dsolve_ready(vh_dsolve *d);
+int vh_dsolve_complete(vh_dsolve *d);
+unsigned long vh_dsolve_remaining(const vh_dsolve *d);
+int vh_dsolve_collect(vh_dsolve *d, double ***td);
+void vh_dsolve_write(vh_dsolve *d, double **td);
+double vh_dsolve_loop_ms(const vh_dsolve *d);
+vgpu_ctx *vh_dsolve_context(vh_dsolve *d);
+void vh_dsolve_close(vh_dsolve *d);
+
+/* out[0] gobox cells, out[1] mesh cells, out[2] time steps; numCells. */
+void vh_solve_sizes(const vh_solve *s, long long out[3]);
+void vh_solve_shape(const vh_solve *s, long long out[3]);
+
+#endif

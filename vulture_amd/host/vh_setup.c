@@ -1275,6 +1275,31 @@ static void free_rc(vh_rcfilter *rc)
   free(rc->q0); free(rc->q1); free(rc->q2); free(rc->poles); free(rc->residues);
 }
 
+void vh_release_grid_arrays(vh_solve *s)
+{
+  real **many[] = {&s->Ex, &s->Ey, &s->Ez, &s->Hx, &s->Hy, &s->Hz, &s->alphaEx, &s->alphaEy, &s->alphaEz,
+                   &s->betaEx, &s->betaEy, &s->betaEz, &s->gammaHx, &s->gammaHy, &s->gammaHz};
+  for (size_t q = 0; q < sizeof(many) / sizeof(many[0]); q++) {
+    free(*many[q]);
+    *many[q] = NULL;
+  }
+  for (int r = 0; r < 6; r++) {
+    real **aux[] = {&s->PPx[r], &s->PPy[r], &s->PPz[r], &s->Px[r], &s->Py[r], &s->Pz[r], &s->Bx[r], &s->By[r],
+                    &s->Bz[r]};
+    for (size_t q = 0; q < sizeof(aux) / sizeof(aux[0]); q++) {
+      free(*aux[q]);
+      *aux[q] = NULL;
+    }
+  }
+  for (int d = 0; d < s->numDebye; d++)
+    for (int f = 0; f < 3; f++) {
+      free(s->debye[d].J[f]);
+      free(s->debye[d].lastE[f]);
+      s->debye[d].J[f] = NULL;
+      s->debye[d].lastE[f] = NULL;
+    }
+}
+
 void vh_destroy(vh_solve *s)
 {
   if (!s) return;

@@ -107,19 +107,23 @@ def _spheres(rng, n, count, rmin, rmax, gap=2):
     return out
 
 
-def debye(steps=3000, n=256, spheres=200, pml=8, seed=3, rmin=4, rmax=20, observers=8, nfreq=64):
+def debye(steps=3000, n=256, spheres=200, pml=8, seed=3, rmin=4, rmax=20, observers=8, nfreq=64, boundary="PML",
+          planewave=True):
     """Debye-sphere volume.  Each sphere is its own single-pole DEBYE medium
     (eps_inf ~ U[2,10], delta_eps ~ U[1,40], tau ~ logU[1e-11,1e-9] s,
     sigma ~ logU[1e-3,1] S/m; residue 0.5*delta_eps/tau, pole -1/tau) and is
     voxelised as one MB card per (i, j) run of cells whose centres lie inside.
     Each run carries the face mask 101011 (XLO, YLO, ZLO, ZHI included; XHI,
     YHI left to the neighbouring run), so no electric edge belongs to two
-    Debye blocks."""
+    Debye blocks.  `boundary` "PML" (the workload), or "PEC"/"MUR" on all
+    faces, and `planewave` False (a hard point source instead) give variants
+    that isolate parts of the time loop when timing it."""
     rng = np.random.default_rng(seed)
     sph = _spheres(rng, n, spheres, min(rmin, n // 4), min(rmax, n // 4))
-    lines = ["VM 1.0.0", f"CE Debye spheres: {n}^3 cubic 2 mm, {len(sph)} inclusions, {pml}-cell UPML",
+    lines = ["VM 1.0.0", f"CE Debye spheres: {n}^3 cubic 2 mm, {len(sph)} inclusions, {boundary} boundaries",
              f"DM {n} {n} {n}", "GS"]
-    lines += [f"BT {f} PML {pml}" for f in ("XLO", "XHI", "YLO", "YHI", "ZLO", "ZHI")]
+    bt = f"PML {pml}" if boundary == "PML" else boundary
+    lines += [f"BT {f} {bt}" for f in ("XLO", "XHI", "YLO", "YHI", "ZLO", "ZHI")]
     for s in range(len(sph)):
         einf = rng.uniform(2, 10)
         de = rng.uniform(1, 40)
@@ -143,7 +147,11 @@ def debye(steps=3000, n=256, spheres=200, pml=8, seed=3, rmin=4, rmax=20, observ
     eta = rng.uniform(0, 360)
     m = max(4, n // 64)
     lines.append(_gauss_card("wf1", 15e9))
-    lines.append(f"PW {m} {n - m} {m} {n - m} {m} {n - m} pw1 wf1 {theta:.4f} {phi:.4f} {eta:.4f} 111111")
+    if planewave:
+        lines.append(f"PW {m} {n - m} {m} {n - m} {m} {n - m} pw1 wf1 {theta:.4f} {phi:.4f} {eta:.4f} 111111")
+    else:
+        c = n // 2
+        lines.append(f"EX {c} {c} {c} {c} {c} {c + 1} src =EZ wf1 1.0")
     for o in range(observers):
         p = rng.integers(m + 2, n - m - 2, size=3)
         lines.append(f"OP {p[0]} {p[0]} {p[1]} {p[1]} {p[2]} {p[2]} t{o} TDOM_ASCII")
@@ -162,3 +170,47 @@ def write(name, outdir, **kw):
     with open(path, "w") as f:
         f.write(WORKLOADS[name](**kw))
     return path
+
+
+def serving_request(rng, nmin=32, nmax=320, steps_min=2000, steps_max=8000, nfreq=64, pml=8):
+    """One solve request of the serving sweep: per-axis cells log-uniform in
+    [nmin, nmax] on random graded lines (neighbour ratio <= 1.2, 1-10 mm),
+    4-24 seeded SIMPLE (eps_r ~ U[1,12]) or single-pole DEBYE blocks (the
+    Debye-sphere ranges), an 8-cell UPML, a TF/SF plane wave at random angles
+    (analytic incident field on the graded mesh), steps ~ U{steps_min ..
+    steps_max} and two observers with a `nfreq`-frequency DFT."""
+    n = [int(round(math.exp(rng.uniform(math.log(nmin), math.log(nmax))))) for _ in range(3)]
+    sizes = []
+    for a in range(3):
+        d = np.empty(n[a])
+        d[0] = rng.uniform(1e-3, 1e-2)
+        for i in range(1, n[a]):
+            d[i] = min(max(d[i - 1] * rng.uniform(1 / 1.2, 1.2), 1e-3), 1e-2)
+        sizes.append(d)
+    lines = ["VM 1.0.0", f"CE serving request {n[0]}x{n[1]}x{n[2]}", f"DM {n[0]} {n[1]} {n[2]}", "GS"]
+    lines += [f"BT {f} PML {pml}" for f in ("XLO", "XHI", "YLO", "YHI", "ZLO", "ZHI")]
+    nb = int(rng.integers(4, 25))
+    for b in range(nb):
+        if rng.uniform() < 0.5:
+            lines.append(f"MT m{b} SIMPLE {rng.uniform(1, 12):.6e} 0.0 1.0")
+        else:
+            einf, de = rng.uniform(2, 10), rng.uniform(1, 40)
+            tau, sig = 10 ** rng.uniform(-11, -9), 10 ** rng.uniform(-3, 0)
+            lines.append(f"MT m{b} DEBYE {einf:.6e} {sig:.6e} 1.0 {0.5 * de / tau:.6e} {-1.0 / tau:.6e}")
+    for b in range(nb):
+        box = []
+        for a in range(3):
+            w = int(rng.integers(1, max(2, n[a] // 3)))
+            lo = int(rng.integers(1, n[a] - w))
+            box += [lo, lo + w]
+        lines.append(f"MB {box[0]} {box[1]} {box[2]} {box[3]} {box[4]} {box[5]} m{b}")
+    lines.append(_gauss_card("wf1", 3e9))
+    lines.append(f"PW 2 {n[0] - 2} 2 {n[1] - 2} 2 {n[2] - 2} pw1 wf1 {rng.uniform(0, 180):.4f} "
+                 f"{rng.uniform(0, 360):.4f} {rng.uniform(0, 360):.4f} 111111")
+    for o in range(2):
+        p = [int(rng.integers(3, n[a] - 3)) for a in range(3)]
+        lines.append(f"OP {p[0]} {p[0]} {p[1]} {p[1]} {p[2]} {p[2]} o{o} FDOM_ASCII")
+    steps = int(rng.integers(steps_min, steps_max + 1))
+    lines += ["GE", f"NT {steps}", f"OF 1e8 3e9 {nfreq}",
+              _lines_card("X", sizes[0]), _lines_card("Y", sizes[1]), _lines_card("Z", sizes[2]), "EN"]
+    return "\n".join(lines) + "\n", n, steps
