@@ -14,14 +14,14 @@ FP64 DFT/SIBC state; `value` is the FP32 form's Mcell-updates/s and the FP64
 figures sit in `fp64`.  Fields stay resident in HBM; W untimed steps run
 first, then K steps are timed with hipEvents on the solve's stream.
 
-`roofline` prices the dominant kernel (the E or H volume sweep, whichever
-takes longer) by its own compulsory bytes per launch: E sweep 15 words per
-interior cell (read Ex..Hz 6, alpha/beta 6, write E 3) and 27 per UPML cell
-(+6 split auxiliaries read and written); H sweep 12 and 18 words.  Its average
-duration comes from hipEvents around every launch in the timed steps.
-`step_roofline` prices the whole step with the compulsory bytes of a fused
-update: 21 words per interior cell, 39 per UPML cell and 4 per Debye site and
-pole (J and lastE read and written), over ms_per_step.
+`roofline` prices the dominant kernel (the fast E or fast H sweep over the
+inner grid, whichever takes longer) by its own compulsory bytes per launch
+(kernel_bytes below: 9 field words and the 2-byte coefficient class per cell,
+plus the fused Debye sites for E).  Its average duration comes from hipEvents
+around every launch in the timed steps.  `step_roofline` prices the whole
+step with the compulsory bytes of one pass in the same representation
+(step_bytes: 12 field words and two 2-byte classes per cell, 18 more words
+per UPML cell, the Debye state), over ms_per_step.
 
 `cpu_baseline` times the CPU restatement of the same loop (-O3 timing build,
 OpenMP) on a bounded number of steps of the same mesh on this host.
@@ -51,20 +51,32 @@ def _traffic(kernel, prec):
     return None
 
 
+def debye_site_words(st):
+    """Compulsory words per Debye site: lastE read and written, J read and
+    written per pole (two words per pole when complex)."""
+    return 2 + 2 * max(st["debye_poles"], 1) * (2 if st["debye_complex"] else 1)
+
+
 def kernel_bytes(st, esz):
-    """Compulsory bytes per launch of the E and H volume sweeps."""
-    interior = st["mesh_cells"]
-    upml = st["cells"] - st["mesh_cells"]
-    e = (15 * interior + 27 * upml) * esz
-    h = (12 * interior + 18 * upml) * esz
+    """Compulsory bytes per launch of the fast E and H kernels.  Fast E: read
+    E and H (6 words) and the coefficient class (2 B; 6 words of alpha/beta
+    without classes), write E (3), plus the fused Debye sites; fast H: read H
+    and E (6), the class (2 B; 3 words of gamma), write H (3)."""
+    ce, ch = st["fast_cells"]
+    coef_e, coef_h = (2, 2) if st["indexed_media"] else (6 * esz, 3 * esz)
+    e = ce * (9 * esz + coef_e) + st["debye_fused_sites"] * debye_site_words(st) * esz
+    h = ch * (9 * esz + coef_h)
     return e, h
 
 
 def step_bytes(st, esz):
-    """Compulsory bytes of one fused step (21 / 39 words, +4 per Debye site and pole)."""
-    interior = st["mesh_cells"]
-    upml = st["cells"] - st["mesh_cells"]
-    return (21 * interior + 39 * upml + 4 * st["debye_sites"] * max(st["debye_poles"], 1)) * esz
+    """Compulsory bytes of one step in the representation the kernels use:
+    every field word read and written once (12 words per cell), the
+    coefficients once (two 2-byte classes, or 9 words), 9 UPML auxiliaries
+    read and written (18 words per UPML cell), and the Debye state."""
+    cells, upml = st["cells"], st["cells"] - st["mesh_cells"]
+    coef = 4 if st["indexed_media"] else 9 * esz
+    return cells * (12 * esz + coef) + upml * 18 * esz + st["debye_sites"] * debye_site_words(st) * esz
 
 
 def measure(mesh, prec, steps, warmup):
@@ -82,9 +94,9 @@ def measure(mesh, prec, steps, warmup):
     esz = 8 if prec == "fp64" else 4
     eb, hb = kernel_bytes(st, esz)
     if e_ms >= h_ms:
-        name, kb, kms = "k_volume_e", eb, e_ms
+        name, kb, kms = "k_e_fast", eb, e_ms
     else:
-        name, kb, kms = "k_volume_h", hb, h_ms
+        name, kb, kms = "k_h_fast", hb, h_ms
     ms_step = total_ms / steps
     achieved = kb / (kms * 1e-3) / 1e9
     sb = step_bytes(st, esz)
@@ -152,7 +164,8 @@ def main():
         "config": {"workload": f"debye spheres {args.n}^3 cubic 2 mm, 8-cell UPML, 200 single-pole Debye "
                                f"spheres, aux-grid TF/SF plane wave, 8 DFT observers (seed 3)",
                    "cells": st["cells"], "mesh_cells": st["mesh_cells"], "debye_sites": st["debye_sites"],
-                   "device_bytes": st["bytes"]},
+                   "device_bytes": st["bytes"], "coefficients": "16-bit classes" if st["indexed_media"] else
+                   "per-cell arrays", "debye_fused_sites": st["debye_fused_sites"]},
         "roofline": head["roofline"], "step_roofline": head["step_roofline"],
     }
     if "fp64" in res and "fp32" in res:

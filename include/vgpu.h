@@ -183,6 +183,10 @@ typedef struct {
   uint64_t sibc_faces;         /* SIBC surface faces */
   uint64_t correction_rows[2]; /* source / TF/SF correction rows, E and H */
   uint64_t bytes;              /* device footprint */
+  uint64_t fast_cells[2];      /* cells of the E and H fast sweep kernels */
+  uint64_t debye_fused_sites;  /* Debye sites applied inside the fast E kernel */
+  int indexed_media;           /* 1: coefficients read as 16-bit classes */
+  int debye_fused;             /* 1: first Debye layer fused into the E sweep */
 } vgpu_stats;
 int vgpu_get_stats(const vgpu_ctx *c, vgpu_stats *out);
 

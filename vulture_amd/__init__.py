@@ -105,7 +105,9 @@ class Stats(ctypes.Structure):
     """vgpu_stats (include/vgpu.h)."""
     _fields_ = [("cells", ctypes.c_uint64), ("mesh_cells", ctypes.c_uint64), ("debye_sites", ctypes.c_uint64),
                 ("debye_poles", ctypes.c_int), ("debye_complex", ctypes.c_int), ("sibc_faces", ctypes.c_uint64),
-                ("correction_rows", ctypes.c_uint64 * 2), ("bytes", ctypes.c_uint64)]
+                ("correction_rows", ctypes.c_uint64 * 2), ("bytes", ctypes.c_uint64),
+                ("fast_cells", ctypes.c_uint64 * 2), ("debye_fused_sites", ctypes.c_uint64),
+                ("indexed_media", ctypes.c_int), ("debye_fused", ctypes.c_int)]
 
 
 class DeviceSolve:
@@ -167,6 +169,7 @@ class DeviceSolve:
         self._check(self._v.vgpu_get_stats(self._c, ctypes.byref(st)))
         d = {k: getattr(st, k) for k, _ in Stats._fields_}
         d["correction_rows"] = [int(x) for x in st.correction_rows]
+        d["fast_cells"] = [int(x) for x in st.fast_cells]
         return d
 
     def download(self, component):

@@ -209,9 +209,8 @@ int vgpu_create(const vgpu_grid_desc *g, vgpu_prec prec, int device, vgpu_ctx **
   c->grid.dex = c->grid.dey = c->grid.dez = nullptr;
   c->grid.idex = c->grid.idey = c->grid.idez = nullptr;
   c->grid.idhx = c->grid.idhy = c->grid.idhz = nullptr;
-  // Fast boxes: where every E (H) component takes the ordinary update; x
-  // chunks of the sweeps start on the E fast box so whole wavefronts fall
-  // inside it.
+  // Fast boxes: where every E (H) component takes the ordinary update, kept
+  // one cell inside the allocation so the fast kernels never test bounds.
   for (int e = 0; e < 2; ++e) {
     int *fb = e == 0 ? c->fastE : c->fastH;
     for (int a = 0; a < 3; ++a) {
@@ -223,8 +222,7 @@ int vgpu_create(const vgpu_grid_desc *g, vgpu_prec prec, int device, vgpu_ctx **
       }
     }
   }
-  c->X0 = c->fastE[0] <= c->fastE[1] ? c->fastE[0] - 64 * ((c->fastE[0] + 63) / 64) : 0;
-  c->NCH = (c->NX - c->X0 + 63) / 64;
+  c->NCH = c->fastE[0] <= c->fastE[1] ? (c->fastE[1] - c->fastE[0] + 64) / 64 : 0;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return VGPU_EDEVICE;
@@ -449,6 +447,11 @@ int vgpu_add_volume_observer(vgpu_ctx *c, const int gbbox[6], const int step[3],
 // finalize: build device lists
 // ---------------------------------------------------------------------------
 
+static bool in_fast(const int *fb, int i, int j, int k)
+{
+  return i >= fb[0] && i <= fb[1] && j >= fb[2] && j <= fb[3] && k >= fb[4] && k <= fb[5];
+}
+
 static int build_debye(Ctx *c)
 {
   if (c->debye.empty()) return VGPU_OK;
@@ -503,7 +506,11 @@ static int build_debye(Ctx *c)
               return set_error(c, VGPU_EINVAL, "Debye block outside the grid");
             const long long cell = c->lin(i, j, k);
             int &cnt = seen[cell * 3 + f];
-            const int layer = cnt++;
+            // layer 0: first block of a cell inside the fast E box (applied
+            // by the fast sweep); layer 1: first block elsewhere; then the
+            // later blocks of a cell, one layer each
+            const int layer = cnt == 0 ? (in_fast(c->fastE, i, j, k) ? 0 : 1) : cnt + 1;
+            ++cnt;
             layers = std::max(layers, layer + 1);
             sites.push_back({layer, f, cell, blockMedium[b]});
           }
@@ -524,7 +531,7 @@ static int build_debye(Ctx *c)
   for (long long s = 0; s < n; ++s) { comp[s] = sites[s].comp; cell[s] = sites[s].cell; med[s] = sites[s].med; }
   c->debyeLayers = layers;
   c->debyeSites = n;
-  c->debyeL0 = c->debyeLayerStart[1];
+  c->debyeL0 = layers > 1 ? c->debyeLayerStart[1] : n;
 
   // Refine the E coefficient classes with each component's layer-0 Debye
   // medium, so the E sweep can apply layer 0 itself.
@@ -547,15 +554,15 @@ static int build_debye(Ctx *c)
     }
     c->ecls[kv.first] = it->second;
   }
-  // First layer-0 site of every (component, row, 64-cell x chunk), for the
-  // E sweep's in-wavefront site numbering.
+  // First layer-0 site of every (component, row, 64-cell x chunk of the fast
+  // box), for the fast sweep's in-wavefront site numbering.
   const size_t nrow = (size_t)c->NZ * c->NY * c->NCH;
   std::vector<int> dbase(3 * nrow + 1, 0);
   for (long long s = 0; s < c->debyeL0; ++s) {
     const long long cl = cell[s];
     const int i = (int)(cl % c->PX);
     const long long row = cl / c->PX;  // k*NY + j
-    const size_t q = (size_t)comp[s] * nrow + (size_t)row * c->NCH + (i - c->X0) / 64;
+    const size_t q = (size_t)comp[s] * nrow + (size_t)row * c->NCH + (i - c->fastE[0]) / 64;
     dbase[q + 1]++;
   }
   if (c->debyeL0 > INT32_MAX) return set_error(c, VGPU_EINVAL, "too many Debye sites");
@@ -565,14 +572,15 @@ static int build_debye(Ctx *c)
   // 64-cell chunk, the cells whose class carries a Debye medium for the
   // component.  Check here that this numbering reproduces the site list
   // exactly, so the sweep can never address a site outside it.
+  const int *fb = c->fastE;
   for (int f = 0; f < 3; ++f) {
-    long long g = f == 0 ? 0 : dbase[(size_t)f * nrow];
-    for (int k = 0; k < c->NZ; ++k)
-      for (int j = 0; j < c->NY; ++j)
-        for (int i = 0; i < c->NX; ++i) {
+    long long g = dbase[(size_t)f * nrow];
+    for (int k = fb[4]; k <= fb[5]; ++k)
+      for (int j = fb[2]; j <= fb[3]; ++j)
+        for (int i = fb[0]; i <= fb[1]; ++i) {
           const long long cl = c->lin(i, j, k);
-          const size_t row = (size_t)f * nrow + ((size_t)k * c->NY + j) * c->NCH + (i - c->X0) / 64;
-          if (((i - c->X0) % 64 == 0 || i == 0) && dbase[row] != g)
+          const size_t row = (size_t)f * nrow + ((size_t)k * c->NY + j) * c->NCH + (i - fb[0]) / 64;
+          if ((i - fb[0]) % 64 == 0 && dbase[row] != g)
             return set_error(c, VGPU_EINVAL, "Debye site numbering (chunk start)");
           const int ds = c->edslot[(size_t)c->ecls[cl] * 3 + f];
           if (ds < 0) continue;
@@ -984,6 +992,7 @@ int vgpu_finalize(vgpu_ctx *c, uint64_t total_steps, uint64_t max_steps_per_run)
   if ((r = build_debye(c)) || (r = build_class_tables(c)) || (r = build_sibc(c)) || (r = build_corrections(c)) ||
       (r = build_observers(c)))
     return r;
+  if ((r = c->prec == VGPU_FP64 ? prepare_sweeps<double>(c) : prepare_sweeps<float>(c))) return r;
   if (hipDeviceSynchronize() != hipSuccess) return set_error(c, VGPU_EDEVICE, "device error during set-up");
   c->finalized = true;
   return VGPU_OK;
@@ -1149,6 +1158,15 @@ int vgpu_get_stats(const vgpu_ctx *c, vgpu_stats *out)
   out->correction_rows[0] = (uint64_t)c->corrRows[0];
   out->correction_rows[1] = (uint64_t)c->corrRows[1];
   out->bytes = c->bytes;
+  for (int e = 0; e < 2; ++e) {
+    const int *fb = e == 0 ? c->fastE : c->fastH;
+    out->fast_cells[e] = (fb[0] <= fb[1] && fb[2] <= fb[3] && fb[4] <= fb[5])
+                             ? (uint64_t)(fb[1] - fb[0] + 1) * (uint64_t)(fb[3] - fb[2] + 1) * (uint64_t)(fb[5] - fb[4] + 1)
+                             : 0;
+  }
+  out->debye_fused_sites = c->debyeFused ? (uint64_t)c->debyeL0 : 0;
+  out->indexed_media = c->indexed ? 1 : 0;
+  out->debye_fused = c->debyeFused ? 1 : 0;
   return VGPU_OK;
 }
 

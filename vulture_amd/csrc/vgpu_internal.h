@@ -17,12 +17,11 @@ namespace vgpu {
 
 // ---- kernel argument blocks (passed by value) ----
 
+// Full argument block of the E sweep, kept in device memory and read through
+// a pointer by the shell kernel and the fused Debye step.
 template <typename T> struct EArgs {
   int NX, NY, NZ, PX;
   long long plane;
-  int k0, kchunk;
-  int x0;                // x of the first lane of x chunk 0 (<= 0): chunks start on the fast box
-  int fast[6];           // box where every component takes the ordinary update
   T *E[3];
   const T *H[3];
   const T *alpha[3], *beta[3];
@@ -38,8 +37,7 @@ template <typename T> struct EArgs {
   const int *edslot;
   // layer-0 Debye sites applied in the sweep
   int dfuse;
-  const int *dbase;      // [3][NZ][NY][NCH] first site of each 64-cell chunk
-  int NCH;
+  const int *dbase;      // [3][NZ][NY][NCH] first site of each 64-cell chunk of the fast box
   T *dlast, *dJ;
   long long dstride;
   const double *dalpha, *dbeta;
@@ -50,9 +48,6 @@ template <typename T> struct EArgs {
 template <typename T> struct HArgs {
   int NX, NY, NZ, PX;
   long long plane;
-  int k0, kchunk;
-  int x0;
-  int fast[6];
   T *H[3];
   const T *E[3];
   const T *gamma[3];
@@ -64,6 +59,42 @@ template <typename T> struct HArgs {
   const T *ad[3], *ah[3], *bh[3], *ibd[3];
   const uint16_t *hcls;  // coefficient classes: per-cell class, [class][3] gamma
   const T *htab;
+};
+
+// Lean by-value arguments of the fast sweep kernels.
+template <typename T> struct FastE {
+  int NX, NY, NZ, PX;
+  long long plane;
+  int box[6];            // fast box (every E component inner-grid)
+  int kchunk, nchx;      // planes per block; 64-cell x chunks from box[0]
+  T *E[3];
+  const T *H[3];
+  const T *idh[3];
+  const uint16_t *ecls;  // coefficient classes
+  const T *etab;
+  const T *alpha[3], *beta[3];  // per-cell arrays when there are no classes
+  int dfuse;             // apply layer-0 Debye sites
+  const EArgs<T> *ap;    // full argument block in device memory (Debye state)
+};
+
+template <typename T> struct FastH {
+  int NX, NY, NZ, PX;
+  long long plane;
+  int box[6];
+  int kchunk, nchx;
+  T *H[3];
+  const T *E[3];
+  const T *ide[3];
+  const uint16_t *hcls;
+  const T *htab;
+  const T *gamma[3];
+};
+
+// Shell of a sweep: up to six boxes, x-fastest, laid end to end.
+struct ShellBoxes {
+  int nb;
+  int box[6][6];
+  long long start[7];    // first thread of each box; start[nb] = total
 };
 
 // Mur tangential-E faces (mur.c:135-395): one box per (face, component).
@@ -293,9 +324,9 @@ struct Ctx {
   // layer-0 Debye sites handled inside the E sweep
   bool debyeFused = false;
   long long debyeL0 = 0;
-  int NCH = 0;                               // 64-cell x chunks per row
-  int X0 = 0;                                // x of the first cell of chunk 0 (<= 0)
-  int fastE[6] = {}, fastH[6] = {};          // boxes of the sweeps' ordinary-update fast path
+  int NCH = 0;                               // 64-cell x chunks of the fast E box
+  int fastE[6] = {}, fastH[6] = {};          // boxes of the sweeps' fast kernels
+  void *d_eargs = nullptr, *d_hargs = nullptr;  // argument blocks in device memory
   void *d_dbase = nullptr;                   // [3][NZ][NY][NCH] + 1: first layer-0 site
 
   // UPML
@@ -377,5 +408,7 @@ void *dev_alloc(Ctx *c, size_t bytes);
 // Enqueues nsteps steps that start at the device step counter, then advances
 // it; with timeKernels the volume sweeps of the first steps are timed.
 template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool timeKernels);
+// Uploads the sweeps' argument blocks (once, at finalize).
+template <typename T> int prepare_sweeps(Ctx *c);
 
 }  // namespace vgpu

@@ -2,3 +2,4 @@
 // kernels perform the reference's double-precision operations one by one.
 #include "vgpu_run.h"
 template int vgpu::run_steps<double>(vgpu::Ctx *, uint64_t, hipStream_t, bool);
+template int vgpu::prepare_sweeps<double>(vgpu::Ctx *);

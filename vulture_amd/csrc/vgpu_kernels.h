@@ -2,12 +2,8 @@
 // -ffp-contract=off so every update performs the reference's operations in
 // the reference's order) and for float (contraction allowed).
 //
-// Volume kernels: a 64x4 thread tile covers 64 x-cells by 4 y-rows and
-// marches along z over a chunk of planes, keeping the previous (E update) or
-// next (H update) plane's values in registers, so each field word is read
-// from HBM once per sweep; x- and y-neighbours come from the same tile's
-// loads through L1/L2.  Every cell of the allocation is visited once and
-// classified per component into the inner grid, a UPML slab or nothing.
+// Volume sweeps: a fast kernel over the box where every component takes the
+// ordinary update, and a shell kernel for the rest (see below).
 #pragma once
 #include "vgpu_internal.h"
 
@@ -26,37 +22,21 @@ template <typename T> VG_DEV long long slab_index(const int *b, int i, int j, in
   return ((long long)(k - b[4]) * nj + (j - b[2])) * ni + (i - b[0]);
 }
 
-// Region of component C at (i,j,k): 6 = ordinary update, 0..5 = UPML slab
-// (then p is the cell's offset in that slab), -1 = not updated by the sweep.
-// The loops are unrolled so every box and pointer is a compile-time field of
-// the kernel arguments (a runtime index into them would spill the argument
-// block to scratch).
-template <int NB, int C, typename A>
-VG_DEV int classify(const A &a, const int (*in)[6], int i, int j, int k, long long &p)
+// Region of component C at (i,j,k) from the argument block in device memory:
+// 6 = ordinary update (one of the NB boxes `in`), 0..5 = UPML slab (p is the
+// cell's offset in it), -1 = not updated by the sweep.
+template <int NB, typename A>
+VG_DEV int classify(const A &a, const int (*in)[6], int C, int i, int j, int k, long long &p)
 {
-#pragma unroll
   for (int b = 0; b < NB; ++b)
     if (in_box(in[b], i, j, k)) return 6;
-  if (a.pmlmask) {
-#pragma unroll
+  if (a.pmlmask)
     for (int r = 0; r < 6; ++r)
       if (((a.pmlmask >> r) & 1u) && in_box(a.pl[r][C], i, j, k)) {
         p = slab_index<int>(a.pl[r][C], i, j, k);
         return r;
       }
-  }
   return -1;
-}
-
-// Slab array of region reg (0..5) for component C, selected without a runtime
-// index into the argument block.
-template <int C, typename T> VG_DEV T *slab_ptr(T *const (&arr)[6][3], int reg)
-{
-  T *q = nullptr;
-#pragma unroll
-  for (int r = 0; r < 6; ++r)
-    if (reg == r) q = arr[r][C];
-  return q;
 }
 
 // Debye ADE correction of one site (debye.c:243-307); shared by the fused E
@@ -99,77 +79,168 @@ VG_DEV T debye_site(T e, T beta, long long g, int m, T *__restrict__ lastE, T *_
   return e;
 }
 
-// Layer-0 Debye correction inside the E sweep.  The chunk's first site and
-// the next chunk's come from dbase (uniform over the wavefront, so rows
-// without Debye sites skip this entirely); a site's number is the chunk's
-// first site plus the count of lower lanes whose class carries a Debye
-// medium for C.  On lanes holding a site, e becomes the corrected value (from
-// E[id] when the sweep did not update the cell, have == false) and the
-// function returns true.
+// ===========================================================================
+// Volume sweeps.  Each half-step is two kernels over disjoint cells:
+//  - the fast kernel over the fast box, where every component takes the
+//    ordinary update (inner grid).  A 64x4 thread tile covers 64 x-cells by
+//    4 y-rows and marches along z over a chunk of planes, carrying the plane
+//    below (E) or above (H) in registers; its arguments are a few pointers
+//    and integers passed by value, so they stay in scalar registers.
+//  - the shell kernel over everything else (UPML slabs, Mur normal H, cells
+//    of one component's inner grid only, ghost layers), one thread per cell,
+//    reading its boxes and pointers from the argument block in device memory.
+// ===========================================================================
+
+// Layer-0 Debye correction inside the fast E kernel (debye.c:243-307).  The
+// chunk's first site comes from dbase (uniform over the wavefront, so rows
+// without Debye sites skip this entirely); a site's number is that plus the
+// count of lower lanes whose class carries a Debye medium for C, which is
+// the site list's (component, cell) order.
 template <typename T, int C>
-VG_DEV bool debye_fused(const EArgs<T> &a, const int n3[3], bool vx, long long id, unsigned cls, T be, bool have, T &e)
+VG_DEV void debye_fused(const FastE<T> &f, int j, int k, bool vx, unsigned cls, T be, T &e)
 {
-  const long long row = ((long long)(C * a.NZ + n3[2]) * a.NY + n3[1]) * a.NCH + blockIdx.x;
+  const EArgs<T> &a = *f.ap;
+  const long long row = ((long long)(C * f.NZ + k) * f.NY + j) * f.nchx + blockIdx.x;
   const int g0 = a.dbase[row];
-  if (a.dbase[row + 1] == g0) return false;
+  if (a.dbase[row + 1] == g0) return;
   const int ds = vx ? a.edslot[cls * 3 + C] : -1;
   const unsigned long long m = __ballot(ds >= 0);
-  if (ds < 0) return false;
+  if (ds < 0) return;
   const long long g = g0 + __popcll(m & ((1ull << __lane_id()) - 1ull));
-  if (!have) e = a.E[C][id];
   e = debye_site<T>(e, be, g, ds, a.dlast, a.dJ, a.dstride, a.dalpha, a.dbeta, a.dnp, a.dmaxP, a.dcplx);
+}
+
+template <typename T, bool IDX, int C>
+VG_DEV void e_fast(const FastE<T> &f, int j, int k, bool vx, long long id, unsigned cls, T curl)
+{
+  T *__restrict__ E = f.E[C];
+  T al, be;
+  if (IDX) {
+    al = f.etab[cls * 6 + 2 * C];
+    be = f.etab[cls * 6 + 2 * C + 1];
+  } else {
+    al = f.alpha[C][id];
+    be = f.beta[C][id];
+  }
+  T e = al * E[id] + be * curl;
+  if (IDX && f.dfuse) debye_fused<T, C>(f, j, k, vx, cls, be, e);
+  if (vx) E[id] = e;
+}
+
+// Fast E kernel: inner grid (grid.c:709-797) and the first Debye layer.
+template <typename T, bool IDX> __global__ __launch_bounds__(256) void k_e_fast(FastE<T> f)
+{
+  const int i = f.box[0] + (int)blockIdx.x * 64 + (int)threadIdx.x;
+  const int j = f.box[2] + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (j > f.box[3]) return;  // whole wavefront
+  const bool vx = i <= f.box[1];
+  const int ic = vx ? i : f.box[1];
+  const int kb = f.box[4] + (int)blockIdx.z * f.kchunk;
+  const int ke = min(kb + f.kchunk, f.box[5] + 1);
+  if (kb >= ke) return;
+  const long long P = f.plane, SJ = f.PX;
+  const T *__restrict__ Hx = f.H[0];
+  const T *__restrict__ Hy = f.H[1];
+  const T *__restrict__ Hz = f.H[2];
+  const T idhx = f.idh[0][ic], idhy = f.idh[1][j];
+  long long id = ((long long)kb * f.NY + j) * SJ + ic;
+  // The fast box starts at index 1 or more on every axis, so the i-1, j-1
+  // and k-1 neighbours are inside the allocation.
+  T hxm = Hx[id - P], hym = Hy[id - P];
+  for (int k = kb; k < ke; ++k, id += P) {
+    const T hx = Hx[id], hy = Hy[id], hz = Hz[id];
+    const T hzy = Hz[id - SJ], hzx = Hz[id - 1], hyx = Hy[id - 1], hxy = Hx[id - SJ];
+    const unsigned cls = IDX ? (unsigned)f.ecls[id] : 0u;
+    const T idhz = f.idh[2][k];
+    e_fast<T, IDX, 0>(f, j, k, vx, id, cls, idhy * (hz - hzy) + idhz * (hym - hy));
+    e_fast<T, IDX, 1>(f, j, k, vx, id, cls, idhz * (hx - hxm) + idhx * (hzx - hz));
+    e_fast<T, IDX, 2>(f, j, k, vx, id, cls, idhx * (hy - hyx) + idhy * (hxy - hx));
+    hxm = hx;
+    hym = hy;
+  }
+}
+
+// Fast H kernel: inner grid (grid.c:800-888).
+template <typename T, bool IDX> __global__ __launch_bounds__(256) void k_h_fast(FastH<T> f)
+{
+  const int i = f.box[0] + (int)blockIdx.x * 64 + (int)threadIdx.x;
+  const int j = f.box[2] + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (j > f.box[3]) return;
+  const bool vx = i <= f.box[1];
+  const int ic = vx ? i : f.box[1];
+  const int kb = f.box[4] + (int)blockIdx.z * f.kchunk;
+  const int ke = min(kb + f.kchunk, f.box[5] + 1);
+  if (kb >= ke) return;
+  const long long P = f.plane, SJ = f.PX;
+  const T *__restrict__ Ex = f.E[0];
+  const T *__restrict__ Ey = f.E[1];
+  const T *__restrict__ Ez = f.E[2];
+  T *__restrict__ H0 = f.H[0];
+  T *__restrict__ H1 = f.H[1];
+  T *__restrict__ H2 = f.H[2];
+  const T idex = f.ide[0][ic], idey = f.ide[1][j];
+  long long id = ((long long)kb * f.NY + j) * SJ + ic;
+  // The fast box ends at index N-2 or less, so i+1, j+1 and k+1 are inside.
+  T ex = Ex[id], ey = Ey[id];
+  for (int k = kb; k < ke; ++k, id += P) {
+    const T exp1 = Ex[id + P], eyp1 = Ey[id + P];
+    const T ez = Ez[id], ezy = Ez[id + SJ], ezx = Ez[id + 1], exy = Ex[id + SJ], eyx = Ey[id + 1];
+    const unsigned cls = IDX ? (unsigned)f.hcls[id] : 0u;
+    const T idez = f.ide[2][k];
+    const T g0 = IDX ? f.htab[cls * 3] : f.gamma[0][id];
+    const T g1 = IDX ? f.htab[cls * 3 + 1] : f.gamma[1][id];
+    const T g2 = IDX ? f.htab[cls * 3 + 2] : f.gamma[2][id];
+    const T h0 = H0[id] + g0 * (idez * (eyp1 - ey) + idey * (ez - ezy));
+    const T h1 = H1[id] + g1 * (idex * (ezx - ez) + idez * (ex - exp1));
+    const T h2 = H2[id] + g2 * (idey * (exy - ex) + idex * (ey - eyx));
+    if (vx) {
+      H0[id] = h0;
+      H1[id] = h1;
+      H2[id] = h2;
+    }
+    ex = exp1;
+    ey = eyp1;
+  }
+}
+
+// Cell (i,j,k) of shell thread t: boxes are x-fastest, laid end to end.
+VG_DEV bool shell_cell(const ShellBoxes &sb, long long t, int &i, int &j, int &k)
+{
+  if (t >= sb.start[sb.nb]) return false;
+  int b = 0;
+  for (int q = 1; q < sb.nb; ++q)
+    if (t >= sb.start[q]) b = q;
+  const int *bx = sb.box[b];
+  const long long l = t - sb.start[b];
+  const long long ni = bx[1] - bx[0] + 1, nj = bx[3] - bx[2] + 1;
+  i = bx[0] + (int)(l % ni);
+  j = bx[2] + (int)((l / ni) % nj);
+  k = bx[4] + (int)(l / (ni * nj));
   return true;
 }
 
-// One E component of one cell: ordinary update, or UPML split update with the
-// axis roles of component C (pml.c:859-1023): P along axis C+1, E along C+2.
-// With IDX the coefficients come from the class table; layer-0 Debye sites
-// are corrected in place (numbered by a wavefront ballot in (component, cell)
-// order, matching the site list).
-template <typename T, bool IDX, int C, typename Curl>
-VG_DEV void e_component(const EArgs<T> &a, const int n3[3], bool vx, long long id, unsigned cls, Curl curlf)
+// One E component of a shell cell: ordinary update, or UPML split update with
+// the axis roles of component C (pml.c:859-1023): P along axis C+1, E along C+2.
+template <typename T, bool IDX, int C>
+VG_DEV void e_shell_component(const EArgs<T> &a, const int n3[3], long long id, unsigned cls)
 {
   constexpr int A1 = (C + 1) % 3, A2 = (C + 2) % 3;
   long long p = 0;
-  const int reg = vx ? classify<1, C>(a, &a.in[C], n3[0], n3[1], n3[2], p) : -1;
-  T *__restrict__ E = a.E[C];
-  T al = T(0), be = T(0);
-  if (vx) {
-    if (IDX) {
-      al = a.etab[cls * 6 + 2 * C];
-      be = a.etab[cls * 6 + 2 * C + 1];
-    } else if (reg >= 0) {
-      al = a.alpha[C][id];
-      be = a.beta[C][id];
-    }
-  }
-  T e = T(0);
-  bool write = false;
-  if (reg == 6) {
-    e = al * E[id] + be * curlf();
-    write = true;
-  } else if (reg >= 0) {
-    const T curl = curlf();
-    T *PP = slab_ptr<C>(a.PP, reg), *PA = slab_ptr<C>(a.P, reg);
-    const T oldPP = PP[p];
-    const T pp = al * oldPP + be * curl;
-    PP[p] = pp;
-    const T oldP = PA[p];
-    const T pn = a.ad[A1][n3[A1]] * oldP + a.bd[A1][n3[A1]] * (pp - oldPP);
-    PA[p] = pn;
-    e = a.ad[A2][n3[A2]] * E[id] + a.bd[A2][n3[A2]] * a.ibh[C][n3[C]] * (pn - a.ah[C][n3[C]] * oldP);
-    write = true;
-  }
-  if (IDX && a.dfuse && debye_fused<T, C>(a, n3, vx, id, cls, be, write, e)) write = true;
-  if (write) E[id] = e;
-}
-
-// Fast path of one E component: every lane of the wavefront is inside the
-// inner grid of all three components (ordinary update only).
-template <typename T, bool IDX, int C>
-VG_DEV void e_fast(const EArgs<T> &a, const int n3[3], bool vx, long long id, unsigned cls, T curl)
-{
-  T *__restrict__ E = a.E[C];
+  const int reg = classify<1>(a, &a.in[C], C, n3[0], n3[1], n3[2], p);
+  if (reg < 0) return;
+  // the curls read the i-1 / j-1 / k-1 neighbours: only cells the sweep
+  // updates evaluate them (never the low ghost layer)
+  const long long P = a.plane, SJ = a.PX;
+  const T *Hx = a.H[0], *Hy = a.H[1], *Hz = a.H[2];
+  const T *idh0 = a.idh[0], *idh1 = a.idh[1], *idh2 = a.idh[2];
+  const int i = n3[0], j = n3[1], k = n3[2];
+  // (the sweep never updates index 0 of any axis; the guards keep every read
+  // inside the allocation regardless)
+  const bool xm = i > 0, ym = j > 0, zm = k > 0;
+  T curl;
+  if (C == 0) curl = idh1[j] * (Hz[id] - (ym ? Hz[id - SJ] : T(0))) + idh2[k] * ((zm ? Hy[id - P] : T(0)) - Hy[id]);
+  else if (C == 1) curl = idh2[k] * (Hx[id] - (zm ? Hx[id - P] : T(0))) + idh0[i] * ((xm ? Hz[id - 1] : T(0)) - Hz[id]);
+  else curl = idh0[i] * (Hy[id] - (xm ? Hy[id - 1] : T(0))) + idh1[j] * ((ym ? Hx[id - SJ] : T(0)) - Hx[id]);
   T al, be;
   if (IDX) {
     al = a.etab[cls * 6 + 2 * C];
@@ -178,77 +249,58 @@ VG_DEV void e_fast(const EArgs<T> &a, const int n3[3], bool vx, long long id, un
     al = a.alpha[C][id];
     be = a.beta[C][id];
   }
-  T e = al * E[id] + be * curl;
-  if (IDX && a.dfuse) debye_fused<T, C>(a, n3, vx, id, cls, be, true, e);
-  if (vx) E[id] = e;
-}
-
-
-// ---------------------------------------------------------------------------
-// E volume sweep: inner grid (grid.c:709-797), UPML slabs (pml.c:859-1023)
-// and, with coefficient classes, the first Debye layer (debye.c:243-307).
-// One wavefront is one 64-cell x segment of one y row.
-// ---------------------------------------------------------------------------
-template <typename T, bool IDX> __global__ __launch_bounds__(256) void k_volume_e(EArgs<T> a)
-{
-  const int x0 = a.x0 + (int)blockIdx.x * 64;
-  const int i = x0 + (int)threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  if (j >= a.NY) return;  // whole wavefront
-  const bool vx = i >= 0 && i < a.NX;
-  const int ic = min(max(i, 0), a.NX - 1);
-  const int kb = a.k0 + blockIdx.z * a.kchunk;
-  const int ke = min(kb + a.kchunk, a.NZ);
-  if (kb >= ke) return;
-  // Wavefront-uniform: do the valid lanes of this row lie in the fast box?
-  const bool fastxy = max(x0, 0) >= a.fast[0] && min(x0 + 63, a.NX - 1) <= a.fast[1] && j >= a.fast[2] &&
-                      j <= a.fast[3];
-  const long long P = a.plane, SJ = a.PX;
-  const T *__restrict__ Hx = a.H[0];
-  const T *__restrict__ Hy = a.H[1];
-  const T *__restrict__ Hz = a.H[2];
-  const T idhx = a.idh[0][ic], idhy = a.idh[1][j];
-  long long id = ((long long)kb * a.NY + j) * SJ + ic;
-  // H at the plane below, carried between iterations.
-  T hxm = (kb > 0) ? Hx[id - P] : T(0), hym = (kb > 0) ? Hy[id - P] : T(0);
-  for (int k = kb; k < ke; ++k, id += P) {
-    const T hx = Hx[id], hy = Hy[id], hz = Hz[id];
-    const T idhz = a.idh[2][k];
-    const unsigned cls = (IDX && vx) ? (unsigned)a.ecls[id] : 0u;
-    const int n3[3] = {i, j, k};
-    if (fastxy && k >= a.fast[4] && k <= a.fast[5]) {
-      // i-1 and j-1 are inside the grid for every lane here (the fast box
-      // starts at 1 and invalid lanes read a clamped in-row cell)
-      e_fast<T, IDX, 0>(a, n3, vx, id, cls, idhy * (hz - Hz[id - SJ]) + idhz * (hym - hy));
-      e_fast<T, IDX, 1>(a, n3, vx, id, cls, idhz * (hx - hxm) + idhx * (Hz[id - 1] - hz));
-      e_fast<T, IDX, 2>(a, n3, vx, id, cls, idhx * (hy - Hy[id - 1]) + idhy * (Hx[id - SJ] - hx));
-    } else {
-      // The curls read the i-1 / j-1 neighbours, so they are evaluated only
-      // for cells the sweep updates (never on the low ghost layer).
-      e_component<T, IDX, 0>(a, n3, vx, id, cls, [&]() { return idhy * (hz - Hz[id - SJ]) + idhz * (hym - hy); });
-      e_component<T, IDX, 1>(a, n3, vx, id, cls, [&]() { return idhz * (hx - hxm) + idhx * (Hz[id - 1] - hz); });
-      e_component<T, IDX, 2>(a, n3, vx, id, cls, [&]() { return idhx * (hy - Hy[id - 1]) + idhy * (Hx[id - SJ] - hx); });
-    }
-    hxm = hx;
-    hym = hy;
+  T *E = a.E[C];
+  if (reg == 6) {
+    E[id] = al * E[id] + be * curl;
+  } else {
+    T *PP = a.PP[reg][C], *PA = a.P[reg][C];
+    const T oldPP = PP[p];
+    const T pp = al * oldPP + be * curl;
+    PP[p] = pp;
+    const T oldP = PA[p];
+    const T pn = a.ad[A1][n3[A1]] * oldP + a.bd[A1][n3[A1]] * (pp - oldPP);
+    PA[p] = pn;
+    E[id] = a.ad[A2][n3[A2]] * E[id] + a.bd[A2][n3[A2]] * a.ibh[C][n3[C]] * (pn - a.ah[C][n3[C]] * oldP);
   }
 }
 
-// One H component: ordinary update (inner grid or Mur normal H), or UPML
-// split update (pml.c:1026-1176): B along axis C+1, H along C+2.
+template <typename T, bool IDX> __global__ __launch_bounds__(256) void k_e_shell(ShellBoxes sb, const EArgs<T> *ap)
+{
+  int i, j, k;
+  if (!shell_cell(sb, blockIdx.x * 256LL + threadIdx.x, i, j, k)) return;
+  const EArgs<T> &a = *ap;
+  const long long id = ((long long)k * a.NY + j) * a.PX + i;
+  const unsigned cls = IDX ? (unsigned)a.ecls[id] : 0u;
+  const int n3[3] = {i, j, k};
+  e_shell_component<T, IDX, 0>(a, n3, id, cls);
+  e_shell_component<T, IDX, 1>(a, n3, id, cls);
+  e_shell_component<T, IDX, 2>(a, n3, id, cls);
+}
+
+// One H component of a shell cell: ordinary update (inner grid or Mur normal
+// H), or UPML split update (pml.c:1026-1176): B along axis C+1, H along C+2.
 template <typename T, bool IDX, int C>
-VG_DEV void h_component(const HArgs<T> &a, const int n3[3], long long id, unsigned cls, T curl)
+VG_DEV void h_shell_component(const HArgs<T> &a, const int n3[3], long long id, unsigned cls)
 {
   constexpr int A1 = (C + 1) % 3, A2 = (C + 2) % 3;
   long long p = 0;
-  const int reg = classify<3, C>(a, a.in[C], n3[0], n3[1], n3[2], p);
+  const int reg = classify<3>(a, a.in[C], C, n3[0], n3[1], n3[2], p);
   if (reg < 0) return;
+  const long long P = a.plane, SJ = a.PX;
+  const T *Ex = a.E[0], *Ey = a.E[1], *Ez = a.E[2];
+  const T *ide0 = a.ide[0], *ide1 = a.ide[1], *ide2 = a.ide[2];
+  const int i = n3[0], j = n3[1], k = n3[2];
+  const bool xin = i + 1 < a.NX, yin = j + 1 < a.NY, zin = k + 1 < a.NZ;
+  T curl;
+  if (C == 0) curl = ide2[k] * ((zin ? Ey[id + P] : T(0)) - Ey[id]) + ide1[j] * (Ez[id] - (yin ? Ez[id + SJ] : T(0)));
+  else if (C == 1) curl = ide0[i] * ((xin ? Ez[id + 1] : T(0)) - Ez[id]) + ide2[k] * (Ex[id] - (zin ? Ex[id + P] : T(0)));
+  else curl = ide1[j] * ((yin ? Ex[id + SJ] : T(0)) - Ex[id]) + ide0[i] * (Ey[id] - (xin ? Ey[id + 1] : T(0)));
   const T g = IDX ? a.htab[cls * 3 + C] : a.gamma[C][id];
-  T *__restrict__ H = a.H[C];
+  T *H = a.H[C];
   if (reg == 6) {
     H[id] = H[id] + g * curl;
   } else {
-    T *Bp = slab_ptr<C>(a.B, reg);
+    T *Bp = a.B[reg][C];
     const T oldB = Bp[p];
     const T bn = a.ah[A1][n3[A1]] * oldB + g * a.bh[A1][n3[A1]] * curl;
     Bp[p] = bn;
@@ -256,66 +308,17 @@ VG_DEV void h_component(const HArgs<T> &a, const int n3[3], long long id, unsign
   }
 }
 
-// ---------------------------------------------------------------------------
-// H volume sweep: inner grid (grid.c:800-888), Mur normal H (mur.c:401-555)
-// and UPML slabs (pml.c:1026-1176)
-// ---------------------------------------------------------------------------
-template <typename T, bool IDX> __global__ __launch_bounds__(256) void k_volume_h(HArgs<T> a)
+template <typename T, bool IDX> __global__ __launch_bounds__(256) void k_h_shell(ShellBoxes sb, const HArgs<T> *ap)
 {
-  const int x0 = a.x0 + (int)blockIdx.x * 64;
-  const int i = x0 + (int)threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  if (j >= a.NY) return;  // whole wavefront
-  const bool vx = i >= 0 && i < a.NX;
-  const int ic = min(max(i, 0), a.NX - 1);
-  const int kb = a.k0 + blockIdx.z * a.kchunk;
-  const int ke = min(kb + a.kchunk, a.NZ);
-  if (kb >= ke) return;
-  const bool fastxy = max(x0, 0) >= a.fast[0] && min(x0 + 63, a.NX - 1) <= a.fast[1] && j >= a.fast[2] &&
-                      j <= a.fast[3];
-  const long long P = a.plane, SJ = a.PX;
-  const T *__restrict__ Ex = a.E[0];
-  const T *__restrict__ Ey = a.E[1];
-  const T *__restrict__ Ez = a.E[2];
-  const bool xin = ic + 1 < a.NX, yin = j + 1 < a.NY;
-  const T idex = a.ide[0][ic], idey = a.ide[1][j];
-  long long id = ((long long)kb * a.NY + j) * SJ + ic;
-  T ex = Ex[id], ey = Ey[id];
-  for (int k = kb; k < ke; ++k, id += P) {
-    const bool zin = k + 1 < a.NZ;
-    const T exp1 = zin ? Ex[id + P] : T(0), eyp1 = zin ? Ey[id + P] : T(0);
-    const T ez = Ez[id];
-    const T idez = a.ide[2][k];
-    const unsigned cls = (IDX && vx) ? (unsigned)a.hcls[id] : 0u;
-    const T ezp = yin ? Ez[id + SJ] : T(0);
-    const T ezx = xin ? Ez[id + 1] : T(0);
-    const T exy = yin ? Ex[id + SJ] : T(0);
-    const T eyx = xin ? Ey[id + 1] : T(0);
-    const T c0 = idez * (eyp1 - ey) + idey * (ez - ezp);
-    const T c1 = idex * (ezx - ez) + idez * (ex - exp1);
-    const T c2 = idey * (exy - ex) + idex * (ey - eyx);
-    if (fastxy && k >= a.fast[4] && k <= a.fast[5]) {
-      T *__restrict__ H0 = a.H[0];
-      T *__restrict__ H1 = a.H[1];
-      T *__restrict__ H2 = a.H[2];
-      const T g0 = IDX ? a.htab[cls * 3] : a.gamma[0][id];
-      const T g1 = IDX ? a.htab[cls * 3 + 1] : a.gamma[1][id];
-      const T g2 = IDX ? a.htab[cls * 3 + 2] : a.gamma[2][id];
-      const T h0 = H0[id] + g0 * c0, h1 = H1[id] + g1 * c1, h2 = H2[id] + g2 * c2;
-      if (vx) {
-        H0[id] = h0;
-        H1[id] = h1;
-        H2[id] = h2;
-      }
-    } else if (vx) {
-      const int n3[3] = {i, j, k};
-      h_component<T, IDX, 0>(a, n3, id, cls, c0);
-      h_component<T, IDX, 1>(a, n3, id, cls, c1);
-      h_component<T, IDX, 2>(a, n3, id, cls, c2);
-    }
-    ex = exp1;
-    ey = eyp1;
-  }
+  int i, j, k;
+  if (!shell_cell(sb, blockIdx.x * 256LL + threadIdx.x, i, j, k)) return;
+  const HArgs<T> &a = *ap;
+  const long long id = ((long long)k * a.NY + j) * a.PX + i;
+  const unsigned cls = IDX ? (unsigned)a.hcls[id] : 0u;
+  const int n3[3] = {i, j, k};
+  h_shell_component<T, IDX, 0>(a, n3, id, cls);
+  h_shell_component<T, IDX, 1>(a, n3, id, cls);
+  h_shell_component<T, IDX, 2>(a, n3, id, cls);
 }
 
 // ---------------------------------------------------------------------------

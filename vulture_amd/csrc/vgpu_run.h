@@ -1,8 +1,10 @@
 // Per-precision time-step driver: launches one step's kernels in the
 // reference's order (simulation.c:157-222):
-//   observers -> Mur E -> E sweep (inner grid + UPML) -> Debye -> SIBC E ->
-//   aux grids E -> sources/TF-SF E -> ghost E -> H sweep (inner grid + Mur
-//   normal H + UPML) -> SIBC H -> aux grids H -> sources/TF-SF H -> ghost H
+//   observers -> Mur E -> E sweep (fast box with the first Debye layer, then
+//   the shell: inner-grid edges and UPML) -> remaining Debye layers -> SIBC E
+//   -> aux grids E -> sources/TF-SF E -> ghost E -> H sweep (fast box, shell:
+//   Mur normal H and UPML) -> SIBC H -> aux grids H -> sources/TF-SF H ->
+//   ghost H
 #pragma once
 #include "vgpu_kernels.h"
 
@@ -10,17 +12,13 @@ namespace vgpu {
 
 template <typename T> static T *P(void *p) { return static_cast<T *>(p); }
 
-template <typename T> static void fill_volume_args(Ctx *c, EArgs<T> &ea, HArgs<T> &ha, int kchunk)
+template <typename T> static void fill_volume_args(Ctx *c, EArgs<T> &ea, HArgs<T> &ha)
 {
   ea.NX = ha.NX = c->NX;
   ea.NY = ha.NY = c->NY;
   ea.NZ = ha.NZ = c->NZ;
   ea.PX = ha.PX = c->PX;
   ea.plane = ha.plane = c->plane;
-  ea.k0 = ha.k0 = 0;
-  ea.kchunk = ha.kchunk = kchunk;
-  ea.x0 = ha.x0 = c->X0;
-  for (int q = 0; q < 6; ++q) { ea.fast[q] = c->fastE[q]; ha.fast[q] = c->fastH[q]; }
   for (int d = 0; d < 3; ++d) {
     ea.E[d] = P<T>(c->F[d]);
     ea.H[d] = P<T>(c->F[3 + d]);
@@ -58,7 +56,7 @@ template <typename T> static void fill_volume_args(Ctx *c, EArgs<T> &ea, HArgs<T
   ha.hcls = P<uint16_t>(c->d_hcls); ha.htab = P<T>(c->d_htab);
   ea.dfuse = c->debyeFused ? 1 : 0;
   if (c->debyeFused) {
-    ea.dbase = P<int>(c->d_dbase); ea.NCH = c->NCH;
+    ea.dbase = P<int>(c->d_dbase);
     ea.dlast = P<T>(c->d_dlast); ea.dJ = P<T>(c->d_dJ); ea.dstride = c->debyeSites;
     ea.dalpha = P<double>(c->d_dalpha); ea.dbeta = P<double>(c->d_dbeta); ea.dnp = P<int>(c->d_dnp);
     ea.dmaxP = c->debyeMaxPoles; ea.dcplx = c->debyeCplx;
@@ -76,23 +74,128 @@ template <typename T> static void fill_volume_args(Ctx *c, EArgs<T> &ea, HArgs<T
     }
 }
 
+// Shell of a fast box inside [0, N-1]^3: up to six boxes (two x slabs of
+// full y/z extent, two y slabs over the box's x range, two z slabs over its
+// x-y range); the whole allocation when the box is empty.
+static ShellBoxes shell_of(const Ctx *c, const int *fb)
+{
+  ShellBoxes sb{};
+  const int N[3] = {c->NX, c->NY, c->NZ};
+  auto add = [&](int x0, int x1, int y0, int y1, int z0, int z1) {
+    if (x1 < x0 || y1 < y0 || z1 < z0) return;
+    int *b = sb.box[sb.nb];
+    b[0] = x0; b[1] = x1; b[2] = y0; b[3] = y1; b[4] = z0; b[5] = z1;
+    sb.start[sb.nb + 1] = sb.start[sb.nb] + (long long)(x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1);
+    ++sb.nb;
+  };
+  if (fb[0] > fb[1] || fb[2] > fb[3] || fb[4] > fb[5]) {
+    add(0, N[0] - 1, 0, N[1] - 1, 0, N[2] - 1);
+    return sb;
+  }
+  add(0, fb[0] - 1, 0, N[1] - 1, 0, N[2] - 1);
+  add(fb[1] + 1, N[0] - 1, 0, N[1] - 1, 0, N[2] - 1);
+  add(fb[0], fb[1], 0, fb[2] - 1, 0, N[2] - 1);
+  add(fb[0], fb[1], fb[3] + 1, N[1] - 1, 0, N[2] - 1);
+  add(fb[0], fb[1], fb[2], fb[3], 0, fb[4] - 1);
+  add(fb[0], fb[1], fb[2], fb[3], fb[5] + 1, N[2] - 1);
+  return sb;
+}
+
+// Once per context (at finalize): the sweeps' argument blocks go to device
+// memory, where the shell kernels and the fused Debye step read them.
+template <typename T> int prepare_sweeps(Ctx *c)
+{
+  // The fast box and its shell must tile the allocation exactly once, with
+  // every box inside it (the shell kernels index from these boxes).
+  for (int e = 0; e < 2; ++e) {
+    const int *fb = e == 0 ? c->fastE : c->fastH;
+    const ShellBoxes sb = shell_of(c, fb);
+    long long n = 0;
+    if (fb[0] <= fb[1] && fb[2] <= fb[3] && fb[4] <= fb[5]) {
+      if (fb[0] < 1 || fb[2] < 1 || fb[4] < 1 || fb[1] > c->NX - 2 || fb[3] > c->NY - 2 || fb[5] > c->NZ - 2)
+        return set_error(c, VGPU_EINVAL, "fast box not inside the grid");
+      n = (long long)(fb[1] - fb[0] + 1) * (fb[3] - fb[2] + 1) * (fb[5] - fb[4] + 1);
+    }
+    const int N[3] = {c->NX, c->NY, c->NZ};
+    for (int b = 0; b < sb.nb; ++b)
+      for (int a = 0; a < 3; ++a)
+        if (sb.box[b][2 * a] < 0 || sb.box[b][2 * a + 1] >= N[a] || sb.box[b][2 * a] > sb.box[b][2 * a + 1])
+          return set_error(c, VGPU_EINVAL, "shell box outside the grid");
+    if (n + sb.start[sb.nb] != (long long)c->NX * c->NY * c->NZ)
+      return set_error(c, VGPU_EINVAL, "fast box and shell do not tile the grid");
+  }
+  EArgs<T> ea{};
+  HArgs<T> ha{};
+  fill_volume_args<T>(c, ea, ha);
+  c->d_eargs = dev_alloc(c, sizeof(ea));
+  c->d_hargs = dev_alloc(c, sizeof(ha));
+  if (!c->d_eargs || !c->d_hargs) return set_error(c, VGPU_ENOMEM, "sweep arguments");
+  if (hipMemcpy(c->d_eargs, &ea, sizeof(ea), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_hargs, &ha, sizeof(ha), hipMemcpyHostToDevice) != hipSuccess)
+    return set_error(c, VGPU_EDEVICE, "sweep arguments upload");
+  return VGPU_OK;
+}
+
 template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool timeKernels)
 {
   const double pi = 4.0 * atan(1.0);
   const double dt = c->grid.dt;
   timeKernels = timeKernels && !c->evE.empty();
-  // Volume launch: 64x4 tiles, x chunks aligned to the fast box.
+  EArgs<T> ea{};
+  HArgs<T> ha{};
+  fill_volume_args<T>(c, ea, ha);
+  // Fast kernels: 64x4 tiles over the fast boxes, marching z in chunks sized
+  // for about 16k blocks at 272^3 (at least 4 planes so the carry pays).
   const dim3 vblk(64, 4, 1);
-  const int gx = c->NCH, gy = (c->NY + 3) / 4;
-  // z planes per block: short marches (about 16k blocks at 272^3) keep many
-  // rows in flight; at least 4 planes so the register carry pays.
-  int kchunk = (int)((long long)c->NZ * gx * gy / 16384);
-  if (kchunk < 4) kchunk = 4;
-  if (kchunk > 32) kchunk = 32;
-  const dim3 vgrid(gx, gy, (c->NZ + kchunk - 1) / kchunk);
-  EArgs<T> ea;
-  HArgs<T> ha;
-  fill_volume_args<T>(c, ea, ha, kchunk);
+  auto fast_grid = [&](const int *fb, int &kchunk) {
+    if (fb[0] > fb[1] || fb[2] > fb[3] || fb[4] > fb[5]) return dim3(0, 0, 0);
+    const int gx = (fb[1] - fb[0] + 64) / 64, gy = (fb[3] - fb[2] + 4) / 4, nz = fb[5] - fb[4] + 1;
+    kchunk = (int)((long long)nz * gx * gy / 16384);
+    kchunk = std::max(4, std::min(kchunk, 32));
+    return dim3(gx, gy, (nz + kchunk - 1) / kchunk);
+  };
+  FastE<T> fe{};
+  FastH<T> fh{};
+  const dim3 egrid = fast_grid(c->fastE, fe.kchunk), hgrid = fast_grid(c->fastH, fh.kchunk);
+  fe.NX = fh.NX = c->NX; fe.NY = fh.NY = c->NY; fe.NZ = fh.NZ = c->NZ; fe.PX = fh.PX = c->PX;
+  fe.plane = fh.plane = c->plane;
+  for (int q = 0; q < 6; ++q) { fe.box[q] = c->fastE[q]; fh.box[q] = c->fastH[q]; }
+  fe.nchx = egrid.x; fh.nchx = hgrid.x;
+  for (int d = 0; d < 3; ++d) {
+    fe.E[d] = ea.E[d]; fe.H[d] = ea.H[d]; fe.idh[d] = ea.idh[d]; fe.alpha[d] = ea.alpha[d]; fe.beta[d] = ea.beta[d];
+    fh.H[d] = ha.H[d]; fh.E[d] = ha.E[d]; fh.ide[d] = ha.ide[d]; fh.gamma[d] = ha.gamma[d];
+  }
+  fe.ecls = ea.ecls; fe.etab = ea.etab; fe.dfuse = ea.dfuse; fe.ap = P<EArgs<T>>(c->d_eargs);
+  fh.hcls = ha.hcls; fh.htab = ha.htab;
+  const ShellBoxes esh = shell_of(c, c->fastE), hsh = shell_of(c, c->fastH);
+  const EArgs<T> *eap = P<EArgs<T>>(c->d_eargs);
+  const HArgs<T> *hap = P<HArgs<T>>(c->d_hargs);
+  auto fast_e = [&]() {
+    if (egrid.x) {
+      if (c->indexed) hipLaunchKernelGGL((k_e_fast<T, true>), egrid, vblk, 0, s, fe);
+      else hipLaunchKernelGGL((k_e_fast<T, false>), egrid, vblk, 0, s, fe);
+    }
+  };
+  auto shell_e = [&]() {
+    if (esh.nb) {
+      const dim3 g((unsigned)((esh.start[esh.nb] + 255) / 256));
+      if (c->indexed) hipLaunchKernelGGL((k_e_shell<T, true>), g, dim3(256), 0, s, esh, eap);
+      else hipLaunchKernelGGL((k_e_shell<T, false>), g, dim3(256), 0, s, esh, eap);
+    }
+  };
+  auto fast_h = [&]() {
+    if (hgrid.x) {
+      if (c->indexed) hipLaunchKernelGGL((k_h_fast<T, true>), hgrid, vblk, 0, s, fh);
+      else hipLaunchKernelGGL((k_h_fast<T, false>), hgrid, vblk, 0, s, fh);
+    }
+  };
+  auto shell_h = [&]() {
+    if (hsh.nb) {
+      const dim3 g((unsigned)((hsh.start[hsh.nb] + 255) / 256));
+      if (c->indexed) hipLaunchKernelGGL((k_h_shell<T, true>), g, dim3(256), 0, s, hsh, hap);
+      else hipLaunchKernelGGL((k_h_shell<T, false>), g, dim3(256), 0, s, hsh, hap);
+    }
+  };
 
   MurArgs<T> ma{};
   int murmax1 = 1, murmax2 = 1;
@@ -242,13 +345,15 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
     }
     if (c->murmask)
       hipLaunchKernelGGL(k_mur_e<T>, dim3((murmax1 + 15) / 16, (murmax2 + 15) / 16, 12), dim3(16, 16), 0, s, ma);
+    // events bracket the fast kernels (the dominant launches)
     if (timeKernels && ne < (int)c->evE.size() / 2) hipEventRecord(c->evE[2 * ne], s);
-    if (c->indexed) hipLaunchKernelGGL((k_volume_e<T, true>), vgrid, vblk, 0, s, ea);
-    else hipLaunchKernelGGL((k_volume_e<T, false>), vgrid, vblk, 0, s, ea);
+    fast_e();
     if (timeKernels && ne < (int)c->evE.size() / 2) { hipEventRecord(c->evE[2 * ne + 1], s); ++ne; }
+    shell_e();
     for (int l = c->debyeFused ? 1 : 0; l < c->debyeLayers; ++l) {
       da.base = c->debyeLayerStart[l];
       da.n = c->debyeLayerStart[l + 1] - c->debyeLayerStart[l];
+      if (da.n == 0) continue;
       hipLaunchKernelGGL(k_debye<T>, dim3((unsigned)((da.n + 255) / 256)), dim3(256), 0, s, da);
     }
     if (c->sibcFaces)
@@ -263,9 +368,9 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
     }
     ghost(1);
     if (timeKernels && nh < (int)c->evH.size() / 2) hipEventRecord(c->evH[2 * nh], s);
-    if (c->indexed) hipLaunchKernelGGL((k_volume_h<T, true>), vgrid, vblk, 0, s, ha);
-    else hipLaunchKernelGGL((k_volume_h<T, false>), vgrid, vblk, 0, s, ha);
+    fast_h();
     if (timeKernels && nh < (int)c->evH.size() / 2) { hipEventRecord(c->evH[2 * nh + 1], s); ++nh; }
+    shell_h();
     if (c->sibcFaces)
       hipLaunchKernelGGL(k_sibc_h<T>, dim3((unsigned)((sh.nrows + 255) / 256)), dim3(256), 0, s, sh);
     if (anyAux) hipLaunchKernelGGL(k_aux_h<T>, dim3(xa.npw), dim3(256), 0, s, xa);
