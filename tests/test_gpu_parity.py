@@ -141,10 +141,10 @@ def test_full_coefficient_arrays_path(name, tmp_path, monkeypatch):
 @pytest.mark.parametrize("name,chunk", [("parplate1D_planewave_xyz", 500), ("planewave_empty_xyz", 64),
                                         ("closedbox_go04", 100), ("parplate1D_debyepml_xyz", 700),
                                         ("dgref_mur", 90)])
-def test_chunked_runs_replay_captured_graphs(name, chunk, tmp_path):
-    """Runs of `chunk` steps after the first are replayed from a captured
-    graph whose kernels read the step number from the device counter; the
-    results (binary volume observers included) match the oracle."""
+def test_chunked_runs(name, chunk, tmp_path):
+    """The loop run as consecutive runs of `chunk` steps, the kernels reading
+    the step number from the device counter, matches the oracle (binary
+    volume observers included)."""
     mesh = _stage(name, tmp_path / "in")
     print(f"case {name} chunk {chunk}", flush=True)
     g, o = tmp_path / "gpu", tmp_path / "oracle"

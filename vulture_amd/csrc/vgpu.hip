@@ -1039,11 +1039,15 @@ int vgpu_run(vgpu_ctx *c, uint64_t first, uint64_t nsteps, void *stream)
   if (c->stepHost != first)
     hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, s, (unsigned long long *)c->d_step,
                        (unsigned long long)first);
-  // Runs of the staging length are replayed from a captured graph once the
-  // solve is long enough to replay it at least twice (not when the kernels
-  // are being timed individually).
-  const bool graphable = s == c->stream && c->evE.empty() && nsteps >= 2 && nsteps == c->maxRun &&
-                         c->totalSteps >= 2 * nsteps && getenv("VGPU_NO_GRAPHS") == nullptr;
+  // Runs of the staging length may be replayed from a captured graph once
+  // the solve is long enough to replay it at least twice.  Capture is off
+  // unless VGPU_GRAPHS=1: with captures running while other host threads
+  // allocated, copied and freed device memory (the solve server), results
+  // diverged and the device faulted, so it stays off until that interaction
+  // is understood.
+  const char *ge = getenv("VGPU_GRAPHS");
+  const bool graphable = ge && ge[0] == '1' && s == c->stream && c->evE.empty() && nsteps >= 2 &&
+                         nsteps == c->maxRun && c->totalSteps >= 2 * nsteps;
   int r = VGPU_OK;
   if (graphable && !c->graphExec && !c->graphTried) capture_graph(c, nsteps);
   if (graphable && c->graphExec && c->graphSteps == nsteps) {
