@@ -158,3 +158,18 @@ def test_chunked_runs(name, chunk, tmp_path):
         assert np.array_equal(ra["ijk"], rb["ijk"])
         fa, fb = ra["f"].astype(np.float64), rb["f"].astype(np.float64)
         assert np.linalg.norm(fa - fb) <= 1e-6 * max(np.linalg.norm(fb), 1e-30)
+
+
+@pytest.mark.parametrize("name", ["parplate1D_debye_xyz", "parplate1D_debyepml_xyz", "parplate1D_debyeslab_zyx"])
+def test_fused_debye_layer(name, tmp_path, monkeypatch):
+    """The first Debye layer applied inside the fast E kernel (numbered by a
+    wavefront ballot) gives the oracle's results."""
+    monkeypatch.setenv("VGPU_DEBYE_FUSION", "1")
+    mesh = _stage(name, tmp_path / "in")
+    _check(_run_pair(mesh, tmp_path, "fp64"), "fp64")
+
+
+def test_fused_debye_spheres(tmp_path, monkeypatch):
+    monkeypatch.setenv("VGPU_DEBYE_FUSION", "1")
+    mesh = meshgen.write("debye", str(tmp_path), n=40, spheres=6, rmin=3, rmax=8, steps=150, seed=7)
+    _check(_run_pair(mesh, tmp_path, "fp64"), "fp64")

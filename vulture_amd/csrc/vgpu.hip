@@ -622,7 +622,13 @@ static int build_class_tables(Ctx *c)
     if (!c->d_ecls || !c->d_hcls || !c->d_etab || !c->d_htab || !c->d_edslot)
       return set_error(c, VGPU_ENOMEM, "coefficient class allocation");
   }
-  c->debyeFused = c->indexed && c->debyeSites > 0 && getenv("VGPU_NO_DEBYE_FUSION") == nullptr;
+  // Applying the first Debye layer inside the fast E kernel saves the
+  // separate pass's field traffic but serialises three dependent table
+  // look-ups per component on Debye rows; measured on the Debye-sphere
+  // workload it is slower, so the separate pass is the default and the
+  // fused one is kept behind VGPU_DEBYE_FUSION=1.
+  const char *df = getenv("VGPU_DEBYE_FUSION");
+  c->debyeFused = c->indexed && c->debyeSites > 0 && df && df[0] == '1';
   std::vector<uint32_t>().swap(c->ecls);
   std::vector<uint32_t>().swap(c->hcls);
   return VGPU_OK;
