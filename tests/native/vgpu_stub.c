@@ -9,6 +9,14 @@
 #include <string.h>
 #include "../../include/vgpu.h"
 
+/* The set-up of the last context, kept when vgpu_stub_capture is set, for
+ * single-threaded checkers that replay the library's host-side planning
+ * (tests/native/shell_check.cpp). */
+int vgpu_stub_capture;
+vgpu_grid_desc vgpu_stub_grid;
+unsigned vgpu_stub_pmlmask, vgpu_stub_murmask;
+int vgpu_stub_fplim[6][6][6], vgpu_stub_fmlim[6][6][6];
+
 struct vgpu_ctx {
   int finalized, npt, nvol, polls;
   size_t bytes;
@@ -21,15 +29,36 @@ int vgpu_create(const vgpu_grid_desc *g, vgpu_prec prec, int device, vgpu_ctx **
 {
   (void)device;
   vgpu_ctx *c = calloc(1, sizeof(*c));
+  if (vgpu_stub_capture) {
+    vgpu_stub_grid = *g;
+    vgpu_stub_pmlmask = vgpu_stub_murmask = 0;
+    memset(vgpu_stub_fplim, 0, sizeof(vgpu_stub_fplim));
+    memset(vgpu_stub_fmlim, 0, sizeof(vgpu_stub_fmlim));
+  }
   c->bytes = (size_t)g->nx * g->ny * g->nz * (prec == VGPU_FP64 ? 8 : 4) * 15;
   *out = c;
   return 0;
 }
 int vgpu_set_media(vgpu_ctx *c, const double *a[3], const double *b[3], const double *g[3])
 { (void)c; (void)a; (void)b; (void)g; return 0; }
-int vgpu_set_pml(vgpu_ctx *c, const vgpu_pml_desc *p) { (void)c; (void)p; return 0; }
+int vgpu_set_pml(vgpu_ctx *c, const vgpu_pml_desc *p)
+{
+  (void)c;
+  if (vgpu_stub_capture) {
+    vgpu_stub_pmlmask = p->region_mask & 0x3Fu;
+    memcpy(vgpu_stub_fplim, p->fplim, sizeof(vgpu_stub_fplim));
+  }
+  return 0;
+}
 int vgpu_set_mur(vgpu_ctx *c, unsigned m, const double z[6], const int f[6][6][6])
-{ (void)c; (void)m; (void)z; (void)f; return 0; }
+{
+  (void)c; (void)z;
+  if (vgpu_stub_capture) {
+    vgpu_stub_murmask = m & 0x3Fu;
+    memcpy(vgpu_stub_fmlim, f, sizeof(vgpu_stub_fmlim));
+  }
+  return 0;
+}
 int vgpu_set_ghosts(vgpu_ctx *c, const int t[6]) { (void)c; (void)t; return 0; }
 int vgpu_add_debye_block(vgpu_ctx *c, const int f[3][6], int n, const double *a, const double *b)
 { (void)c; (void)f; (void)n; (void)a; (void)b; return 0; }

@@ -209,18 +209,9 @@ int vgpu_create(const vgpu_grid_desc *g, vgpu_prec prec, int device, vgpu_ctx **
   c->grid.dex = c->grid.dey = c->grid.dez = nullptr;
   c->grid.idex = c->grid.idey = c->grid.idez = nullptr;
   c->grid.idhx = c->grid.idhy = c->grid.idhz = nullptr;
-  // Fast boxes: where every E (H) component takes the ordinary update, kept
-  // one cell inside the allocation so the fast kernels never test bounds.
-  for (int e = 0; e < 2; ++e) {
-    int *fb = e == 0 ? c->fastE : c->fastH;
-    for (int a = 0; a < 3; ++a) {
-      fb[2 * a] = 1;
-      fb[2 * a + 1] = (a == 0 ? c->NX : a == 1 ? c->NY : c->NZ) - 2;
-      for (int d = 0; d < 3; ++d) {
-        fb[2 * a] = std::max(fb[2 * a], g->gfilim[3 * e + d][2 * a]);
-        fb[2 * a + 1] = std::min(fb[2 * a + 1], g->gfilim[3 * e + d][2 * a + 1]);
-      }
-    }
+  {
+    const int N[3] = {c->NX, c->NY, c->NZ};
+    fast_boxes(g->gfilim, N, c->fastE, c->fastH);
   }
   c->NCH = c->fastE[0] <= c->fastE[1] ? (c->fastE[1] - c->fastE[0] + 64) / 64 : 0;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -880,24 +871,34 @@ static int build_corrections(Ctx *c)
     }
   for (int e = 0; e < 2; ++e) {
     std::stable_sort(rows[e].begin(), rows[e].end(), [](const RowEntry &a, const RowEntry &b) { return a.key < b.key; });
-    std::vector<int> rcomp, rstart;
-    std::vector<long long> rcell;
-    std::vector<CorrEntry> ents;
+    std::vector<int> rstart;
+    std::vector<long long> rkey;
+    std::vector<CorrEnt> ents;
     for (size_t q = 0; q < rows[e].size(); ++q) {
       if (q == 0 || rows[e][q].key != rows[e][q - 1].key) {
-        rcomp.push_back((int)(rows[e][q].key % 3));
-        rcell.push_back(rows[e][q].key / 3);
+        rkey.push_back(rows[e][q].key / 3 << 2 | rows[e][q].key % 3);
         rstart.push_back((int)ents.size());
       }
-      ents.push_back(rows[e][q].e);
+      const CorrEntry &ce = rows[e][q].e;
+      if (ce.idx < 0 || ce.idx >= (1 << 23)) return set_error(c, VGPU_EINVAL, "too many sources or plane waves");
+      CorrEnt d{};
+      d.code = ce.kind | ce.inc << 2 | (ce.sign < 0 ? 1 : 0) << 5 | ce.idx << 8;
+      d.auxid = ce.auxid;
+      d.geo = ce.geo;
+      d.il = ce.il;
+      ents.push_back(d);
     }
     rstart.push_back((int)ents.size());
-    c->corrRows[e] = (long long)rcomp.size();
+    if (ents.size() >= (size_t)1 << 31) return set_error(c, VGPU_EINVAL, "too many correction entries");
+    c->corrRows[e] = (long long)rkey.size();
     if (c->corrRows[e]) {
-      c->d_ccomp[e] = upload_vec(c, rcomp);
-      c->d_ccell[e] = upload_vec(c, rcell);
+      c->d_ckey[e] = upload_vec(c, rkey);
       c->d_cstart[e] = upload_vec(c, rstart);
       c->d_cent[e] = upload_vec(c, ents);
+      c->d_ccoef[e] = dev_alloc(c, rkey.size() * c->esz);
+      c->d_calpha[e] = dev_alloc(c, rkey.size() * c->esz);
+      if (!c->d_ckey[e] || !c->d_cstart[e] || !c->d_cent[e] || !c->d_ccoef[e] || !c->d_calpha[e])
+        return set_error(c, VGPU_ENOMEM, "correction rows");
     }
   }
   // waveform, source and plane-wave tables

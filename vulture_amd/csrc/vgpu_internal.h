@@ -67,6 +67,7 @@ template <typename T> struct FastE {
   long long plane;
   int box[6];            // fast box (every E component inner-grid)
   int kchunk, nchx;      // planes per block; 64-cell x chunks from box[0]
+  int gy, gz;            // 4-row y blocks, z chunks (blocks = nchx*gy*gz, launched in XCD order)
   T *E[3];
   const T *H[3];
   const T *idh[3];
@@ -82,6 +83,7 @@ template <typename T> struct FastH {
   long long plane;
   int box[6];
   int kchunk, nchx;
+  int gy, gz;
   T *H[3];
   const T *E[3];
   const T *ide[3];
@@ -90,11 +92,27 @@ template <typename T> struct FastH {
   const T *gamma[3];
 };
 
-// Shell of a sweep: up to six boxes, x-fastest, laid end to end.
-struct ShellBoxes {
-  int nb;
-  int box[6][6];
-  long long start[7];    // first thread of each box; start[nb] = total
+// A piece of a sweep's shell (the cells outside its fast box): a box whose
+// rows share, per component, one sequence of x runs, each run one region
+// (ordinary update or one UPML slab) over a range of x.  Cells run x-fastest,
+// 256 to a block.
+constexpr int SHELL_RUNS = 8;
+struct ShellRun {
+  int xa, xb;            // inclusive x range
+  int reg, pad_;         // 6 ordinary, 0..5 UPML slab
+  long long p0;          // UPML: slab index of (xa, box[2], box[4])
+  long long psy, psz;    // slab index strides of j and k (1 for i)
+  void *a, *b;           // UPML auxiliaries of the slab: E sweep PP and P, H sweep B
+};
+struct ShellPiece {
+  int box[6];            // inclusive cell limits
+  int ni, nj, cells;     // x and y extents, cell count
+  int nrun[3];           // runs per component (cells outside every run are not updated)
+  ShellRun run[3][SHELL_RUNS];
+};
+
+struct ShellWork {
+  int piece, block;
 };
 
 // Mur tangential-E faces (mur.c:135-395): one box per (face, component).
@@ -190,6 +208,16 @@ struct CorrEntry {
   double geo;            // dp (aux grid) or d/v (analytic)
 };
 
+// Device form of a correction entry (24 bytes).  code: kind (bits 0-1),
+// incident component (bits 2-4), negative sign (bit 5), source or plane-wave
+// index (bits 8-31).
+struct CorrEnt {
+  int code;
+  int auxid;
+  double geo;            // dp (aux grid) or d/v (analytic)
+  double il;             // inverse edge length (TF/SF)
+};
+
 struct WaveformDev {
   int type;
   double size, delay, width, frequency;
@@ -212,13 +240,12 @@ struct PwDev {
 
 template <typename T> struct CorrArgs {
   long long nrows;
-  const int *rowcomp;
-  const long long *rowcell;
+  const long long *rowkey;   // cell << 2 | component, ascending
   const int *rowstart;
-  const CorrEntry *ent;
+  const CorrEnt *ent;
+  const T *rowcoef;      // the row cell's beta (E) or gamma (H)
+  const T *rowalpha;     // the row cell's alpha (E only)
   T *F[3];               // E or H components
-  const T *alpha[3];     // E only
-  const T *coef[3];      // beta (E) or gamma (H)
   int electric;
   const unsigned long long *step;  // device step counter: n = *step + soff
   int soff;
@@ -285,6 +312,24 @@ template <typename T> struct GhostArgs {
 
 struct Region { int lim[6]; };
 
+// Fast boxes of the two sweeps: where every E (H) component takes the
+// ordinary update (the intersection of the inner-grid limits gfilim), kept
+// one cell inside the allocation so the fast kernels never test bounds.
+inline void fast_boxes(const int gfilim[6][6], const int N[3], int fastE[6], int fastH[6])
+{
+  for (int e = 0; e < 2; ++e) {
+    int *fb = e == 0 ? fastE : fastH;
+    for (int a = 0; a < 3; ++a) {
+      fb[2 * a] = 1;
+      fb[2 * a + 1] = N[a] - 2;
+      for (int d = 0; d < 3; ++d) {
+        fb[2 * a] = fb[2 * a] > gfilim[3 * e + d][2 * a] ? fb[2 * a] : gfilim[3 * e + d][2 * a];
+        fb[2 * a + 1] = fb[2 * a + 1] < gfilim[3 * e + d][2 * a + 1] ? fb[2 * a + 1] : gfilim[3 * e + d][2 * a + 1];
+      }
+    }
+  }
+}
+
 struct Ctx {
   std::string err;
   vgpu_prec prec;
@@ -327,6 +372,10 @@ struct Ctx {
   int NCH = 0;                               // 64-cell x chunks of the fast E box
   int fastE[6] = {}, fastH[6] = {};          // boxes of the sweeps' fast kernels
   void *d_eargs = nullptr, *d_hargs = nullptr;  // argument blocks in device memory
+  void *d_epcs = nullptr, *d_ework = nullptr;   // E shell: pieces, launch blocks
+  void *d_hpcs = nullptr, *d_hwork = nullptr;   // H shell
+  int nEwork = 0, nHwork = 0, nEpcs = 0, nHpcs = 0;
+  long long shellCells[2] = {};              // cells of the E and H shell pieces
   void *d_dbase = nullptr;                   // [3][NZ][NY][NCH] + 1: first layer-0 site
 
   // UPML
@@ -375,7 +424,7 @@ struct Ctx {
   void *d_wf = nullptr, *d_src = nullptr, *d_pw = nullptr, *d_aux = nullptr, *d_auxpml = nullptr, *d_auxprof = nullptr;
   int auxmax = 0;
   long long corrRows[2] = {0, 0};
-  void *d_ccomp[2] = {}, *d_ccell[2] = {}, *d_cstart[2] = {}, *d_cent[2] = {};
+  void *d_ckey[2] = {}, *d_cstart[2] = {}, *d_cent[2] = {}, *d_ccoef[2] = {}, *d_calpha[2] = {};
 
   // observers
   struct PObs { int i, j, k; uint64_t first, last; std::vector<double> omega; };

@@ -10,13 +10,14 @@
 namespace vgpu {
 
 #define VG_DEV __device__ __forceinline__
+#define VG_HD __host__ __device__ __forceinline__
 
-VG_DEV bool in_box(const int *b, int i, int j, int k)
+VG_HD bool in_box(const int *b, int i, int j, int k)
 {
   return i >= b[0] && i <= b[1] && j >= b[2] && j <= b[3] && k >= b[4] && k <= b[5];
 }
 
-template <typename T> VG_DEV long long slab_index(const int *b, int i, int j, int k)
+template <typename T> VG_HD long long slab_index(const int *b, int i, int j, int k)
 {
   long long ni = b[1] - b[0] + 1, nj = b[3] - b[2] + 1;
   return ((long long)(k - b[4]) * nj + (j - b[2])) * ni + (i - b[0]);
@@ -26,7 +27,7 @@ template <typename T> VG_DEV long long slab_index(const int *b, int i, int j, in
 // 6 = ordinary update (one of the NB boxes `in`), 0..5 = UPML slab (p is the
 // cell's offset in it), -1 = not updated by the sweep.
 template <int NB, typename A>
-VG_DEV int classify(const A &a, const int (*in)[6], int C, int i, int j, int k, long long &p)
+VG_HD int classify(const A &a, const int (*in)[6], int C, int i, int j, int k, long long &p)
 {
   for (int b = 0; b < NB; ++b)
     if (in_box(in[b], i, j, k)) return 6;
@@ -91,16 +92,28 @@ VG_DEV T debye_site(T e, T beta, long long g, int m, T *__restrict__ lastE, T *_
 //    reading its boxes and pointers from the argument block in device memory.
 // ===========================================================================
 
+// XCD-aware block order.  Workgroups are dealt to the eight XCDs round-robin
+// by launch index, and each XCD has its own L2, so consecutive launch indices
+// would scatter neighbouring rows and planes over different L2s.  The grid is
+// padded to a multiple of eight and launch index b runs logical block
+// (b % 8) * (n / 8) + b / 8: every XCD sweeps one contiguous range of blocks
+// (a z range of the domain), where neighbouring rows and planes share its L2.
+VG_DEV unsigned xcd_block()
+{
+  const unsigned per = gridDim.x / 8;
+  return (blockIdx.x % 8) * per + blockIdx.x / 8;
+}
+
 // Layer-0 Debye correction inside the fast E kernel (debye.c:243-307).  The
 // chunk's first site comes from dbase (uniform over the wavefront, so rows
 // without Debye sites skip this entirely); a site's number is that plus the
 // count of lower lanes whose class carries a Debye medium for C, which is
 // the site list's (component, cell) order.
 template <typename T, int C>
-VG_DEV void debye_fused(const FastE<T> &f, int j, int k, bool vx, unsigned cls, T be, T &e)
+VG_DEV void debye_fused(const FastE<T> &f, int bx, int j, int k, bool vx, unsigned cls, T be, T &e)
 {
   const EArgs<T> &a = *f.ap;
-  const long long row = ((long long)(C * f.NZ + k) * f.NY + j) * f.nchx + blockIdx.x;
+  const long long row = ((long long)(C * f.NZ + k) * f.NY + j) * f.nchx + bx;
   const int g0 = a.dbase[row];
   if (a.dbase[row + 1] == g0) return;
   const int ds = vx ? a.edslot[cls * 3 + C] : -1;
@@ -111,7 +124,7 @@ VG_DEV void debye_fused(const FastE<T> &f, int j, int k, bool vx, unsigned cls, 
 }
 
 template <typename T, bool IDX, int C>
-VG_DEV void e_fast(const FastE<T> &f, int j, int k, bool vx, long long id, unsigned cls, T curl)
+VG_DEV void e_fast(const FastE<T> &f, int bx, int j, int k, bool vx, long long id, unsigned cls, T curl)
 {
   T *__restrict__ E = f.E[C];
   T al, be;
@@ -123,19 +136,22 @@ VG_DEV void e_fast(const FastE<T> &f, int j, int k, bool vx, long long id, unsig
     be = f.beta[C][id];
   }
   T e = al * E[id] + be * curl;
-  if (IDX && f.dfuse) debye_fused<T, C>(f, j, k, vx, cls, be, e);
+  if (IDX && f.dfuse) debye_fused<T, C>(f, bx, j, k, vx, cls, be, e);
   if (vx) E[id] = e;
 }
 
 // Fast E kernel: inner grid (grid.c:709-797) and the first Debye layer.
 template <typename T, bool IDX> __global__ __launch_bounds__(256) void k_e_fast(FastE<T> f)
 {
-  const int i = f.box[0] + (int)blockIdx.x * 64 + (int)threadIdx.x;
-  const int j = f.box[2] + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const unsigned L = xcd_block();
+  if (L >= (unsigned)(f.nchx * f.gy * f.gz)) return;
+  const int bx = (int)(L % f.nchx), by = (int)((L / f.nchx) % f.gy), bz = (int)(L / (f.nchx * f.gy));
+  const int i = f.box[0] + bx * 64 + (int)threadIdx.x;
+  const int j = f.box[2] + by * 4 + (int)threadIdx.y;
   if (j > f.box[3]) return;  // whole wavefront
   const bool vx = i <= f.box[1];
   const int ic = vx ? i : f.box[1];
-  const int kb = f.box[4] + (int)blockIdx.z * f.kchunk;
+  const int kb = f.box[4] + bz * f.kchunk;
   const int ke = min(kb + f.kchunk, f.box[5] + 1);
   if (kb >= ke) return;
   const long long P = f.plane, SJ = f.PX;
@@ -152,9 +168,9 @@ template <typename T, bool IDX> __global__ __launch_bounds__(256) void k_e_fast(
     const T hzy = Hz[id - SJ], hzx = Hz[id - 1], hyx = Hy[id - 1], hxy = Hx[id - SJ];
     const unsigned cls = IDX ? (unsigned)f.ecls[id] : 0u;
     const T idhz = f.idh[2][k];
-    e_fast<T, IDX, 0>(f, j, k, vx, id, cls, idhy * (hz - hzy) + idhz * (hym - hy));
-    e_fast<T, IDX, 1>(f, j, k, vx, id, cls, idhz * (hx - hxm) + idhx * (hzx - hz));
-    e_fast<T, IDX, 2>(f, j, k, vx, id, cls, idhx * (hy - hyx) + idhy * (hxy - hx));
+    e_fast<T, IDX, 0>(f, bx, j, k, vx, id, cls, idhy * (hz - hzy) + idhz * (hym - hy));
+    e_fast<T, IDX, 1>(f, bx, j, k, vx, id, cls, idhz * (hx - hxm) + idhx * (hzx - hz));
+    e_fast<T, IDX, 2>(f, bx, j, k, vx, id, cls, idhx * (hy - hyx) + idhy * (hxy - hx));
     hxm = hx;
     hym = hy;
   }
@@ -163,12 +179,15 @@ template <typename T, bool IDX> __global__ __launch_bounds__(256) void k_e_fast(
 // Fast H kernel: inner grid (grid.c:800-888).
 template <typename T, bool IDX> __global__ __launch_bounds__(256) void k_h_fast(FastH<T> f)
 {
-  const int i = f.box[0] + (int)blockIdx.x * 64 + (int)threadIdx.x;
-  const int j = f.box[2] + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const unsigned L = xcd_block();
+  if (L >= (unsigned)(f.nchx * f.gy * f.gz)) return;
+  const int bx = (int)(L % f.nchx), by = (int)((L / f.nchx) % f.gy), bz = (int)(L / (f.nchx * f.gy));
+  const int i = f.box[0] + bx * 64 + (int)threadIdx.x;
+  const int j = f.box[2] + by * 4 + (int)threadIdx.y;
   if (j > f.box[3]) return;
   const bool vx = i <= f.box[1];
   const int ic = vx ? i : f.box[1];
-  const int kb = f.box[4] + (int)blockIdx.z * f.kchunk;
+  const int kb = f.box[4] + bz * f.kchunk;
   const int ke = min(kb + f.kchunk, f.box[5] + 1);
   if (kb >= ke) return;
   const long long P = f.plane, SJ = f.PX;
@@ -203,45 +222,63 @@ template <typename T, bool IDX> __global__ __launch_bounds__(256) void k_h_fast(
   }
 }
 
-// Cell (i,j,k) of shell thread t: boxes are x-fastest, laid end to end.
-VG_DEV bool shell_cell(const ShellBoxes &sb, long long t, int &i, int &j, int &k)
+// Shell sweeps.  Each block is one 256-cell block of one shell piece
+// (ShellPiece): the rows of a piece share one sequence of x runs per
+// component, so a lane finds its region among a few uniform runs, and all of
+// a cell's loads are issued before its first store.  The launch order of the
+// blocks (ShellWork) gives each XCD one contiguous eighth of every piece.
+VG_DEV bool shell_cell(const ShellPiece *pcs, const ShellWork *work, const ShellPiece *&pc, int &di, int &dj, int &dk)
 {
-  if (t >= sb.start[sb.nb]) return false;
-  int b = 0;
-  for (int q = 1; q < sb.nb; ++q)
-    if (t >= sb.start[q]) b = q;
-  const int *bx = sb.box[b];
-  const long long l = t - sb.start[b];
-  const long long ni = bx[1] - bx[0] + 1, nj = bx[3] - bx[2] + 1;
-  i = bx[0] + (int)(l % ni);
-  j = bx[2] + (int)((l / ni) % nj);
-  k = bx[4] + (int)(l / (ni * nj));
+  const ShellWork w = work[blockIdx.x];
+  if (w.piece < 0) return false;
+  pc = pcs + w.piece;
+  const int l = w.block * 256 + (int)threadIdx.x;
+  if (l >= pc->cells) return false;
+  const int r = l / pc->ni;
+  di = l - r * pc->ni;
+  dj = r % pc->nj;
+  dk = r / pc->nj;
   return true;
 }
 
-// One E component of a shell cell: ordinary update, or UPML split update with
-// the axis roles of component C (pml.c:859-1023): P along axis C+1, E along C+2.
-template <typename T, bool IDX, int C>
-VG_DEV void e_shell_component(const EArgs<T> &a, const int n3[3], long long id, unsigned cls)
+// Region of component C at x = i of a piece row: 6 ordinary, 0..5 UPML slab
+// with its slab index p and auxiliaries a/b; -1 not updated.
+template <int C>
+VG_HD int shell_region(const ShellPiece *pc, int i, int dj, int dk, long long &p, void *&xa, void *&xb)
 {
-  constexpr int A1 = (C + 1) % 3, A2 = (C + 2) % 3;
-  long long p = 0;
-  const int reg = classify<1>(a, &a.in[C], C, n3[0], n3[1], n3[2], p);
+  int reg = -1;
+  for (int q = 0; q < pc->nrun[C]; ++q) {
+    const ShellRun &rn = pc->run[C][q];
+    if (i >= rn.xa && i <= rn.xb) {
+      reg = rn.reg;
+      p = rn.p0 + (i - rn.xa) + dj * rn.psy + dk * rn.psz;
+      xa = rn.a;
+      xb = rn.b;
+    }
+  }
+  return reg;
+}
+
+// E shell (grid.c:709-795 ordinary update; pml.c:859-1023 UPML split update
+// with the axis roles of component C: P along axis C+1, E along C+2).
+template <typename T, bool IDX, int C>
+VG_DEV void e_shell_load(const EArgs<T> &a, const ShellPiece *pc, const int n3[3], int dj, int dk, long long id,
+                         unsigned cls, int &reg, long long &p, T *&pp, T *&pa, T &curl, T &e, T &al, T &be, T &opp,
+                         T &op, T (&cf)[6])
+{
+  void *xa = nullptr, *xb = nullptr;
+  reg = shell_region<C>(pc, n3[0], dj, dk, p, xa, xb);
   if (reg < 0) return;
-  // the curls read the i-1 / j-1 / k-1 neighbours: only cells the sweep
-  // updates evaluate them (never the low ghost layer)
+  const int i = n3[0], j = n3[1], k = n3[2];
   const long long P = a.plane, SJ = a.PX;
   const T *Hx = a.H[0], *Hy = a.H[1], *Hz = a.H[2];
-  const T *idh0 = a.idh[0], *idh1 = a.idh[1], *idh2 = a.idh[2];
-  const int i = n3[0], j = n3[1], k = n3[2];
   // (the sweep never updates index 0 of any axis; the guards keep every read
   // inside the allocation regardless)
   const bool xm = i > 0, ym = j > 0, zm = k > 0;
-  T curl;
-  if (C == 0) curl = idh1[j] * (Hz[id] - (ym ? Hz[id - SJ] : T(0))) + idh2[k] * ((zm ? Hy[id - P] : T(0)) - Hy[id]);
-  else if (C == 1) curl = idh2[k] * (Hx[id] - (zm ? Hx[id - P] : T(0))) + idh0[i] * ((xm ? Hz[id - 1] : T(0)) - Hz[id]);
-  else curl = idh0[i] * (Hy[id] - (xm ? Hy[id - 1] : T(0))) + idh1[j] * ((ym ? Hx[id - SJ] : T(0)) - Hx[id]);
-  T al, be;
+  if (C == 0) curl = a.idh[1][j] * (Hz[id] - (ym ? Hz[id - SJ] : T(0))) + a.idh[2][k] * ((zm ? Hy[id - P] : T(0)) - Hy[id]);
+  else if (C == 1) curl = a.idh[2][k] * (Hx[id] - (zm ? Hx[id - P] : T(0))) + a.idh[0][i] * ((xm ? Hz[id - 1] : T(0)) - Hz[id]);
+  else curl = a.idh[0][i] * (Hy[id] - (xm ? Hy[id - 1] : T(0))) + a.idh[1][j] * ((ym ? Hx[id - SJ] : T(0)) - Hx[id]);
+  e = a.E[C][id];
   if (IDX) {
     al = a.etab[cls * 6 + 2 * C];
     be = a.etab[cls * 6 + 2 * C + 1];
@@ -249,76 +286,122 @@ VG_DEV void e_shell_component(const EArgs<T> &a, const int n3[3], long long id, 
     al = a.alpha[C][id];
     be = a.beta[C][id];
   }
-  T *E = a.E[C];
-  if (reg == 6) {
-    E[id] = al * E[id] + be * curl;
-  } else {
-    T *PP = a.PP[reg][C], *PA = a.P[reg][C];
-    const T oldPP = PP[p];
-    const T pp = al * oldPP + be * curl;
-    PP[p] = pp;
-    const T oldP = PA[p];
-    const T pn = a.ad[A1][n3[A1]] * oldP + a.bd[A1][n3[A1]] * (pp - oldPP);
-    PA[p] = pn;
-    E[id] = a.ad[A2][n3[A2]] * E[id] + a.bd[A2][n3[A2]] * a.ibh[C][n3[C]] * (pn - a.ah[C][n3[C]] * oldP);
+  if (reg < 6) {
+    constexpr int A1 = (C + 1) % 3, A2 = (C + 2) % 3;
+    pp = static_cast<T *>(xa);
+    pa = static_cast<T *>(xb);
+    opp = pp[p];
+    op = pa[p];
+    cf[0] = a.ad[A1][n3[A1]];
+    cf[1] = a.bd[A1][n3[A1]];
+    cf[2] = a.ad[A2][n3[A2]];
+    cf[3] = a.bd[A2][n3[A2]];
+    cf[4] = a.ibh[C][n3[C]];
+    cf[5] = a.ah[C][n3[C]];
   }
 }
 
-template <typename T, bool IDX> __global__ __launch_bounds__(256) void k_e_shell(ShellBoxes sb, const EArgs<T> *ap)
+template <typename T, int C>
+VG_DEV void e_shell_store(const EArgs<T> &a, long long id, int reg, long long p, T *pp, T *pa, T curl, T e, T al,
+                          T be, T opp, T op, const T (&cf)[6])
 {
-  int i, j, k;
-  if (!shell_cell(sb, blockIdx.x * 256LL + threadIdx.x, i, j, k)) return;
-  const EArgs<T> &a = *ap;
-  const long long id = ((long long)k * a.NY + j) * a.PX + i;
-  const unsigned cls = IDX ? (unsigned)a.ecls[id] : 0u;
-  const int n3[3] = {i, j, k};
-  e_shell_component<T, IDX, 0>(a, n3, id, cls);
-  e_shell_component<T, IDX, 1>(a, n3, id, cls);
-  e_shell_component<T, IDX, 2>(a, n3, id, cls);
+  if (reg == 6) {
+    a.E[C][id] = al * e + be * curl;
+  } else if (reg >= 0) {
+    const T ppn = al * opp + be * curl;
+    pp[p] = ppn;
+    const T pn = cf[0] * op + cf[1] * (ppn - opp);
+    pa[p] = pn;
+    a.E[C][id] = cf[2] * e + cf[3] * cf[4] * (pn - cf[5] * op);
+  }
 }
 
-// One H component of a shell cell: ordinary update (inner grid or Mur normal
-// H), or UPML split update (pml.c:1026-1176): B along axis C+1, H along C+2.
-template <typename T, bool IDX, int C>
-VG_DEV void h_shell_component(const HArgs<T> &a, const int n3[3], long long id, unsigned cls)
+template <typename T, bool IDX>
+__global__ __launch_bounds__(256) void k_e_shell(const ShellPiece *pcs, const ShellWork *work, const EArgs<T> *ap)
 {
-  constexpr int A1 = (C + 1) % 3, A2 = (C + 2) % 3;
-  long long p = 0;
-  const int reg = classify<3>(a, a.in[C], C, n3[0], n3[1], n3[2], p);
+  const ShellPiece *pc;
+  int di, dj, dk;
+  if (!shell_cell(pcs, work, pc, di, dj, dk)) return;
+  const EArgs<T> &a = *ap;
+  const int n3[3] = {pc->box[0] + di, pc->box[2] + dj, pc->box[4] + dk};
+  const long long id = ((long long)n3[2] * a.NY + n3[1]) * a.PX + n3[0];
+  const unsigned cls = IDX ? (unsigned)a.ecls[id] : 0u;
+  int reg[3];
+  long long p[3];
+  T *pp[3], *pa[3];
+  T curl[3], e[3], al[3], be[3], opp[3], op[3], cf[3][6];
+  e_shell_load<T, IDX, 0>(a, pc, n3, dj, dk, id, cls, reg[0], p[0], pp[0], pa[0], curl[0], e[0], al[0], be[0], opp[0], op[0], cf[0]);
+  e_shell_load<T, IDX, 1>(a, pc, n3, dj, dk, id, cls, reg[1], p[1], pp[1], pa[1], curl[1], e[1], al[1], be[1], opp[1], op[1], cf[1]);
+  e_shell_load<T, IDX, 2>(a, pc, n3, dj, dk, id, cls, reg[2], p[2], pp[2], pa[2], curl[2], e[2], al[2], be[2], opp[2], op[2], cf[2]);
+  e_shell_store<T, 0>(a, id, reg[0], p[0], pp[0], pa[0], curl[0], e[0], al[0], be[0], opp[0], op[0], cf[0]);
+  e_shell_store<T, 1>(a, id, reg[1], p[1], pp[1], pa[1], curl[1], e[1], al[1], be[1], opp[1], op[1], cf[1]);
+  e_shell_store<T, 2>(a, id, reg[2], p[2], pp[2], pa[2], curl[2], e[2], al[2], be[2], opp[2], op[2], cf[2]);
+}
+
+// H shell: ordinary update (inner grid or Mur normal H, grid.c:797-888) or
+// UPML split update (pml.c:1026-1176): B along axis C+1, H along C+2.
+template <typename T, bool IDX, int C>
+VG_DEV void h_shell_load(const HArgs<T> &a, const ShellPiece *pc, const int n3[3], int dj, int dk, long long id,
+                         unsigned cls, int &reg, long long &p, T *&pb, T &curl, T &h, T &g, T &ob, T (&cf)[6])
+{
+  void *xa = nullptr, *xb = nullptr;
+  reg = shell_region<C>(pc, n3[0], dj, dk, p, xa, xb);
   if (reg < 0) return;
+  const int i = n3[0], j = n3[1], k = n3[2];
   const long long P = a.plane, SJ = a.PX;
   const T *Ex = a.E[0], *Ey = a.E[1], *Ez = a.E[2];
-  const T *ide0 = a.ide[0], *ide1 = a.ide[1], *ide2 = a.ide[2];
-  const int i = n3[0], j = n3[1], k = n3[2];
   const bool xin = i + 1 < a.NX, yin = j + 1 < a.NY, zin = k + 1 < a.NZ;
-  T curl;
-  if (C == 0) curl = ide2[k] * ((zin ? Ey[id + P] : T(0)) - Ey[id]) + ide1[j] * (Ez[id] - (yin ? Ez[id + SJ] : T(0)));
-  else if (C == 1) curl = ide0[i] * ((xin ? Ez[id + 1] : T(0)) - Ez[id]) + ide2[k] * (Ex[id] - (zin ? Ex[id + P] : T(0)));
-  else curl = ide1[j] * ((yin ? Ex[id + SJ] : T(0)) - Ex[id]) + ide0[i] * (Ey[id] - (xin ? Ey[id + 1] : T(0)));
-  const T g = IDX ? a.htab[cls * 3 + C] : a.gamma[C][id];
-  T *H = a.H[C];
-  if (reg == 6) {
-    H[id] = H[id] + g * curl;
-  } else {
-    T *Bp = a.B[reg][C];
-    const T oldB = Bp[p];
-    const T bn = a.ah[A1][n3[A1]] * oldB + g * a.bh[A1][n3[A1]] * curl;
-    Bp[p] = bn;
-    H[id] = a.ah[A2][n3[A2]] * H[id] + a.bh[A2][n3[A2]] * a.ibd[C][n3[C]] * (bn - a.ad[C][n3[C]] * oldB);
+  if (C == 0) curl = a.ide[2][k] * ((zin ? Ey[id + P] : T(0)) - Ey[id]) + a.ide[1][j] * (Ez[id] - (yin ? Ez[id + SJ] : T(0)));
+  else if (C == 1) curl = a.ide[0][i] * ((xin ? Ez[id + 1] : T(0)) - Ez[id]) + a.ide[2][k] * (Ex[id] - (zin ? Ex[id + P] : T(0)));
+  else curl = a.ide[1][j] * ((yin ? Ex[id + SJ] : T(0)) - Ex[id]) + a.ide[0][i] * (Ey[id] - (xin ? Ey[id + 1] : T(0)));
+  h = a.H[C][id];
+  g = IDX ? a.htab[cls * 3 + C] : a.gamma[C][id];
+  if (reg < 6) {
+    constexpr int A1 = (C + 1) % 3, A2 = (C + 2) % 3;
+    pb = static_cast<T *>(xa);
+    ob = pb[p];
+    cf[0] = a.ah[A1][n3[A1]];
+    cf[1] = a.bh[A1][n3[A1]];
+    cf[2] = a.ah[A2][n3[A2]];
+    cf[3] = a.bh[A2][n3[A2]];
+    cf[4] = a.ibd[C][n3[C]];
+    cf[5] = a.ad[C][n3[C]];
   }
 }
 
-template <typename T, bool IDX> __global__ __launch_bounds__(256) void k_h_shell(ShellBoxes sb, const HArgs<T> *ap)
+template <typename T, int C>
+VG_DEV void h_shell_store(const HArgs<T> &a, long long id, int reg, long long p, T *pb, T curl, T h, T g, T ob,
+                          const T (&cf)[6])
 {
-  int i, j, k;
-  if (!shell_cell(sb, blockIdx.x * 256LL + threadIdx.x, i, j, k)) return;
+  if (reg == 6) {
+    a.H[C][id] = h + g * curl;
+  } else if (reg >= 0) {
+    const T bn = cf[0] * ob + g * cf[1] * curl;
+    pb[p] = bn;
+    a.H[C][id] = cf[2] * h + cf[3] * cf[4] * (bn - cf[5] * ob);
+  }
+}
+
+template <typename T, bool IDX>
+__global__ __launch_bounds__(256) void k_h_shell(const ShellPiece *pcs, const ShellWork *work, const HArgs<T> *ap)
+{
+  const ShellPiece *pc;
+  int di, dj, dk;
+  if (!shell_cell(pcs, work, pc, di, dj, dk)) return;
   const HArgs<T> &a = *ap;
-  const long long id = ((long long)k * a.NY + j) * a.PX + i;
+  const int n3[3] = {pc->box[0] + di, pc->box[2] + dj, pc->box[4] + dk};
+  const long long id = ((long long)n3[2] * a.NY + n3[1]) * a.PX + n3[0];
   const unsigned cls = IDX ? (unsigned)a.hcls[id] : 0u;
-  const int n3[3] = {i, j, k};
-  h_shell_component<T, IDX, 0>(a, n3, id, cls);
-  h_shell_component<T, IDX, 1>(a, n3, id, cls);
-  h_shell_component<T, IDX, 2>(a, n3, id, cls);
+  int reg[3];
+  long long p[3];
+  T *pb[3];
+  T curl[3], h[3], g[3], ob[3], cf[3][6];
+  h_shell_load<T, IDX, 0>(a, pc, n3, dj, dk, id, cls, reg[0], p[0], pb[0], curl[0], h[0], g[0], ob[0], cf[0]);
+  h_shell_load<T, IDX, 1>(a, pc, n3, dj, dk, id, cls, reg[1], p[1], pb[1], curl[1], h[1], g[1], ob[1], cf[1]);
+  h_shell_load<T, IDX, 2>(a, pc, n3, dj, dk, id, cls, reg[2], p[2], pb[2], curl[2], h[2], g[2], ob[2], cf[2]);
+  h_shell_store<T, 0>(a, id, reg[0], p[0], pb[0], curl[0], h[0], g[0], ob[0], cf[0]);
+  h_shell_store<T, 1>(a, id, reg[1], p[1], pb[1], curl[1], h[1], g[1], ob[1], cf[1]);
+  h_shell_store<T, 2>(a, id, reg[2], p[2], pb[2], curl[2], h[2], g[2], ob[2], cf[2]);
 }
 
 // ---------------------------------------------------------------------------
@@ -581,40 +664,56 @@ template <typename T> __global__ void k_aux_h(AuxArgs<T> a)
 // Sources and TF/SF corrections, one thread per touched cell, entries in
 // reference order (source.c:602-753, planewave.c:576-1007).
 // ---------------------------------------------------------------------------
+// Row coefficients, gathered once: the row cell's beta (E) or gamma (H), and
+// its alpha (E).
+template <typename T>
+__global__ void k_corr_gather(long long nrows, const long long *key, const T *const *coef, const T *const *alpha,
+                              T *rowcoef, T *rowalpha)
+{
+  const long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  const int c = (int)(key[r] & 3);
+  const long long id = key[r] >> 2;
+  rowcoef[r] = coef[c][id];
+  rowalpha[r] = alpha ? alpha[c][id] : T(1);
+}
+
 template <typename T> __global__ void k_corr(CorrArgs<T> a, double pi)
 {
   const long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (r >= a.nrows) return;
   const double t = ((double)(*a.step + (unsigned long long)a.soff) + a.half) * a.dt;
-  const int c = a.rowcomp[r];
-  const long long id = a.rowcell[r];
-  T *F = a.F[c];
+  const long long key = a.rowkey[r];
+  const long long id = key >> 2;
+  T *F = a.F[key & 3];
   T f = F[id];
-  const T coef = a.coef[c][id];
-  const T alpha = a.electric ? a.alpha[c][id] : T(1);
+  const T coef = a.rowcoef[r];
+  const T alpha = a.rowalpha[r];
   for (int e = a.rowstart[r]; e < a.rowstart[r + 1]; ++e) {
-    const CorrEntry &en = a.ent[e];
-    if (en.kind <= 1) {
-      const SourceDev &s = a.src[en.idx];
+    const CorrEnt en = a.ent[e];
+    const int kind = en.code & 3, inc = (en.code >> 2) & 7, idx = en.code >> 8;
+    if (kind <= 1) {
+      const SourceDev &s = a.src[idx];
       const T source = T(s.size * wf_value(a.wf[s.waveform], t, s.delay, pi));
-      if (en.kind == 0) f = T(s.soft) * f + source;
+      if (kind == 0) f = T(s.soft) * f + source;
       else f = T(s.soft) * f - coef * source;
     } else {
-      const PwDev &pw = a.pw[en.idx];
-      T inc;
-      if (en.kind == 2) {
+      const PwDev &pw = a.pw[idx];
+      T incv;
+      if (kind == 2) {
         const T dp = T(en.geo);
-        const T *Eyi = a.aux + (long long)en.idx * 2 * a.auxmax, *Hzi = Eyi + a.auxmax;
+        const T *Eyi = a.aux + (long long)idx * 2 * a.auxmax, *Hzi = Eyi + a.auxmax;
         T value;
-        if (en.inc < 3) value = (T(1) - dp) * Eyi[2 + en.auxid] + dp * Eyi[2 + en.auxid + 1];
+        if (inc < 3) value = (T(1) - dp) * Eyi[2 + en.auxid] + dp * Eyi[2 + en.auxid + 1];
         else value = T(a.eta0) * ((T(1) - dp) * Hzi[1 + en.auxid] + dp * Hzi[2 + en.auxid]);
-        inc = T(pw.Finc[en.inc]) * value;
+        incv = T(pw.Finc[inc]) * value;
       } else {
-        inc = T(pw.Finc[en.inc]) * T(wf_value(a.wf[pw.waveform], t - en.geo, pw.delay, pi));
+        incv = T(pw.Finc[inc]) * T(wf_value(a.wf[pw.waveform], t - en.geo, pw.delay, pi));
       }
-      const T term = coef * (T(en.il) * inc);
-      if (a.electric) f = alpha * f + T(en.sign) * term;
-      else f = f + T(en.sign) * term;
+      const T term = coef * (T(en.il) * incv);
+      const T sign = (en.code >> 5) & 1 ? T(-1) : T(1);
+      if (a.electric) f = alpha * f + sign * term;
+      else f = f + sign * term;
     }
   }
   F[id] = f;

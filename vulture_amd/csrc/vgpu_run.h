@@ -7,6 +7,10 @@
 //   ghost H
 #pragma once
 #include "vgpu_kernels.h"
+#include <algorithm>
+#include <map>
+#include <string>
+#include <vector>
 
 namespace vgpu {
 
@@ -74,65 +78,257 @@ template <typename T> static void fill_volume_args(Ctx *c, EArgs<T> &ea, HArgs<T
     }
 }
 
-// Shell of a fast box inside [0, N-1]^3: up to six boxes (two x slabs of
-// full y/z extent, two y slabs over the box's x range, two z slabs over its
-// x-y range); the whole allocation when the box is empty.
-static ShellBoxes shell_of(const Ctx *c, const int *fb)
+// UPML auxiliaries of slab r, component C, carried by a shell run.
+template <typename T> static void run_aux(const EArgs<T> &a, int r, int C, void *&x, void *&y)
 {
-  ShellBoxes sb{};
-  const int N[3] = {c->NX, c->NY, c->NZ};
-  auto add = [&](int x0, int x1, int y0, int y1, int z0, int z1) {
-    if (x1 < x0 || y1 < y0 || z1 < z0) return;
-    int *b = sb.box[sb.nb];
-    b[0] = x0; b[1] = x1; b[2] = y0; b[3] = y1; b[4] = z0; b[5] = z1;
-    sb.start[sb.nb + 1] = sb.start[sb.nb] + (long long)(x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1);
-    ++sb.nb;
-  };
-  if (fb[0] > fb[1] || fb[2] > fb[3] || fb[4] > fb[5]) {
-    add(0, N[0] - 1, 0, N[1] - 1, 0, N[2] - 1);
-    return sb;
+  x = a.PP[r][C];
+  y = a.P[r][C];
+}
+template <typename T> static void run_aux(const HArgs<T> &a, int r, int C, void *&x, void *&y)
+{
+  x = a.B[r][C];
+  y = nullptr;
+}
+
+// Shell of a sweep: the cells outside its fast box, cut into pieces.  The
+// cut planes on y and z are the limits of the fast box, of every component's
+// ordinary-update boxes and of its UPML slabs, so over each product of a y
+// and a z interval a cell's regions (classify) depend on x alone.  Such a
+// product is one row segment in x (two, either side of the fast box, when
+// the rows pass through it), whose regions along x form each component's
+// runs; products with equal segments and runs are merged along y, then z.
+// The launch blocks are then ordered so that block b runs on XCD b % 8 and
+// each XCD gets one contiguous eighth of every piece's blocks.
+template <int NB, typename A>
+static int build_shell(const A &a, const int (*in)[NB][6], const int *fb, const int N[3], std::vector<ShellPiece> &pcs,
+                       std::vector<ShellWork> &work, long long &cells, std::string &err)
+{
+  const bool hasFast = fb[0] <= fb[1] && fb[2] <= fb[3] && fb[4] <= fb[5];
+  std::vector<int> cut[3];
+  for (int ax = 1; ax < 3; ++ax) {
+    auto add = [&](int lo, int hi) {
+      if (lo > hi) return;
+      cut[ax].push_back(std::max(0, std::min(lo, N[ax])));
+      cut[ax].push_back(std::max(0, std::min(hi + 1, N[ax])));
+    };
+    add(0, N[ax] - 1);
+    if (hasFast) add(fb[2 * ax], fb[2 * ax + 1]);
+    for (int C = 0; C < 3; ++C) {
+      for (int b = 0; b < NB; ++b) add(in[C][b][2 * ax], in[C][b][2 * ax + 1]);
+      for (int r = 0; r < 6; ++r)
+        if ((a.pmlmask >> r) & 1u) add(a.pl[r][C][2 * ax], a.pl[r][C][2 * ax + 1]);
+    }
+    std::sort(cut[ax].begin(), cut[ax].end());
+    cut[ax].erase(std::unique(cut[ax].begin(), cut[ax].end()), cut[ax].end());
   }
-  add(0, fb[0] - 1, 0, N[1] - 1, 0, N[2] - 1);
-  add(fb[1] + 1, N[0] - 1, 0, N[1] - 1, 0, N[2] - 1);
-  add(fb[0], fb[1], 0, fb[2] - 1, 0, N[2] - 1);
-  add(fb[0], fb[1], fb[3] + 1, N[1] - 1, 0, N[2] - 1);
-  add(fb[0], fb[1], fb[2], fb[3], 0, fb[4] - 1);
-  add(fb[0], fb[1], fb[2], fb[3], fb[5] + 1, N[2] - 1);
-  return sb;
+  const int ny = (int)cut[1].size() - 1, nz = (int)cut[2].size() - 1;
+  std::vector<ShellPiece> raw;
+  long long fastCells = 0, idle = 0;
+  std::vector<int> reg[3];
+  std::vector<long long> pidx[3];
+  for (int tz = 0; tz < nz; ++tz)
+    for (int ty = 0; ty < ny; ++ty) {
+      const int y0 = cut[1][ty], y1 = cut[1][ty + 1] - 1, z0 = cut[2][tz], z1 = cut[2][tz + 1] - 1;
+      const long long rows = (long long)(y1 - y0 + 1) * (z1 - z0 + 1);
+      // regions along the row, checked at both corners of the product
+      for (int C = 0; C < 3; ++C) {
+        reg[C].assign(N[0], -1);
+        pidx[C].assign(N[0], 0);
+        for (int i = 0; i < N[0]; ++i) {
+          long long p1 = 0;
+          reg[C][i] = classify<NB>(a, in[C], C, i, y0, z0, pidx[C][i]);
+          if (classify<NB>(a, in[C], C, i, y1, z1, p1) != reg[C][i]) {
+            err = "shell region not constant over a y-z product";
+            return -1;
+          }
+        }
+      }
+      const bool through = hasFast && y0 >= fb[2] && y1 <= fb[3] && z0 >= fb[4] && z1 <= fb[5];
+      int seg[2][2] = {{0, N[0] - 1}, {1, 0}};
+      int nseg = 1;
+      if (through) {
+        seg[0][1] = fb[0] - 1;
+        seg[1][0] = fb[1] + 1;
+        seg[1][1] = N[0] - 1;
+        nseg = 2;
+        fastCells += (long long)(fb[1] - fb[0] + 1) * rows;
+      }
+      for (int sg = 0; sg < nseg; ++sg) {
+        int x0 = seg[sg][0], x1 = seg[sg][1];
+        if (x0 > x1) continue;
+        const long long segCells = (long long)(x1 - x0 + 1) * rows;
+        // trim cells with no component updated from both ends
+        auto idleAt = [&](int i) { return reg[0][i] < 0 && reg[1][i] < 0 && reg[2][i] < 0; };
+        while (x0 <= x1 && idleAt(x0)) ++x0;
+        while (x1 >= x0 && idleAt(x1)) --x1;
+        long long used = 0;
+        // pieces of at most SHELL_RUNS runs per component
+        int xs = x0;
+        while (xs <= x1) {
+          ShellPiece pc{};
+          int xe = xs;
+          for (; xe <= x1; ++xe) {
+            bool fits = true;
+            for (int C = 0; C < 3 && fits; ++C) {
+              const bool opens = reg[C][xe] >= 0 && (xe == xs || reg[C][xe] != reg[C][xe - 1]);
+              if (opens && pc.nrun[C] == SHELL_RUNS) fits = false;
+            }
+            if (!fits) break;
+            for (int C = 0; C < 3; ++C) {
+              const int rg = reg[C][xe];
+              if (rg < 0) continue;
+              if (xe > xs && rg == reg[C][xe - 1]) {
+                pc.run[C][pc.nrun[C] - 1].xb = xe;
+                continue;
+              }
+              ShellRun &rn = pc.run[C][pc.nrun[C]++];
+              rn.xa = rn.xb = xe;
+              rn.reg = rg;
+              if (rg < 6) {
+                const int *sl = a.pl[rg][C];
+                rn.p0 = pidx[C][xe];
+                rn.psy = sl[1] - sl[0] + 1;
+                rn.psz = rn.psy * (sl[3] - sl[2] + 1);
+                run_aux(a, rg, C, rn.a, rn.b);
+              }
+            }
+          }
+          pc.box[0] = xs; pc.box[1] = xe - 1;
+          pc.box[2] = y0; pc.box[3] = y1;
+          pc.box[4] = z0; pc.box[5] = z1;
+          raw.push_back(pc);
+          used += (long long)(xe - xs) * rows;
+          xs = xe;
+        }
+        idle += segCells - used;
+      }
+    }
+  // merge along y, then z: a piece extends the last one with the same x
+  // range and runs and the same extent on the third axis when adjacent
+  auto key_of = [](const ShellPiece &pc, int ax) {
+    std::vector<long long> key = {pc.box[0], pc.box[1]};
+    const int q = ax == 1 ? 2 : 1;
+    key.push_back(pc.box[2 * q]);
+    key.push_back(pc.box[2 * q + 1]);
+    for (int C = 0; C < 3; ++C) {
+      key.push_back(pc.nrun[C]);
+      for (int r = 0; r < pc.nrun[C]; ++r) {
+        key.push_back(pc.run[C][r].xa);
+        key.push_back(pc.run[C][r].xb);
+        key.push_back(pc.run[C][r].reg);
+      }
+    }
+    return key;
+  };
+  auto merge = [&](std::vector<ShellPiece> &v, int ax) {
+    std::vector<ShellPiece> out;
+    std::map<std::vector<long long>, size_t> last;
+    for (const ShellPiece &pc : v) {
+      const std::vector<long long> key = key_of(pc, ax);
+      auto it = last.find(key);
+      if (it != last.end() && out[it->second].box[2 * ax + 1] + 1 == pc.box[2 * ax]) {
+        out[it->second].box[2 * ax + 1] = pc.box[2 * ax + 1];
+      } else {
+        last[key] = out.size();
+        out.push_back(pc);
+      }
+    }
+    v.swap(out);
+  };
+  merge(raw, 1);
+  merge(raw, 2);
+  cells = 0;
+  pcs.clear();
+  for (ShellPiece pc : raw) {
+    pc.ni = pc.box[1] - pc.box[0] + 1;
+    pc.nj = pc.box[3] - pc.box[2] + 1;
+    const long long n = (long long)pc.ni * pc.nj * (pc.box[5] - pc.box[4] + 1);
+    if (n > (1LL << 30)) {
+      err = "shell piece too large";
+      return -1;
+    }
+    pc.cells = (int)n;
+    cells += n;
+    pcs.push_back(pc);
+  }
+  if (fastCells + idle + cells != (long long)N[0] * N[1] * N[2]) {
+    err = "fast box and shell do not tile the grid";
+    return -1;
+  }
+  // launch order: XCD x runs blocks 8*slot + x, slot over its eighths
+  std::vector<ShellWork> per[8];
+  for (int q = 0; q < (int)pcs.size(); ++q) {
+    const int nb = (pcs[q].cells + 255) / 256;
+    for (int x = 0; x < 8; ++x)
+      for (int b = (int)((long long)nb * x / 8); b < (int)((long long)nb * (x + 1) / 8); ++b) per[x].push_back({q, b});
+  }
+  size_t len = 0;
+  for (int x = 0; x < 8; ++x) len = std::max(len, per[x].size());
+  work.assign(8 * len, ShellWork{-1, 0});
+  for (int x = 0; x < 8; ++x)
+    for (size_t t = 0; t < per[x].size(); ++t) work[8 * t + x] = per[x][t];
+  return 0;
 }
 
 // Once per context (at finalize): the sweeps' argument blocks go to device
 // memory, where the shell kernels and the fused Debye step read them.
 template <typename T> int prepare_sweeps(Ctx *c)
 {
-  // The fast box and its shell must tile the allocation exactly once, with
-  // every box inside it (the shell kernels index from these boxes).
   for (int e = 0; e < 2; ++e) {
     const int *fb = e == 0 ? c->fastE : c->fastH;
-    const ShellBoxes sb = shell_of(c, fb);
-    long long n = 0;
-    if (fb[0] <= fb[1] && fb[2] <= fb[3] && fb[4] <= fb[5]) {
-      if (fb[0] < 1 || fb[2] < 1 || fb[4] < 1 || fb[1] > c->NX - 2 || fb[3] > c->NY - 2 || fb[5] > c->NZ - 2)
-        return set_error(c, VGPU_EINVAL, "fast box not inside the grid");
-      n = (long long)(fb[1] - fb[0] + 1) * (fb[3] - fb[2] + 1) * (fb[5] - fb[4] + 1);
-    }
-    const int N[3] = {c->NX, c->NY, c->NZ};
-    for (int b = 0; b < sb.nb; ++b)
-      for (int a = 0; a < 3; ++a)
-        if (sb.box[b][2 * a] < 0 || sb.box[b][2 * a + 1] >= N[a] || sb.box[b][2 * a] > sb.box[b][2 * a + 1])
-          return set_error(c, VGPU_EINVAL, "shell box outside the grid");
-    if (n + sb.start[sb.nb] != (long long)c->NX * c->NY * c->NZ)
-      return set_error(c, VGPU_EINVAL, "fast box and shell do not tile the grid");
+    if (fb[0] <= fb[1] && fb[2] <= fb[3] && fb[4] <= fb[5] &&
+        (fb[0] < 1 || fb[2] < 1 || fb[4] < 1 || fb[1] > c->NX - 2 || fb[3] > c->NY - 2 || fb[5] > c->NZ - 2))
+      return set_error(c, VGPU_EINVAL, "fast box not inside the grid");
   }
   EArgs<T> ea{};
   HArgs<T> ha{};
   fill_volume_args<T>(c, ea, ha);
+  const int N[3] = {c->NX, c->NY, c->NZ};
+  std::vector<ShellPiece> ep, hp;
+  std::vector<ShellWork> ew, hw;
+  std::string err;
+  int ein[3][1][6];
+  for (int C = 0; C < 3; ++C)
+    for (int q = 0; q < 6; ++q) ein[C][0][q] = ea.in[C][q];
+  if (build_shell<1>(ea, ein, c->fastE, N, ep, ew, c->shellCells[0], err) ||
+      build_shell<3>(ha, ha.in, c->fastH, N, hp, hw, c->shellCells[1], err))
+    return set_error(c, VGPU_EINVAL, "%s", err.c_str());
+  c->nEpcs = (int)ep.size(); c->nHpcs = (int)hp.size();
+  c->nEwork = (int)ew.size(); c->nHwork = (int)hw.size();
   c->d_eargs = dev_alloc(c, sizeof(ea));
   c->d_hargs = dev_alloc(c, sizeof(ha));
-  if (!c->d_eargs || !c->d_hargs) return set_error(c, VGPU_ENOMEM, "sweep arguments");
-  if (hipMemcpy(c->d_eargs, &ea, sizeof(ea), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(c->d_hargs, &ha, sizeof(ha), hipMemcpyHostToDevice) != hipSuccess)
+  c->d_epcs = dev_alloc(c, ep.size() * sizeof(ShellPiece));
+  c->d_hpcs = dev_alloc(c, hp.size() * sizeof(ShellPiece));
+  c->d_ework = dev_alloc(c, ew.size() * sizeof(ShellWork));
+  c->d_hwork = dev_alloc(c, hw.size() * sizeof(ShellWork));
+  if (!c->d_eargs || !c->d_hargs || !c->d_epcs || !c->d_hpcs || !c->d_ework || !c->d_hwork)
+    return set_error(c, VGPU_ENOMEM, "sweep arguments");
+  auto up = [](void *d, const void *h, size_t n) { return n == 0 || hipMemcpy(d, h, n, hipMemcpyHostToDevice) == hipSuccess; };
+  if (!up(c->d_eargs, &ea, sizeof(ea)) || !up(c->d_hargs, &ha, sizeof(ha)) ||
+      !up(c->d_epcs, ep.data(), ep.size() * sizeof(ShellPiece)) ||
+      !up(c->d_hpcs, hp.data(), hp.size() * sizeof(ShellPiece)) ||
+      !up(c->d_ework, ew.data(), ew.size() * sizeof(ShellWork)) ||
+      !up(c->d_hwork, hw.data(), hw.size() * sizeof(ShellWork)))
     return set_error(c, VGPU_EDEVICE, "sweep arguments upload");
+  // correction rows: each row cell's coefficients, gathered once
+  for (int e = 0; e < 2; ++e) {
+    if (!c->corrRows[e]) continue;
+    const T *coef[3], *alpha[3];
+    for (int d = 0; d < 3; ++d) {
+      coef[d] = P<T>(e == 0 ? c->beta[d] : c->gamma[d]);
+      alpha[d] = P<T>(c->alpha[d]);
+    }
+    void *dc = dev_alloc(c, sizeof(coef)), *da = dev_alloc(c, sizeof(alpha));
+    if (!dc || !da) return set_error(c, VGPU_ENOMEM, "correction coefficients");
+    if (!up(dc, coef, sizeof(coef)) || !up(da, alpha, sizeof(alpha)))
+      return set_error(c, VGPU_EDEVICE, "correction coefficients upload");
+    const long long n = c->corrRows[e];
+    hipLaunchKernelGGL((k_corr_gather<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, n,
+                       P<long long>(c->d_ckey[e]), P<const T *const>(dc), e == 0 ? P<const T *const>(da) : nullptr,
+                       P<T>(c->d_ccoef[e]), P<T>(c->d_calpha[e]));
+    if (hipStreamSynchronize(c->stream) != hipSuccess)
+      return set_error(c, VGPU_EDEVICE, "correction coefficients: %s", hipGetErrorString(hipGetLastError()));
+  }
   return VGPU_OK;
 }
 
@@ -161,39 +357,45 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
   fe.plane = fh.plane = c->plane;
   for (int q = 0; q < 6; ++q) { fe.box[q] = c->fastE[q]; fh.box[q] = c->fastH[q]; }
   fe.nchx = egrid.x; fh.nchx = hgrid.x;
+  fe.gy = egrid.y; fe.gz = egrid.z; fh.gy = hgrid.y; fh.gz = hgrid.z;
+  // 1-D launches padded to a multiple of eight blocks (XCD order, xcd_block)
+  auto pad8 = [](unsigned long long n) { return dim3((unsigned)((n + 7) / 8 * 8)); };
+  const dim3 eg1 = pad8((unsigned long long)egrid.x * egrid.y * egrid.z);
+  const dim3 hg1 = pad8((unsigned long long)hgrid.x * hgrid.y * hgrid.z);
   for (int d = 0; d < 3; ++d) {
     fe.E[d] = ea.E[d]; fe.H[d] = ea.H[d]; fe.idh[d] = ea.idh[d]; fe.alpha[d] = ea.alpha[d]; fe.beta[d] = ea.beta[d];
     fh.H[d] = ha.H[d]; fh.E[d] = ha.E[d]; fh.ide[d] = ha.ide[d]; fh.gamma[d] = ha.gamma[d];
   }
   fe.ecls = ea.ecls; fe.etab = ea.etab; fe.dfuse = ea.dfuse; fe.ap = P<EArgs<T>>(c->d_eargs);
   fh.hcls = ha.hcls; fh.htab = ha.htab;
-  const ShellBoxes esh = shell_of(c, c->fastE), hsh = shell_of(c, c->fastH);
   const EArgs<T> *eap = P<EArgs<T>>(c->d_eargs);
   const HArgs<T> *hap = P<HArgs<T>>(c->d_hargs);
+  const ShellPiece *epc = P<ShellPiece>(c->d_epcs), *hpc = P<ShellPiece>(c->d_hpcs);
+  const ShellWork *ewk = P<ShellWork>(c->d_ework), *hwk = P<ShellWork>(c->d_hwork);
   auto fast_e = [&]() {
     if (egrid.x) {
-      if (c->indexed) hipLaunchKernelGGL((k_e_fast<T, true>), egrid, vblk, 0, s, fe);
-      else hipLaunchKernelGGL((k_e_fast<T, false>), egrid, vblk, 0, s, fe);
+      if (c->indexed) hipLaunchKernelGGL((k_e_fast<T, true>), eg1, vblk, 0, s, fe);
+      else hipLaunchKernelGGL((k_e_fast<T, false>), eg1, vblk, 0, s, fe);
     }
   };
   auto shell_e = [&]() {
-    if (esh.nb) {
-      const dim3 g((unsigned)((esh.start[esh.nb] + 255) / 256));
-      if (c->indexed) hipLaunchKernelGGL((k_e_shell<T, true>), g, dim3(256), 0, s, esh, eap);
-      else hipLaunchKernelGGL((k_e_shell<T, false>), g, dim3(256), 0, s, esh, eap);
+    if (c->nEwork) {
+      const dim3 g((unsigned)c->nEwork);
+      if (c->indexed) hipLaunchKernelGGL((k_e_shell<T, true>), g, dim3(256), 0, s, epc, ewk, eap);
+      else hipLaunchKernelGGL((k_e_shell<T, false>), g, dim3(256), 0, s, epc, ewk, eap);
     }
   };
   auto fast_h = [&]() {
     if (hgrid.x) {
-      if (c->indexed) hipLaunchKernelGGL((k_h_fast<T, true>), hgrid, vblk, 0, s, fh);
-      else hipLaunchKernelGGL((k_h_fast<T, false>), hgrid, vblk, 0, s, fh);
+      if (c->indexed) hipLaunchKernelGGL((k_h_fast<T, true>), hg1, vblk, 0, s, fh);
+      else hipLaunchKernelGGL((k_h_fast<T, false>), hg1, vblk, 0, s, fh);
     }
   };
   auto shell_h = [&]() {
-    if (hsh.nb) {
-      const dim3 g((unsigned)((hsh.start[hsh.nb] + 255) / 256));
-      if (c->indexed) hipLaunchKernelGGL((k_h_shell<T, true>), g, dim3(256), 0, s, hsh, hap);
-      else hipLaunchKernelGGL((k_h_shell<T, false>), g, dim3(256), 0, s, hsh, hap);
+    if (c->nHwork) {
+      const dim3 g((unsigned)c->nHwork);
+      if (c->indexed) hipLaunchKernelGGL((k_h_shell<T, true>), g, dim3(256), 0, s, hpc, hwk, hap);
+      else hipLaunchKernelGGL((k_h_shell<T, false>), g, dim3(256), 0, s, hpc, hwk, hap);
     }
   };
 
@@ -255,13 +457,9 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
   for (int e = 0; e < 2; ++e) {
     CorrArgs<T> &a = ce[e];
     a.nrows = c->corrRows[e];
-    a.rowcomp = P<int>(c->d_ccomp[e]); a.rowcell = P<long long>(c->d_ccell[e]);
-    a.rowstart = P<int>(c->d_cstart[e]); a.ent = P<CorrEntry>(c->d_cent[e]);
-    for (int d = 0; d < 3; ++d) {
-      a.F[d] = P<T>(c->F[3 * (1 - e) + d]);
-      a.alpha[d] = P<T>(c->alpha[d]);
-      a.coef[d] = e == 0 ? P<T>(c->beta[d]) : P<T>(c->gamma[d]);
-    }
+    a.rowkey = P<long long>(c->d_ckey[e]);
+    a.rowstart = P<int>(c->d_cstart[e]); a.ent = P<CorrEnt>(c->d_cent[e]);
+    a.rowcoef = P<T>(c->d_ccoef[e]); a.rowalpha = P<T>(c->d_calpha[e]);
     a.electric = e == 0;
     a.wf = P<WaveformDev>(c->d_wf); a.src = P<SourceDev>(c->d_src); a.pw = P<PwDev>(c->d_pw);
     a.aux = P<T>(c->d_aux); a.auxmax = c->auxmax; a.eta0 = c->grid.eta0;
