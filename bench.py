@@ -115,6 +115,29 @@ def measure(mesh, prec, steps, warmup):
     }
 
 
+def sibc_box(work, steps, warmup=5):
+    """The SIBC + Mur box (384x384x256 cubic 1.5 mm, Mur faces, six fitted
+    conductor walls, 64 dielectric blocks, current-moment dipole) in FP32:
+    throughput and the split between the volume sweeps and the rest."""
+    from vulture_amd import DeviceSolve, meshgen
+    mesh = meshgen.write("sibc_box", os.path.join(work, "sibc"), steps=warmup + steps)
+    t0 = time.time()
+    with DeviceSolve(mesh, precision="fp32", steps=warmup + steps, kernel_events=steps) as ds:
+        setup_s = time.time() - t0
+        st = ds.stats()
+        ds.run(0, warmup)
+        ds.synchronize()
+        ds.run(warmup, steps)
+        total_ms = ds.synchronize()
+        e_ms, h_ms = ds.kernel_ms()
+    return {"workload": "SIBC + Mur box 384x384x256 cubic 1.5 mm, six fitted conductor walls, 64 dielectric "
+                        "blocks, current-moment dipole (seed 4)", "dtype": "fp32", "steps": steps,
+            "cells": st["cells"], "sibc_faces": st["sibc_faces"],
+            "value": round(st["cells"] * steps / (total_ms * 1e-3) / 1e6, 1),
+            "ms_per_step": round(total_ms / steps, 4), "fast_e_ms": round(e_ms, 5), "fast_h_ms": round(h_ms, 5),
+            "setup_s": round(setup_s, 2)}
+
+
 def cpu_baseline(mesh, steps):
     import oracle
     threads = min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
@@ -140,6 +163,8 @@ def main():
     ap.add_argument("--serve-active", type=int, default=8, help="solves in flight at once when serving")
     ap.add_argument("--serve-nmax", type=int, default=128, help="largest cells per axis of a serving request")
     ap.add_argument("--serve-steps", type=int, nargs=2, default=[200, 800], help="step range of serving requests")
+    ap.add_argument("--sibc-steps", type=int, default=30,
+                    help="timed steps of the SIBC + Mur box at 384x384x256 in FP32 (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -174,6 +199,10 @@ def main():
                        "roofline": f64["roofline"], "step_roofline": f64["step_roofline"]}
         out["fp32_over_fp64"] = round(res["fp32"]["value"] / f64["value"], 3)
     out["cpu_baseline"] = cpu_baseline(mesh, args.cpu_steps) if args.cpu_steps > 0 else None
+    if args.sibc_steps > 0:
+        out["sibc_box"] = sibc_box(work, args.sibc_steps)
+        print(f"# sibc_box: {out['sibc_box']['value']:.1f} Mcell-updates/s, {out['sibc_box']['ms_per_step']:.3f} "
+              f"ms/step", file=sys.stderr, flush=True)
     if args.serve > 0:
         from vulture_amd import serve
         rep = serve.sweep(os.path.join(work, "serve"), count=args.serve, load=0.7, precision="fp32",

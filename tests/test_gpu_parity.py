@@ -79,14 +79,27 @@ WORKLOADS = [
     ("debye", dict(n=32, spheres=4, rmin=3, rmax=6, steps=200)),
     ("debye", dict(n=64, spheres=12, rmin=3, rmax=10, steps=150, seed=11)),
     ("enclosure", dict(nx=48, ny=40, nz=32, pml=6, steps=300, nfreq=64)),
+    ("sibc_box", dict(nx=40, ny=36, nz=32, blocks=6, steps=300, observers=2, nfreq=16)),
 ]
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 @pytest.mark.parametrize("name,kw", WORKLOADS, ids=[f"{w}-{i}" for i, (w, _) in enumerate(WORKLOADS)])
 def test_workload(name, kw, precision, tmp_path):
+    """Generated workloads.  In FP32 the time-series bar is 1e-4 or 1.5 times
+    the error of the reference's own float arithmetic (the float oracle,
+    byte-identical to the reference build) on the same mesh, whichever is
+    larger: in the closed SIBC box the float reference itself sits at about
+    1.8e-4 against FP64."""
     mesh = meshgen.write(name, str(tmp_path), **kw)
-    _check(_run_pair(mesh, tmp_path, precision), precision)
+    r = _run_pair(mesh, tmp_path, precision)
+    if precision == "fp32":
+        oracle.run_mesh(mesh, str(tmp_path / "oracle32"), precision="fp32")
+        ref = compare_outputs(str(tmp_path / "oracle32"), str(tmp_path / "oracle"))
+        assert r["files"] > 0 and r["fd_db"] <= 0.05, r
+        assert r["td"] <= max(1e-4, 1.5 * ref["td"]), (r, ref)
+    else:
+        _check(r, precision)
 
 
 def _pw_scaled(text, size):

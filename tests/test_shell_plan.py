@@ -70,7 +70,9 @@ def test_generated_workloads(checker, tmp_path):
                             boundary="MUR"),
               meshgen.write("debye", str(tmp_path / "d48p"), n=48, spheres=3, rmin=3, rmax=6, steps=10,
                             boundary="PEC"),
-              meshgen.write("enclosure", str(tmp_path / "enc"), nx=48, ny=40, nz=32, pml=6, steps=10, nfreq=4)]
+              meshgen.write("enclosure", str(tmp_path / "enc"), nx=48, ny=40, nz=32, pml=6, steps=10, nfreq=4),
+              meshgen.write("sibc_box", str(tmp_path / "sb"), nx=40, ny=36, nz=32, blocks=6, steps=10,
+                            observers=2, nfreq=4)]
     rng = np.random.default_rng(3)
     for q in range(6):
         text, _, _ = meshgen.serving_request(rng, nmin=12, nmax=40, steps_min=5, steps_max=10, nfreq=2)

@@ -635,7 +635,7 @@ static int build_sibc(Ctx *c)
   std::vector<unsigned char> adjA(4 * NF), adjB(4 * NF);
   std::vector<double> A(16 * ns), B(16 * ns), asy(16 * ns), q((size_t)16 * ns * maxP * 6, 0.0);
   // Gathered H corrections: (cell key, sequence) -> entry.
-  struct HE { long long key; long long seq; SibcHEntry e; };
+  struct HE { long long key; long long seq; SibcHEnt e; };
   std::vector<HE> hents;
   std::vector<long long> zeroKeys;
   long long f = 0, seq = 0;
@@ -672,18 +672,22 @@ static int build_sibc(Ctx *c)
           const unsigned char *a = &c->sibc[s].adjA[4 * lf], *b = &c->sibc[s].adjB[4 * lf];
           for (int r = 0; r < 4; ++r) { adjA[4 * f + r] = a[r]; adjB[4 * f + r] = b[r]; }
           // terms in sibc.c:638-757 order: (comp, di, dj, dk, sign, weight dir/index, Etan slot, id index)
+          auto wbit = [](double w) { return w == 1.0 ? 1 : 0; };  // weights are 0.5 or 1
           auto edge = [&](int comp, int ci, int cj, int ck, double sign, double w, int slot, int idaxis, int idi) {
-            SibcHEntry e{};
-            e.face = (int)f; e.kind = 0; e.ea = slot; e.eb = slot; e.sign = sign; e.w = w;
-            e.ida = (*ide[idaxis])[idi]; e.idb = 0.0; e.na = e.nb = 0.0;
+            SibcHEnt e{};
+            e.face = (int)f;
+            e.bits = 0 | slot << 1 | slot << 3 | (sign < 0) << 5 | wbit(w) << 6 | idaxis << 9;
+            e.ia = e.ib = idi;
             hents.push_back({c->lin(ci, cj, ck) * 6 + comp, seq++, e});
           };
           auto adj = [&](int comp, int ci, int cj, int ck, double sign, double w, int sa, int sb, int idaxis,
                          int ia, int ib, int r) {
-            SibcHEntry e{};
-            e.face = (int)f; e.kind = 1; e.ea = sa; e.eb = sb; e.sign = sign; e.w = w;
-            e.ida = (*ide[idaxis])[ia]; e.idb = (*ide[idaxis])[ib];
-            e.na = a[r] ? 0.0 : 1.0; e.nb = b[r] ? 0.0 : 1.0;
+            SibcHEnt e{};
+            e.face = (int)f;
+            e.bits = 1 | sa << 1 | sb << 3 | (sign < 0) << 5 | wbit(w) << 6 | (a[r] ? 0 : 1) << 7 |
+                     (b[r] ? 0 : 1) << 8 | idaxis << 9;
+            e.ia = ia;
+            e.ib = ib;
             hents.push_back({c->lin(ci, cj, ck) * 6 + comp, seq++, e});
           };
           if (d.normal == 0) {
@@ -741,18 +745,16 @@ static int build_sibc(Ctx *c)
   std::stable_sort(hents.begin(), hents.end(), [](const HE &a, const HE &b) { return a.key < b.key; });
   std::sort(zeroKeys.begin(), zeroKeys.end());
   zeroKeys.erase(std::unique(zeroKeys.begin(), zeroKeys.end()), zeroKeys.end());
-  std::vector<int> rcomp, rstart, rzero;
-  std::vector<long long> rcell;
-  std::vector<SibcHEntry> ents;
+  std::vector<int> rstart;
+  std::vector<long long> rkey;
+  std::vector<SibcHEnt> ents;
   size_t e = 0, zq = 0;
   while (e < hents.size() || zq < zeroKeys.size()) {
     long long key;
     if (zq < zeroKeys.size() && (e >= hents.size() || zeroKeys[zq] <= hents[e].key)) key = zeroKeys[zq];
     else key = hents[e].key;
     const bool isZero = zq < zeroKeys.size() && zeroKeys[zq] == key;
-    rcomp.push_back((int)(key % 6) - 3);
-    rcell.push_back(key / 6);
-    rzero.push_back(isZero ? 1 : 0);
+    rkey.push_back(key / 6 << 3 | (isZero ? 4 : 0) | (key % 6 - 3));
     rstart.push_back((int)ents.size());
     while (e < hents.size() && hents[e].key == key) {
       if (!isZero) ents.push_back(hents[e].e);
@@ -761,19 +763,27 @@ static int build_sibc(Ctx *c)
     if (isZero) ++zq;
   }
   rstart.push_back((int)ents.size());
-  c->sibcHRows = (long long)rcomp.size();
+  c->sibcHRows = (long long)rkey.size();
   c->d_sijk = upload_vec(c, ijk); c->d_ssurf = upload_vec(c, surf);
   c->d_sadjA = upload_vec(c, adjA); c->d_sadjB = upload_vec(c, adjB);
   c->d_snormal = upload_vec(c, normal); c->d_sA = upload_vec(c, A); c->d_sB = upload_vec(c, B);
   c->d_sset = upload_vec(c, set); c->d_snp = upload_vec(c, npl); c->d_sasymp = upload_vec(c, asy);
   c->d_sq = upload_vec(c, q);
-  c->d_szeta = dev_alloc(c, (size_t)16 * maxP * NF * 2 * sizeof(double));
+  // With every coefficient real (real poles and residues, as in fitted
+  // conductor slabs) each state's imaginary part stays exactly zero, so the
+  // state is kept real: the same values, half the state traffic.
+  c->sibcReal = true;
+  for (size_t e = 1; e < q.size(); e += 2)
+    if (q[e] != 0.0) c->sibcReal = false;
+  c->d_szeta = dev_alloc(c, (size_t)16 * maxP * NF * (c->sibcReal ? 1 : 2) * sizeof(double));
   c->d_sold = dev_alloc(c, (size_t)16 * NF * sizeof(double));
   c->d_setan = dev_alloc(c, (size_t)4 * NF * c->esz);
-  c->d_shcomp = upload_vec(c, rcomp); c->d_shcell = upload_vec(c, rcell);
-  c->d_shstart = upload_vec(c, rstart); c->d_shzero = upload_vec(c, rzero);
+  c->d_shkey = upload_vec(c, rkey);
+  c->d_shstart = upload_vec(c, rstart);
   c->d_shent = upload_vec(c, ents);
-  if (!c->d_szeta || !c->d_sold || !c->d_setan) return set_error(c, VGPU_ENOMEM, "SIBC state allocation");
+  c->d_shg = dev_alloc(c, rkey.size() * c->esz);
+  if (!c->d_szeta || !c->d_sold || !c->d_setan || !c->d_shkey || !c->d_shstart || !c->d_shent || !c->d_shg)
+    return set_error(c, VGPU_ENOMEM, "SIBC state allocation");
   return VGPU_OK;
 }
 

@@ -162,7 +162,8 @@ template <typename T> struct SibcEArgs {
   const double *asymp;   // [set][16]
   const double *q;       // [set][16][maxPoles][3][2]
   int maxPoles;
-  double *zeta;          // [(filter*maxPoles + pole)][face][2]
+  int real;              // all coefficients real: zeta holds real states only
+  double *zeta;          // [(filter*maxPoles + pole)][face][2], or [..][face] when real
   double *old;           // [filter][face]
   T *Etan;               // [face][4]
   T *E[3];
@@ -173,26 +174,26 @@ template <typename T> struct SibcEArgs {
 
 // Gathered SIBC H corrections (sibc.c:616-776): one row per touched H cell,
 // entries in the reference's application order.
-struct SibcHEntry {
+// Device form (16 bytes).  bits: kind (bit 0: 0 edge term, 1 parallel-
+// adjacent term), Etan slots ea (bits 1-2) and eb (3-4), negative sign (5),
+// full edge weight (6: weight 1, else 1/2 = 0.5*(1 + PMC edge)), !isAdjA
+// (7), !isAdjB (8), axis of the inverse lengths (9-10); ia/ib index the
+// inverse primary edge lengths of that axis.
+struct SibcHEnt {
   int face;
-  int kind;              // 0 edge term, 1 parallel-adjacent term
-  int ea, eb;            // Etan slots
-  double sign;           // +1 / -1
-  double w;              // edge weight 0.5*(1 + PMC edge)
-  double ida, idb;       // inverse lengths
-  double na, nb;         // !isAdjA, !isAdjB (adjacent term)
+  int bits;
+  int ia, ib;
 };
 
 template <typename T> struct SibcHArgs {
   long long nrows;
-  const int *rowcomp;
-  const long long *rowcell;
+  const long long *rowkey;   // cell << 3 | zero << 2 | component; zero: normal H of a face, set to 0
   const int *rowstart;   // [nrows+1]
-  const int *rowzero;    // 1: normal H of a SIBC face, set to zero
-  const SibcHEntry *ent;
+  const T *rowg;         // the row cell's gamma
+  const SibcHEnt *ent;
   const T *Etan;
   T *H[3];
-  const T *gamma[3];
+  const T *ide[3];
 };
 
 // Sources and TF/SF corrections applied after the volume update, one row
@@ -408,11 +409,12 @@ struct Ctx {
   std::vector<SibcSurf> sibc;
   long long sibcFaces = 0;
   int sibcMaxPoles = 1;
+  bool sibcReal = false;                     // all filter coefficients real: real filter state
   void *d_sijk = nullptr, *d_ssurf = nullptr, *d_sadjA = nullptr, *d_sadjB = nullptr, *d_snormal = nullptr;
   void *d_sA = nullptr, *d_sB = nullptr, *d_sset = nullptr, *d_snp = nullptr, *d_sasymp = nullptr, *d_sq = nullptr;
   void *d_szeta = nullptr, *d_sold = nullptr, *d_setan = nullptr;
   long long sibcHRows = 0;
-  void *d_shcomp = nullptr, *d_shcell = nullptr, *d_shstart = nullptr, *d_shzero = nullptr, *d_shent = nullptr;
+  void *d_shkey = nullptr, *d_shstart = nullptr, *d_shent = nullptr, *d_shg = nullptr;
 
   // waveforms, sources, plane waves
   std::vector<WaveformDev> wf;

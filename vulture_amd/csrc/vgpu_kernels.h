@@ -514,6 +514,16 @@ template <typename T> __global__ void k_sibc_e(SibcEArgs<T> a)
         double *old = &a.old[flt * NF + f];
         const double xo = *old;
         const double *qq = a.q + ((long long)(16 * set + flt) * a.maxPoles) * 6;
+        if (a.real) {
+          // real coefficients: the complex step below with zero imaginary
+          // parts, which leave the same real part bit for bit
+          for (int kp = 0; kp < np; ++kp) {
+            double *z = &a.zeta[((long long)flt * a.maxPoles + kp) * NF + f];
+            const double nr = (qq[6 * kp] * *z + qq[6 * kp + 2] * xo) + qq[6 * kp + 4] * x;
+            *z = nr;
+            y = y + nr;
+          }
+        } else
         for (int kp = 0; kp < np; ++kp) {
           double *z = &a.zeta[(((long long)flt * a.maxPoles + kp) * NF + f) * 2];
           const double q0r = qq[6 * kp], q0i = qq[6 * kp + 1], q1r = qq[6 * kp + 2], q1i = qq[6 * kp + 3];
@@ -543,21 +553,25 @@ template <typename T> __global__ void k_sibc_h(SibcHArgs<T> a)
 {
   const long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (r >= a.nrows) return;
-  T *H = a.H[a.rowcomp[r]];
-  const long long id = a.rowcell[r];
-  if (a.rowzero[r]) {
+  const long long key = a.rowkey[r];
+  T *H = a.H[key & 3];
+  const long long id = key >> 3;
+  if (key & 4) {
     H[id] = T(0);
     return;
   }
   T h = H[id];
-  const T g = a.gamma[a.rowcomp[r]][id];
+  const T g = a.rowg[r];
   for (int e = a.rowstart[r]; e < a.rowstart[r + 1]; ++e) {
-    const SibcHEntry &en = a.ent[e];
+    const SibcHEnt en = a.ent[e];
+    const int b = en.bits, ax = (b >> 9) & 3, ea = (b >> 1) & 3, eb = (b >> 3) & 3;
+    const T *ide = ax == 0 ? a.ide[0] : ax == 1 ? a.ide[1] : a.ide[2];
+    const T w = (b >> 6) & 1 ? T(1) : T(0.5);
     const T *Et = a.Etan + 4 * (long long)en.face;
     T c;
-    if (en.kind == 0) c = (g * T(en.w)) * (T(en.ida) * Et[en.ea]);
-    else c = ((g * T(0.5)) * T(en.w)) * (T(en.na) * (T(en.ida) * Et[en.ea]) + T(en.nb) * (T(en.idb) * Et[en.eb]));
-    h = h + T(en.sign) * c;
+    if (!(b & 1)) c = (g * w) * (ide[en.ia] * Et[ea]);
+    else c = ((g * T(0.5)) * w) * (T((b >> 7) & 1) * (ide[en.ia] * Et[ea]) + T((b >> 8) & 1) * (ide[en.ib] * Et[eb]));
+    h = h + ((b >> 5) & 1 ? T(-1) : T(1)) * c;
   }
   H[id] = h;
 }
@@ -666,16 +680,17 @@ template <typename T> __global__ void k_aux_h(AuxArgs<T> a)
 // ---------------------------------------------------------------------------
 // Row coefficients, gathered once: the row cell's beta (E) or gamma (H), and
 // its alpha (E).
+// Row keys hold the cell above bit `shift` and the component in bits 0-1.
 template <typename T>
-__global__ void k_corr_gather(long long nrows, const long long *key, const T *const *coef, const T *const *alpha,
-                              T *rowcoef, T *rowalpha)
+__global__ void k_corr_gather(long long nrows, const long long *key, int shift, const T *const *coef,
+                              const T *const *alpha, T *rowcoef, T *rowalpha)
 {
   const long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (r >= nrows) return;
   const int c = (int)(key[r] & 3);
-  const long long id = key[r] >> 2;
+  const long long id = key[r] >> shift;
   rowcoef[r] = coef[c][id];
-  rowalpha[r] = alpha ? alpha[c][id] : T(1);
+  if (rowalpha) rowalpha[r] = alpha ? alpha[c][id] : T(1);
 }
 
 template <typename T> __global__ void k_corr(CorrArgs<T> a, double pi)

@@ -310,7 +310,18 @@ template <typename T> int prepare_sweeps(Ctx *c)
       !up(c->d_ework, ew.data(), ew.size() * sizeof(ShellWork)) ||
       !up(c->d_hwork, hw.data(), hw.size() * sizeof(ShellWork)))
     return set_error(c, VGPU_EDEVICE, "sweep arguments upload");
-  // correction rows: each row cell's coefficients, gathered once
+  // correction rows (and SIBC H rows): each row cell's coefficients, gathered once
+  if (c->sibcHRows) {
+    const T *g[3] = {P<T>(c->gamma[0]), P<T>(c->gamma[1]), P<T>(c->gamma[2])};
+    void *dg = dev_alloc(c, sizeof(g));
+    if (!dg) return set_error(c, VGPU_ENOMEM, "SIBC coefficients");
+    if (!up(dg, g, sizeof(g))) return set_error(c, VGPU_EDEVICE, "SIBC coefficients upload");
+    const long long n = c->sibcHRows;
+    hipLaunchKernelGGL((k_corr_gather<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, n,
+                       P<long long>(c->d_shkey), 3, P<const T *const>(dg), nullptr, P<T>(c->d_shg), nullptr);
+    if (hipStreamSynchronize(c->stream) != hipSuccess)
+      return set_error(c, VGPU_EDEVICE, "SIBC coefficients: %s", hipGetErrorString(hipGetLastError()));
+  }
   for (int e = 0; e < 2; ++e) {
     if (!c->corrRows[e]) continue;
     const T *coef[3], *alpha[3];
@@ -324,7 +335,7 @@ template <typename T> int prepare_sweeps(Ctx *c)
       return set_error(c, VGPU_EDEVICE, "correction coefficients upload");
     const long long n = c->corrRows[e];
     hipLaunchKernelGGL((k_corr_gather<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, n,
-                       P<long long>(c->d_ckey[e]), P<const T *const>(dc), e == 0 ? P<const T *const>(da) : nullptr,
+                       P<long long>(c->d_ckey[e]), 2, P<const T *const>(dc), e == 0 ? P<const T *const>(da) : nullptr,
                        P<T>(c->d_ccoef[e]), P<T>(c->d_calpha[e]));
     if (hipStreamSynchronize(c->stream) != hipSuccess)
       return set_error(c, VGPU_EDEVICE, "correction coefficients: %s", hipGetErrorString(hipGetLastError()));
@@ -444,13 +455,14 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
     se.normal = P<int>(c->d_snormal); se.A = P<double>(c->d_sA); se.B = P<double>(c->d_sB);
     se.filtset = P<int>(c->d_sset); se.npoles = P<int>(c->d_snp); se.asymp = P<double>(c->d_sasymp);
     se.q = P<double>(c->d_sq); se.maxPoles = c->sibcMaxPoles; se.zeta = P<double>(c->d_szeta);
+    se.real = c->sibcReal ? 1 : 0;
     se.old = P<double>(c->d_sold); se.Etan = P<T>(c->d_setan);
     for (int d = 0; d < 3; ++d) { se.E[d] = P<T>(c->F[d]); se.H[d] = P<T>(c->F[3 + d]); }
     se.NY = c->NY; se.PX = c->PX; se.plane = c->plane;
-    sh.nrows = c->sibcHRows; sh.rowcomp = P<int>(c->d_shcomp); sh.rowcell = P<long long>(c->d_shcell);
-    sh.rowstart = P<int>(c->d_shstart); sh.rowzero = P<int>(c->d_shzero);
-    sh.ent = P<SibcHEntry>(c->d_shent); sh.Etan = P<T>(c->d_setan);
-    for (int d = 0; d < 3; ++d) { sh.H[d] = P<T>(c->F[3 + d]); sh.gamma[d] = P<T>(c->gamma[d]); }
+    sh.nrows = c->sibcHRows; sh.rowkey = P<long long>(c->d_shkey);
+    sh.rowstart = P<int>(c->d_shstart); sh.rowg = P<T>(c->d_shg);
+    sh.ent = P<SibcHEnt>(c->d_shent); sh.Etan = P<T>(c->d_setan);
+    for (int d = 0; d < 3; ++d) { sh.H[d] = P<T>(c->F[3 + d]); sh.ide[d] = P<T>(c->ide[d]); }
   }
 
   CorrArgs<T> ce[2];

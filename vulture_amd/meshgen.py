@@ -17,6 +17,9 @@ Workloads
                at seeded angles (auxiliary-grid path), 8 observers with a
                64-frequency DFT, 3000 steps.  ``n`` and ``spheres`` shrink it
                for parity runs.
+``sibc_box``   384x384x256 cubic 1.5 mm box with Mur outer faces and six SIBC
+               conductor walls (fitted slab .pfe files), 64 dielectric
+               blocks, a soft current-moment dipole, 6000 steps.
 """
 import math
 import os
@@ -160,15 +163,70 @@ def debye(steps=3000, n=256, spheres=200, pml=8, seed=3, rmin=4, rmax=20, observ
     return "\n".join(lines) + "\n"
 
 
-WORKLOADS = {"freespace": freespace, "enclosure": enclosure, "debye": debye}
+def sibc_box(steps=6000, nx=384, ny=384, nz=256, dx=1.5e-3, blocks=64, seed=4, observers=4, nfreq=64):
+    """Shielded box: cubic `dx` mesh with first-order Mur on all six faces and
+    six SIBC walls at index 2 and n-2 on every axis.  Each wall is its own
+    conductor slab, conductivity logU[1e5, 6e7] S/m and thickness
+    U[0.1, 2] mm, written as a passive pole-residue fit of the slab's
+    two-port impedance (vulture_amd/pfe.py, two poles per decade, about 14
+    real poles per entry) to wallK.pfe.  Inside: `blocks`
+    SIMPLE dielectric boxes (eps_r ~ U[1, 12]), a soft current-moment dipole
+    (IDZ) at a seeded position driven by a Gaussian pulse reaching -20 dB at
+    5 GHz, and seeded point observers with TD and FD output.  Returns
+    (mesh text, {file name: text}) for the .pfe files next to the mesh."""
+    from . import pfe
+    rng = np.random.default_rng(seed)
+    n = (nx, ny, nz)
+    lines = ["VM 1.0.0", f"CE SIBC + Mur box {nx}x{ny}x{nz} cubic {dx * 1e3:g} mm, {blocks} dielectric blocks",
+             f"DM {nx} {ny} {nz}", "GS"]
+    lines += [f"BT {f} MUR" for f in ("XLO", "XHI", "YLO", "YHI", "ZLO", "ZHI")]
+    files = {}
+    fmax = C0 / (10 * dx)
+    for w in range(6):
+        sig = 10 ** rng.uniform(5, math.log10(6e7))
+        thick = rng.uniform(1e-4, 2e-3)
+        files[f"wall{w}.pfe"] = pfe.pfe_text(pfe.fit_slab(sig, thick, fmax=fmax, per_decade=2))
+        lines.append(f'BT wall{w} SIBC "wall{w}.pfe"')
+    lo = 2
+    hi = [v - 2 for v in n]
+    lines += [f"TB {lo} {lo} {lo} {hi[1]} {lo} {hi[2]} wall0", f"TB {hi[0]} {hi[0]} {lo} {hi[1]} {lo} {hi[2]} wall1",
+              f"TB {lo} {hi[0]} {lo} {lo} {lo} {hi[2]} wall2", f"TB {lo} {hi[0]} {hi[1]} {hi[1]} {lo} {hi[2]} wall3",
+              f"TB {lo} {hi[0]} {lo} {hi[1]} {lo} {lo} wall4", f"TB {lo} {hi[0]} {lo} {hi[1]} {hi[2]} {hi[2]} wall5"]
+    for b in range(blocks):
+        lines.append(f"MT m{b} SIMPLE {rng.uniform(1, 12):.6e} 0.0 1.0")
+    for b in range(blocks):
+        box = []
+        for a in range(3):
+            w = int(rng.integers(1, max(2, n[a] // 6)))
+            x0 = int(rng.integers(lo + 2, hi[a] - 1 - w))
+            box += [x0, x0 + w]
+        lines.append(f"MB {box[0]} {box[1]} {box[2]} {box[3]} {box[4]} {box[5]} m{b}")
+    lines.append(_gauss_card("wf1", 5e9))
+    c = [int(rng.integers(lo + 3, hi[a] - 3)) for a in range(3)]
+    lines.append(f"EX {c[0]} {c[0]} {c[1]} {c[1]} {c[2]} {c[2] + 1} src IDZ wf1 1.0")
+    for o in range(observers):
+        p = [int(rng.integers(lo + 2, hi[a] - 1)) for a in range(3)]
+        lines.append(f"OP {p[0]} {p[0]} {p[1]} {p[1]} {p[2]} {p[2]} t{o} TDOM_ASCII")
+        lines.append(f"OP {p[0]} {p[0]} {p[1]} {p[1]} {p[2]} {p[2]} f{o} FDOM_ASCII")
+    lines += ["GE", f"NT {steps}", f"OF 1e8 5e9 {nfreq}", f"MS {dx:g}", "EN"]
+    return "\n".join(lines) + "\n", files
+
+
+WORKLOADS = {"freespace": freespace, "enclosure": enclosure, "debye": debye, "sibc_box": sibc_box}
 
 
 def write(name, outdir, **kw):
-    """Write workload `name` to `outdir`/<name>.mesh and return the path."""
+    """Write workload `name` to `outdir`/<name>.mesh (and any files it needs
+    beside it) and return the mesh path."""
     os.makedirs(outdir, exist_ok=True)
     path = os.path.join(outdir, f"{name}.mesh")
+    out = WORKLOADS[name](**kw)
+    text, files = out if isinstance(out, tuple) else (out, {})
     with open(path, "w") as f:
-        f.write(WORKLOADS[name](**kw))
+        f.write(text)
+    for fn, body in files.items():
+        with open(os.path.join(outdir, fn), "w") as f:
+            f.write(body)
     return path
 
 
