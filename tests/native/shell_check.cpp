@@ -86,21 +86,29 @@ static int check_sweep(const char *mesh, const A &a, const int (*in)[NB][6], con
             if (reg != -1) return fail(mesh, "updated cell in no piece", i, j, k, C);
           } else {
             const ShellPiece &pc = pcs[o];
-            long long q = -1;
-            void *xa = nullptr, *xb = nullptr;
             const int dj = j - pc.box[2], dk = k - pc.box[4];
-            const int pr = C == 0 ? shell_region<0>(&pc, i, dj, dk, q, xa, xb)
-                         : C == 1 ? shell_region<1>(&pc, i, dj, dk, q, xa, xb)
-                                  : shell_region<2>(&pc, i, dj, dk, q, xa, xb);
-            if (pr != reg) return fail(mesh, "piece region differs from the cell's", i, j, k, C);
-            if (reg >= 0 && reg < 6 && q != p) return fail(mesh, "UPML slab index", i, j, k, C);
+            // the kernels take the region at the first plane of their z chunk
+            // and step the slab index by psz per plane
+            const int dk0 = dk / pc.kc * pc.kc;
+            const ColRegion rg = C == 0 ? shell_region<0>(&pc, i, dj, dk0)
+                               : C == 1 ? shell_region<1>(&pc, i, dj, dk0)
+                                        : shell_region<2>(&pc, i, dj, dk0);
+            if (rg.reg != reg) return fail(mesh, "piece region differs from the cell's", i, j, k, C);
+            if (reg >= 0 && reg < 6 && rg.p + (dk - dk0) * rg.psz != p)
+              return fail(mesh, "UPML slab index", i, j, k, C);
           }
         }
       }
   // launch table
   if (work.size() % 8) return fail(mesh, "launch table not a multiple of 8", 0, 0, 0, -1);
   std::vector<std::vector<int>> seen(pcs.size());
-  for (size_t q = 0; q < pcs.size(); ++q) seen[q].assign((pcs[q].cells + 255) / 256, -1);
+  for (size_t q = 0; q < pcs.size(); ++q) {
+    const ShellPiece &pc = pcs[q];
+    if (pc.nk != pc.box[5] - pc.box[4] + 1 || pc.nxy != pc.ni * pc.nj || pc.nxyb != (pc.nxy + 255) / 256 ||
+        pc.kc < 1 || pc.kc > pc.nk || pc.nblk != pc.nxyb * ((pc.nk + pc.kc - 1) / pc.kc))
+      return fail(mesh, "piece block layout", pc.box[0], pc.box[2], pc.box[4], -1);
+    seen[q].assign(pc.nblk, -1);
+  }
   for (size_t b = 0; b < work.size(); ++b) {
     const ShellWork w = work[b];
     if (w.piece < 0) continue;

@@ -584,15 +584,21 @@ static int build_debye(Ctx *c)
   }
   c->d_dbase = upload_vec(c, dbase);
 
-  c->d_dcomp = upload_vec(c, comp);
-  c->d_dcell = upload_vec(c, cell);
-  c->d_dmed = upload_vec(c, med);
+  std::vector<long long> key(n);
+  for (long long g = 0; g < n; ++g) {
+    if (cell[g] >= (1LL << 40) || med[g] < 0 || med[g] >= (1 << 21))
+      return set_error(c, VGPU_EINVAL, "Debye site index out of range");
+    key[g] = (long long)med[g] << 42 | cell[g] << 2 | comp[g];
+  }
+  c->d_dkey = upload_vec(c, key);
+  c->d_dsbeta = dev_alloc(c, (size_t)n * c->esz);
   c->d_dlast = dev_alloc(c, (size_t)n * c->esz);
   c->d_dJ = dev_alloc(c, (size_t)n * maxP * (cplx ? 2 : 1) * c->esz);
   c->d_dalpha = upload_vec(c, da);
   c->d_dbeta = upload_vec(c, db);
   c->d_dnp = upload_vec(c, np);
-  if (!c->d_dlast || !c->d_dJ || !c->d_dbase) return set_error(c, VGPU_ENOMEM, "Debye state allocation");
+  if (!c->d_dkey || !c->d_dsbeta || !c->d_dlast || !c->d_dJ || !c->d_dbase)
+    return set_error(c, VGPU_ENOMEM, "Debye state allocation");
   return VGPU_OK;
 }
 

@@ -94,9 +94,12 @@ template <typename T> struct FastH {
 
 // A piece of a sweep's shell (the cells outside its fast box): a box whose
 // rows share, per component, one sequence of x runs, each run one region
-// (ordinary update or one UPML slab) over a range of x.  Cells run x-fastest,
-// 256 to a block.
+// (ordinary update or one UPML slab) over a range of x.  A block covers 256
+// consecutive cells of the piece's x-y plane (x fastest) and marches over
+// `kc` planes; block b of the piece is x-y block b % nxyb of z chunk
+// b / nxyb.
 constexpr int SHELL_RUNS = 8;
+constexpr int SHELL_KC = 4;  // planes a shell thread marches over
 struct ShellRun {
   int xa, xb;            // inclusive x range
   int reg, pad_;         // 6 ordinary, 0..5 UPML slab
@@ -106,10 +109,15 @@ struct ShellRun {
 };
 struct ShellPiece {
   int box[6];            // inclusive cell limits
-  int ni, nj, cells;     // x and y extents, cell count
+  int ni, nj, nk, cells; // extents, cell count
+  int nxy, nxyb, kc, nblk;  // x-y plane cells, its 256-cell blocks, planes per block, blocks
   int nrun[3];           // runs per component (cells outside every run are not updated)
+  int pad_;
   ShellRun run[3][SHELL_RUNS];
 };
+
+// One block of a shell launch: its piece and the piece's block (piece < 0:
+// nothing to do).
 
 struct ShellWork {
   int piece, block;
@@ -132,11 +140,9 @@ template <typename T> struct MurArgs {
 // per E cell, layers run in block order.
 template <typename T> struct DebyeArgs {
   long long n;           // sites in this layer
-  const int *comp;       // component of each site
-  const long long *cell; // device linear index
-  const int *medium;     // Debye medium slot of each site
+  const long long *key;  // per site: medium << 42 | device linear index << 2 | component
+  const T *sbeta;        // per site: beta of the site's edge
   T *E[3];
-  const T *beta[3];
   T *lastE;              // per site
   T *J;                  // [pole][site] (real poles) or [pole][site][2]
   long long stride;      // sites in the whole state block (J pole stride)
@@ -145,8 +151,6 @@ template <typename T> struct DebyeArgs {
   const int *npoles;     // [medium]
   int maxPoles;
   int cplx;
-  const uint16_t *ecls;  // coefficient classes (beta from etab when set)
-  const T *etab;
 };
 
 // SIBC faces (sibc.c:443-586): per-face geometry, per-surface constants.
@@ -400,7 +404,7 @@ struct Ctx {
   std::vector<long long> debyeLayerStart;
   long long debyeSites = 0;
   int debyeMaxPoles = 0, debyeCplx = 0;
-  void *d_dcomp = nullptr, *d_dcell = nullptr, *d_dmed = nullptr, *d_dlast = nullptr, *d_dJ = nullptr;
+  void *d_dkey = nullptr, *d_dsbeta = nullptr, *d_dlast = nullptr, *d_dJ = nullptr;
   void *d_dalpha = nullptr, *d_dbeta = nullptr, *d_dnp = nullptr;
 
   // SIBC

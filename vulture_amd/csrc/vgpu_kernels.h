@@ -88,8 +88,9 @@ VG_DEV T debye_site(T e, T beta, long long g, int m, T *__restrict__ lastE, T *_
 //    below (E) or above (H) in registers; its arguments are a few pointers
 //    and integers passed by value, so they stay in scalar registers.
 //  - the shell kernel over everything else (UPML slabs, Mur normal H, cells
-//    of one component's inner grid only, ghost layers), one thread per cell,
-//    reading its boxes and pointers from the argument block in device memory.
+//    of one component's inner grid only, ghost layers), planned on the host
+//    as pieces of x runs (ShellPiece), reading its pointers from the argument
+//    block in device memory.
 // ===========================================================================
 
 // XCD-aware block order.  Workgroups are dealt to the eight XCDs round-robin
@@ -222,186 +223,214 @@ template <typename T, bool IDX> __global__ __launch_bounds__(256) void k_h_fast(
   }
 }
 
-// Shell sweeps.  Each block is one 256-cell block of one shell piece
-// (ShellPiece): the rows of a piece share one sequence of x runs per
-// component, so a lane finds its region among a few uniform runs, and all of
-// a cell's loads are issued before its first store.  The launch order of the
-// blocks (ShellWork) gives each XCD one contiguous eighth of every piece.
-VG_DEV bool shell_cell(const ShellPiece *pcs, const ShellWork *work, const ShellPiece *&pc, int &di, int &dj, int &dk)
+// Shell sweeps.  A block is 256 cells of one shell piece's x-y plane
+// marching over a few planes (ShellPiece): the rows of a piece share one
+// sequence of x runs per component, so a lane finds its regions once, among a
+// few uniform runs; the march carries the neighbouring plane's fields in
+// registers, and each plane issues all its loads before its first store.  The
+// launch order of the blocks (ShellWork) gives each XCD one contiguous eighth
+// of every piece.
+VG_DEV bool shell_column(const ShellPiece *pcs, const ShellWork *work, const ShellPiece *&pc, int &di, int &dj,
+                         int &dk0, int &dk1)
 {
   const ShellWork w = work[blockIdx.x];
   if (w.piece < 0) return false;
   pc = pcs + w.piece;
-  const int l = w.block * 256 + (int)threadIdx.x;
-  if (l >= pc->cells) return false;
-  const int r = l / pc->ni;
-  di = l - r * pc->ni;
-  dj = r % pc->nj;
-  dk = r / pc->nj;
+  const int zc = w.block / pc->nxyb, xyb = w.block - zc * pc->nxyb;
+  const int l = xyb * 256 + (int)threadIdx.x;
+  if (l >= pc->nxy) return false;
+  dj = l / pc->ni;
+  di = l - dj * pc->ni;
+  dk0 = zc * pc->kc;
+  dk1 = min(dk0 + pc->kc, pc->nk);
   return true;
 }
 
-// Region of component C at x = i of a piece row: 6 ordinary, 0..5 UPML slab
-// with its slab index p and auxiliaries a/b; -1 not updated.
-template <int C>
-VG_HD int shell_region(const ShellPiece *pc, int i, int dj, int dk, long long &p, void *&xa, void *&xb)
+// Region of component C at x = i of a piece: reg 6 ordinary, 0..5 UPML slab
+// (p its slab index at plane dk of the piece, psz the step of p per plane,
+// a/b the slab's auxiliaries); reg -1 not updated.
+struct ColRegion {
+  int reg;
+  long long p, psz;
+  void *a, *b;
+};
+
+template <int C> VG_HD ColRegion shell_region(const ShellPiece *pc, int i, int dj, int dk)
 {
-  int reg = -1;
+  ColRegion r{-1, 0, 0, nullptr, nullptr};
   for (int q = 0; q < pc->nrun[C]; ++q) {
     const ShellRun &rn = pc->run[C][q];
     if (i >= rn.xa && i <= rn.xb) {
-      reg = rn.reg;
-      p = rn.p0 + (i - rn.xa) + dj * rn.psy + dk * rn.psz;
-      xa = rn.a;
-      xb = rn.b;
+      r.reg = rn.reg;
+      r.p = rn.p0 + (i - rn.xa) + dj * rn.psy + dk * rn.psz;
+      r.psz = rn.psz;
+      r.a = rn.a;
+      r.b = rn.b;
     }
   }
-  return reg;
+  return r;
 }
 
-// E shell (grid.c:709-795 ordinary update; pml.c:859-1023 UPML split update
-// with the axis roles of component C: P along axis C+1, E along C+2).
-template <typename T, bool IDX, int C>
-VG_DEV void e_shell_load(const EArgs<T> &a, const ShellPiece *pc, const int n3[3], int dj, int dk, long long id,
-                         unsigned cls, int &reg, long long &p, T *&pp, T *&pa, T &curl, T &e, T &al, T &be, T &opp,
-                         T &op, T (&cf)[6])
-{
-  void *xa = nullptr, *xb = nullptr;
-  reg = shell_region<C>(pc, n3[0], dj, dk, p, xa, xb);
-  if (reg < 0) return;
-  const int i = n3[0], j = n3[1], k = n3[2];
-  const long long P = a.plane, SJ = a.PX;
-  const T *Hx = a.H[0], *Hy = a.H[1], *Hz = a.H[2];
-  // (the sweep never updates index 0 of any axis; the guards keep every read
-  // inside the allocation regardless)
-  const bool xm = i > 0, ym = j > 0, zm = k > 0;
-  if (C == 0) curl = a.idh[1][j] * (Hz[id] - (ym ? Hz[id - SJ] : T(0))) + a.idh[2][k] * ((zm ? Hy[id - P] : T(0)) - Hy[id]);
-  else if (C == 1) curl = a.idh[2][k] * (Hx[id] - (zm ? Hx[id - P] : T(0))) + a.idh[0][i] * ((xm ? Hz[id - 1] : T(0)) - Hz[id]);
-  else curl = a.idh[0][i] * (Hy[id] - (xm ? Hy[id - 1] : T(0))) + a.idh[1][j] * ((ym ? Hx[id - SJ] : T(0)) - Hx[id]);
-  e = a.E[C][id];
-  if (IDX) {
-    al = a.etab[cls * 6 + 2 * C];
-    be = a.etab[cls * 6 + 2 * C + 1];
-  } else {
-    al = a.alpha[C][id];
-    be = a.beta[C][id];
+// One E component of a shell cell (grid.c:709-795 ordinary update;
+// pml.c:859-1023 UPML split update with the axis roles of component C: P
+// along axis C+1, E along C+2): loads, then the stores.
+template <typename T, bool IDX, int C> struct EShellCell {
+  T e, al, be, opp, op, cf[6];
+  VG_DEV void load(const EArgs<T> &a, const ColRegion &rg, const int n3[3], long long id, unsigned cls)
+  {
+    if (rg.reg < 0) return;
+    e = a.E[C][id];
+    if (IDX) {
+      al = a.etab[cls * 6 + 2 * C];
+      be = a.etab[cls * 6 + 2 * C + 1];
+    } else {
+      al = a.alpha[C][id];
+      be = a.beta[C][id];
+    }
+    if (rg.reg < 6) {
+      constexpr int A1 = (C + 1) % 3, A2 = (C + 2) % 3;
+      opp = static_cast<const T *>(rg.a)[rg.p];
+      op = static_cast<const T *>(rg.b)[rg.p];
+      cf[0] = a.ad[A1][n3[A1]];
+      cf[1] = a.bd[A1][n3[A1]];
+      cf[2] = a.ad[A2][n3[A2]];
+      cf[3] = a.bd[A2][n3[A2]];
+      cf[4] = a.ibh[C][n3[C]];
+      cf[5] = a.ah[C][n3[C]];
+    }
   }
-  if (reg < 6) {
-    constexpr int A1 = (C + 1) % 3, A2 = (C + 2) % 3;
-    pp = static_cast<T *>(xa);
-    pa = static_cast<T *>(xb);
-    opp = pp[p];
-    op = pa[p];
-    cf[0] = a.ad[A1][n3[A1]];
-    cf[1] = a.bd[A1][n3[A1]];
-    cf[2] = a.ad[A2][n3[A2]];
-    cf[3] = a.bd[A2][n3[A2]];
-    cf[4] = a.ibh[C][n3[C]];
-    cf[5] = a.ah[C][n3[C]];
+  VG_DEV void store(const EArgs<T> &a, const ColRegion &rg, long long id, T curl) const
+  {
+    if (rg.reg == 6) {
+      a.E[C][id] = al * e + be * curl;
+    } else if (rg.reg >= 0) {
+      const T ppn = al * opp + be * curl;
+      static_cast<T *>(rg.a)[rg.p] = ppn;
+      const T pn = cf[0] * op + cf[1] * (ppn - opp);
+      static_cast<T *>(rg.b)[rg.p] = pn;
+      a.E[C][id] = cf[2] * e + cf[3] * cf[4] * (pn - cf[5] * op);
+    }
   }
-}
-
-template <typename T, int C>
-VG_DEV void e_shell_store(const EArgs<T> &a, long long id, int reg, long long p, T *pp, T *pa, T curl, T e, T al,
-                          T be, T opp, T op, const T (&cf)[6])
-{
-  if (reg == 6) {
-    a.E[C][id] = al * e + be * curl;
-  } else if (reg >= 0) {
-    const T ppn = al * opp + be * curl;
-    pp[p] = ppn;
-    const T pn = cf[0] * op + cf[1] * (ppn - opp);
-    pa[p] = pn;
-    a.E[C][id] = cf[2] * e + cf[3] * cf[4] * (pn - cf[5] * op);
-  }
-}
+};
 
 template <typename T, bool IDX>
 __global__ __launch_bounds__(256) void k_e_shell(const ShellPiece *pcs, const ShellWork *work, const EArgs<T> *ap)
 {
   const ShellPiece *pc;
-  int di, dj, dk;
-  if (!shell_cell(pcs, work, pc, di, dj, dk)) return;
+  int di, dj, dk0, dk1;
+  if (!shell_column(pcs, work, pc, di, dj, dk0, dk1)) return;
   const EArgs<T> &a = *ap;
-  const int n3[3] = {pc->box[0] + di, pc->box[2] + dj, pc->box[4] + dk};
-  const long long id = ((long long)n3[2] * a.NY + n3[1]) * a.PX + n3[0];
-  const unsigned cls = IDX ? (unsigned)a.ecls[id] : 0u;
-  int reg[3];
-  long long p[3];
-  T *pp[3], *pa[3];
-  T curl[3], e[3], al[3], be[3], opp[3], op[3], cf[3][6];
-  e_shell_load<T, IDX, 0>(a, pc, n3, dj, dk, id, cls, reg[0], p[0], pp[0], pa[0], curl[0], e[0], al[0], be[0], opp[0], op[0], cf[0]);
-  e_shell_load<T, IDX, 1>(a, pc, n3, dj, dk, id, cls, reg[1], p[1], pp[1], pa[1], curl[1], e[1], al[1], be[1], opp[1], op[1], cf[1]);
-  e_shell_load<T, IDX, 2>(a, pc, n3, dj, dk, id, cls, reg[2], p[2], pp[2], pa[2], curl[2], e[2], al[2], be[2], opp[2], op[2], cf[2]);
-  e_shell_store<T, 0>(a, id, reg[0], p[0], pp[0], pa[0], curl[0], e[0], al[0], be[0], opp[0], op[0], cf[0]);
-  e_shell_store<T, 1>(a, id, reg[1], p[1], pp[1], pa[1], curl[1], e[1], al[1], be[1], opp[1], op[1], cf[1]);
-  e_shell_store<T, 2>(a, id, reg[2], p[2], pp[2], pa[2], curl[2], e[2], al[2], be[2], opp[2], op[2], cf[2]);
-}
-
-// H shell: ordinary update (inner grid or Mur normal H, grid.c:797-888) or
-// UPML split update (pml.c:1026-1176): B along axis C+1, H along C+2.
-template <typename T, bool IDX, int C>
-VG_DEV void h_shell_load(const HArgs<T> &a, const ShellPiece *pc, const int n3[3], int dj, int dk, long long id,
-                         unsigned cls, int &reg, long long &p, T *&pb, T &curl, T &h, T &g, T &ob, T (&cf)[6])
-{
-  void *xa = nullptr, *xb = nullptr;
-  reg = shell_region<C>(pc, n3[0], dj, dk, p, xa, xb);
-  if (reg < 0) return;
-  const int i = n3[0], j = n3[1], k = n3[2];
+  const int i = pc->box[0] + di, j = pc->box[2] + dj;
+  ColRegion rg[3] = {shell_region<0>(pc, i, dj, dk0), shell_region<1>(pc, i, dj, dk0),
+                     shell_region<2>(pc, i, dj, dk0)};
+  if (rg[0].reg < 0 && rg[1].reg < 0 && rg[2].reg < 0) return;
   const long long P = a.plane, SJ = a.PX;
-  const T *Ex = a.E[0], *Ey = a.E[1], *Ez = a.E[2];
-  const bool xin = i + 1 < a.NX, yin = j + 1 < a.NY, zin = k + 1 < a.NZ;
-  if (C == 0) curl = a.ide[2][k] * ((zin ? Ey[id + P] : T(0)) - Ey[id]) + a.ide[1][j] * (Ez[id] - (yin ? Ez[id + SJ] : T(0)));
-  else if (C == 1) curl = a.ide[0][i] * ((xin ? Ez[id + 1] : T(0)) - Ez[id]) + a.ide[2][k] * (Ex[id] - (zin ? Ex[id + P] : T(0)));
-  else curl = a.ide[1][j] * ((yin ? Ex[id + SJ] : T(0)) - Ex[id]) + a.ide[0][i] * (Ey[id] - (xin ? Ey[id + 1] : T(0)));
-  h = a.H[C][id];
-  g = IDX ? a.htab[cls * 3 + C] : a.gamma[C][id];
-  if (reg < 6) {
-    constexpr int A1 = (C + 1) % 3, A2 = (C + 2) % 3;
-    pb = static_cast<T *>(xa);
-    ob = pb[p];
-    cf[0] = a.ah[A1][n3[A1]];
-    cf[1] = a.bh[A1][n3[A1]];
-    cf[2] = a.ah[A2][n3[A2]];
-    cf[3] = a.bh[A2][n3[A2]];
-    cf[4] = a.ibd[C][n3[C]];
-    cf[5] = a.ad[C][n3[C]];
+  const T *Hx = a.H[0], *Hy = a.H[1], *Hz = a.H[2];
+  // (the sweep never updates index 0 of any axis; the guards keep every read
+  // inside the allocation regardless)
+  const bool xm = i > 0, ym = j > 0;
+  const T idh0 = a.idh[0][i], idh1 = a.idh[1][j];
+  int k = pc->box[4] + dk0;
+  long long id = ((long long)k * a.NY + j) * SJ + i;
+  T hxm = k > 0 ? Hx[id - P] : T(0), hym = k > 0 ? Hy[id - P] : T(0);  // plane k-1, carried
+  for (int dk = dk0; dk < dk1; ++dk, ++k, id += P) {
+    const T hx = Hx[id], hy = Hy[id], hz = Hz[id];
+    const T hzy = ym ? Hz[id - SJ] : T(0), hzx = xm ? Hz[id - 1] : T(0);
+    const T hyx = xm ? Hy[id - 1] : T(0), hxy = ym ? Hx[id - SJ] : T(0);
+    const T idh2 = a.idh[2][k];
+    const unsigned cls = IDX ? (unsigned)a.ecls[id] : 0u;
+    const int n3[3] = {i, j, k};
+    EShellCell<T, IDX, 0> c0;
+    EShellCell<T, IDX, 1> c1;
+    EShellCell<T, IDX, 2> c2;
+    c0.load(a, rg[0], n3, id, cls);
+    c1.load(a, rg[1], n3, id, cls);
+    c2.load(a, rg[2], n3, id, cls);
+    c0.store(a, rg[0], id, idh1 * (hz - hzy) + idh2 * (hym - hy));
+    c1.store(a, rg[1], id, idh2 * (hx - hxm) + idh0 * (hzx - hz));
+    c2.store(a, rg[2], id, idh0 * (hy - hyx) + idh1 * (hxy - hx));
+    hxm = hx;
+    hym = hy;
+    rg[0].p += rg[0].psz;
+    rg[1].p += rg[1].psz;
+    rg[2].p += rg[2].psz;
   }
 }
 
-template <typename T, int C>
-VG_DEV void h_shell_store(const HArgs<T> &a, long long id, int reg, long long p, T *pb, T curl, T h, T g, T ob,
-                          const T (&cf)[6])
-{
-  if (reg == 6) {
-    a.H[C][id] = h + g * curl;
-  } else if (reg >= 0) {
-    const T bn = cf[0] * ob + g * cf[1] * curl;
-    pb[p] = bn;
-    a.H[C][id] = cf[2] * h + cf[3] * cf[4] * (bn - cf[5] * ob);
+// One H component of a shell cell: ordinary update (inner grid or Mur normal
+// H, grid.c:797-888) or UPML split update (pml.c:1026-1176): B along axis
+// C+1, H along C+2.
+template <typename T, bool IDX, int C> struct HShellCell {
+  T h, g, ob, cf[6];
+  VG_DEV void load(const HArgs<T> &a, const ColRegion &rg, const int n3[3], long long id, unsigned cls)
+  {
+    if (rg.reg < 0) return;
+    h = a.H[C][id];
+    g = IDX ? a.htab[cls * 3 + C] : a.gamma[C][id];
+    if (rg.reg < 6) {
+      constexpr int A1 = (C + 1) % 3, A2 = (C + 2) % 3;
+      ob = static_cast<const T *>(rg.a)[rg.p];
+      cf[0] = a.ah[A1][n3[A1]];
+      cf[1] = a.bh[A1][n3[A1]];
+      cf[2] = a.ah[A2][n3[A2]];
+      cf[3] = a.bh[A2][n3[A2]];
+      cf[4] = a.ibd[C][n3[C]];
+      cf[5] = a.ad[C][n3[C]];
+    }
   }
-}
+  VG_DEV void store(const HArgs<T> &a, const ColRegion &rg, long long id, T curl) const
+  {
+    if (rg.reg == 6) {
+      a.H[C][id] = h + g * curl;
+    } else if (rg.reg >= 0) {
+      const T bn = cf[0] * ob + g * cf[1] * curl;
+      static_cast<T *>(rg.a)[rg.p] = bn;
+      a.H[C][id] = cf[2] * h + cf[3] * cf[4] * (bn - cf[5] * ob);
+    }
+  }
+};
 
 template <typename T, bool IDX>
 __global__ __launch_bounds__(256) void k_h_shell(const ShellPiece *pcs, const ShellWork *work, const HArgs<T> *ap)
 {
   const ShellPiece *pc;
-  int di, dj, dk;
-  if (!shell_cell(pcs, work, pc, di, dj, dk)) return;
+  int di, dj, dk0, dk1;
+  if (!shell_column(pcs, work, pc, di, dj, dk0, dk1)) return;
   const HArgs<T> &a = *ap;
-  const int n3[3] = {pc->box[0] + di, pc->box[2] + dj, pc->box[4] + dk};
-  const long long id = ((long long)n3[2] * a.NY + n3[1]) * a.PX + n3[0];
-  const unsigned cls = IDX ? (unsigned)a.hcls[id] : 0u;
-  int reg[3];
-  long long p[3];
-  T *pb[3];
-  T curl[3], h[3], g[3], ob[3], cf[3][6];
-  h_shell_load<T, IDX, 0>(a, pc, n3, dj, dk, id, cls, reg[0], p[0], pb[0], curl[0], h[0], g[0], ob[0], cf[0]);
-  h_shell_load<T, IDX, 1>(a, pc, n3, dj, dk, id, cls, reg[1], p[1], pb[1], curl[1], h[1], g[1], ob[1], cf[1]);
-  h_shell_load<T, IDX, 2>(a, pc, n3, dj, dk, id, cls, reg[2], p[2], pb[2], curl[2], h[2], g[2], ob[2], cf[2]);
-  h_shell_store<T, 0>(a, id, reg[0], p[0], pb[0], curl[0], h[0], g[0], ob[0], cf[0]);
-  h_shell_store<T, 1>(a, id, reg[1], p[1], pb[1], curl[1], h[1], g[1], ob[1], cf[1]);
-  h_shell_store<T, 2>(a, id, reg[2], p[2], pb[2], curl[2], h[2], g[2], ob[2], cf[2]);
+  const int i = pc->box[0] + di, j = pc->box[2] + dj;
+  ColRegion rg[3] = {shell_region<0>(pc, i, dj, dk0), shell_region<1>(pc, i, dj, dk0),
+                     shell_region<2>(pc, i, dj, dk0)};
+  if (rg[0].reg < 0 && rg[1].reg < 0 && rg[2].reg < 0) return;
+  const long long P = a.plane, SJ = a.PX;
+  const T *Ex = a.E[0], *Ey = a.E[1], *Ez = a.E[2];
+  const bool xin = i + 1 < a.NX, yin = j + 1 < a.NY;
+  const T ide0 = a.ide[0][i], ide1 = a.ide[1][j];
+  int k = pc->box[4] + dk0;
+  long long id = ((long long)k * a.NY + j) * SJ + i;
+  T exc = Ex[id], eyc = Ey[id];  // this plane's Ex, Ey: carried from the previous plane's k+1 loads
+  for (int dk = dk0; dk < dk1; ++dk, ++k, id += P) {
+    const bool zin = k + 1 < a.NZ;
+    const T exp1 = zin ? Ex[id + P] : T(0), eyp1 = zin ? Ey[id + P] : T(0);
+    const T ez = Ez[id], ezy = yin ? Ez[id + SJ] : T(0), ezx = xin ? Ez[id + 1] : T(0);
+    const T eyx = xin ? Ey[id + 1] : T(0), exy = yin ? Ex[id + SJ] : T(0);
+    const T ide2 = a.ide[2][k];
+    const unsigned cls = IDX ? (unsigned)a.hcls[id] : 0u;
+    const int n3[3] = {i, j, k};
+    HShellCell<T, IDX, 0> c0;
+    HShellCell<T, IDX, 1> c1;
+    HShellCell<T, IDX, 2> c2;
+    c0.load(a, rg[0], n3, id, cls);
+    c1.load(a, rg[1], n3, id, cls);
+    c2.load(a, rg[2], n3, id, cls);
+    c0.store(a, rg[0], id, ide2 * (eyp1 - eyc) + ide1 * (ez - ezy));
+    c1.store(a, rg[1], id, ide0 * (ezx - ez) + ide2 * (exc - exp1));
+    c2.store(a, rg[2], id, ide1 * (exy - exc) + ide0 * (eyc - eyx));
+    exc = exp1;
+    eyc = eyp1;
+    rg[0].p += rg[0].psz;
+    rg[1].p += rg[1].psz;
+    rg[2].p += rg[2].psz;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -449,12 +478,25 @@ template <typename T> __global__ void k_debye(DebyeArgs<T> a)
   const long long s = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (s >= a.n) return;
   const long long g = a.base + s;
-  const int c = a.comp[g];
-  const long long id = a.cell[g];
+  const long long key = a.key[g];
+  const int c = (int)(key & 3), m = (int)(key >> 42);
+  const long long id = (key >> 2) & ((1LL << 40) - 1);
   T *E = a.E[c];
-  const T beta = a.etab ? a.etab[(unsigned)a.ecls[id] * 6 + 2 * c + 1] : a.beta[c][id];
-  E[id] = debye_site<T>(E[id], beta, g, a.medium[g], a.lastE, a.J, a.stride, a.dalpha, a.dbeta, a.npoles, a.maxPoles,
+  E[id] = debye_site<T>(E[id], a.sbeta[g], g, m, a.lastE, a.J, a.stride, a.dalpha, a.dbeta, a.npoles, a.maxPoles,
                         a.cplx);
+}
+
+// Each Debye site's beta, gathered once: from the coefficient classes when
+// there are classes, else from the per-cell array.
+template <typename T>
+__global__ void k_debye_gather(long long n, const long long *key, const T *const *beta, const uint16_t *ecls,
+                               const T *etab, T *sbeta)
+{
+  const long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const int c = (int)(key[g] & 3);
+  const long long id = (key[g] >> 2) & ((1LL << 40) - 1);
+  sbeta[g] = etab ? etab[(unsigned)ecls[id] * 6 + 2 * c + 1] : beta[c][id];
 }
 
 // ---------------------------------------------------------------------------

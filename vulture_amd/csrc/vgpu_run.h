@@ -237,17 +237,28 @@ static int build_shell(const A &a, const int (*in)[NB][6], const int *fb, const 
   };
   merge(raw, 1);
   merge(raw, 2);
+  // planes per block: up to SHELL_KC, fewer when the shell is too small to
+  // give about 2048 blocks (8 waves per SIMD) that way
+  long long rawCells = 0;
+  for (const ShellPiece &pc : raw)
+    rawCells += (long long)(pc.box[1] - pc.box[0] + 1) * (pc.box[3] - pc.box[2] + 1) * (pc.box[5] - pc.box[4] + 1);
+  const int kcmax = (int)std::max(1LL, std::min((long long)SHELL_KC, rawCells / (256LL * 2048)));
   cells = 0;
   pcs.clear();
   for (ShellPiece pc : raw) {
     pc.ni = pc.box[1] - pc.box[0] + 1;
     pc.nj = pc.box[3] - pc.box[2] + 1;
-    const long long n = (long long)pc.ni * pc.nj * (pc.box[5] - pc.box[4] + 1);
+    pc.nk = pc.box[5] - pc.box[4] + 1;
+    const long long n = (long long)pc.ni * pc.nj * pc.nk;
     if (n > (1LL << 30)) {
       err = "shell piece too large";
       return -1;
     }
     pc.cells = (int)n;
+    pc.nxy = pc.ni * pc.nj;
+    pc.nxyb = (pc.nxy + 255) / 256;
+    pc.kc = std::min(pc.nk, kcmax);
+    pc.nblk = pc.nxyb * ((pc.nk + pc.kc - 1) / pc.kc);
     cells += n;
     pcs.push_back(pc);
   }
@@ -258,7 +269,7 @@ static int build_shell(const A &a, const int (*in)[NB][6], const int *fb, const 
   // launch order: XCD x runs blocks 8*slot + x, slot over its eighths
   std::vector<ShellWork> per[8];
   for (int q = 0; q < (int)pcs.size(); ++q) {
-    const int nb = (pcs[q].cells + 255) / 256;
+    const int nb = pcs[q].nblk;
     for (int x = 0; x < 8; ++x)
       for (int b = (int)((long long)nb * x / 8); b < (int)((long long)nb * (x + 1) / 8); ++b) per[x].push_back({q, b});
   }
@@ -310,7 +321,19 @@ template <typename T> int prepare_sweeps(Ctx *c)
       !up(c->d_ework, ew.data(), ew.size() * sizeof(ShellWork)) ||
       !up(c->d_hwork, hw.data(), hw.size() * sizeof(ShellWork)))
     return set_error(c, VGPU_EDEVICE, "sweep arguments upload");
-  // correction rows (and SIBC H rows): each row cell's coefficients, gathered once
+  // Debye sites, correction rows and SIBC H rows: each one's coefficients, gathered once
+  if (c->debyeSites) {
+    const T *b[3] = {P<T>(c->beta[0]), P<T>(c->beta[1]), P<T>(c->beta[2])};
+    void *db = dev_alloc(c, sizeof(b));
+    if (!db) return set_error(c, VGPU_ENOMEM, "Debye coefficients");
+    if (!up(db, b, sizeof(b))) return set_error(c, VGPU_EDEVICE, "Debye coefficients upload");
+    const long long n = c->debyeSites;
+    hipLaunchKernelGGL((k_debye_gather<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, n,
+                       P<long long>(c->d_dkey), P<const T *const>(db), c->indexed ? P<uint16_t>(c->d_ecls) : nullptr,
+                       c->indexed ? P<T>(c->d_etab) : nullptr, P<T>(c->d_dsbeta));
+    if (hipStreamSynchronize(c->stream) != hipSuccess)
+      return set_error(c, VGPU_EDEVICE, "Debye coefficients: %s", hipGetErrorString(hipGetLastError()));
+  }
   if (c->sibcHRows) {
     const T *g[3] = {P<T>(c->gamma[0]), P<T>(c->gamma[1]), P<T>(c->gamma[2])};
     void *dg = dev_alloc(c, sizeof(g));
@@ -439,12 +462,11 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
 
   DebyeArgs<T> da{};
   if (c->debyeSites) {
-    da.comp = P<int>(c->d_dcomp); da.cell = P<long long>(c->d_dcell); da.medium = P<int>(c->d_dmed);
-    for (int d = 0; d < 3; ++d) { da.E[d] = P<T>(c->F[d]); da.beta[d] = P<T>(c->beta[d]); }
+    da.key = P<long long>(c->d_dkey); da.sbeta = P<T>(c->d_dsbeta);
+    for (int d = 0; d < 3; ++d) da.E[d] = P<T>(c->F[d]);
     da.lastE = P<T>(c->d_dlast); da.J = P<T>(c->d_dJ); da.stride = c->debyeSites;
     da.dalpha = P<double>(c->d_dalpha); da.dbeta = P<double>(c->d_dbeta); da.npoles = P<int>(c->d_dnp);
     da.maxPoles = c->debyeMaxPoles; da.cplx = c->debyeCplx;
-    if (c->indexed) { da.ecls = P<uint16_t>(c->d_ecls); da.etab = P<T>(c->d_etab); }
   }
 
   SibcEArgs<T> se{};
