@@ -74,6 +74,23 @@ def test_reference_mesh_fp32(name, tmp_path):
     _check(_run_pair(mesh, tmp_path, "fp32"), "fp32")
 
 
+@pytest.mark.parametrize("name", [n for n in REF_MESHES if n not in FP32_MESHES])
+def test_reference_mesh_fp32_vs_float_reference(name, tmp_path):
+    """The remaining reference meshes in FP32: time series within 1e-4 or 1.5
+    times the reference's own float arithmetic's error (the float oracle,
+    byte-identical to the reference build) against FP64, whichever is larger,
+    and spectra within 0.05 dB or 1.5 times the float reference's deviation.
+    Meshes whose fields are cancellation-dominated (sgref_*) or that ring for
+    thousands of steps carry float errors above 1e-4 in the reference too."""
+    mesh = _stage(name, tmp_path / "in")
+    r = _run_pair(mesh, tmp_path, "fp32")
+    oracle.run_mesh(mesh, str(tmp_path / "oracle32"), precision="fp32")
+    ref = compare_outputs(str(tmp_path / "oracle32"), str(tmp_path / "oracle"))
+    assert r["files"] > 0
+    assert r["td"] <= max(1e-4, 1.5 * ref["td"]), (r, ref)
+    assert r["fd_db"] <= max(0.05, 1.5 * ref["fd_db"]), (r, ref)
+
+
 WORKLOADS = [
     ("freespace", {}),
     ("debye", dict(n=32, spheres=4, rmin=3, rmax=6, steps=200)),

@@ -382,6 +382,8 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
     const int gx = (fb[1] - fb[0] + 64) / 64, gy = (fb[3] - fb[2] + 4) / 4, nz = fb[5] - fb[4] + 1;
     kchunk = (int)((long long)nz * gx * gy / 16384);
     kchunk = std::max(4, std::min(kchunk, 32));
+    if (const char *ev = getenv("VGPU_FAST_KCHUNK"))  // tuning override: planes per block
+      kchunk = std::max(1, std::min(atoi(ev), nz));
     return dim3(gx, gy, (nz + kchunk - 1) / kchunk);
   };
   FastE<T> fe{};
