@@ -5,10 +5,12 @@ Outputs (git-ignored, shipped with the tree to the GPU host):
   vulture_amd/lib/libvhost.so    mesh parser / set-up / device driver (C)
   vulture_amd/bin/vgvulture      command-line solver
 
-hipcc cross-compiles for gfx950 without a GPU present.  The FP64 kernels
-and the C-ABI translation unit are compiled with -ffp-contract=off so the
-double-precision path performs the reference's operations one by one; the
-FP32 kernels may contract to FMA.
+hipcc cross-compiles for gfx950 without a GPU present.  Every unit is
+compiled with -ffp-contract=off: each path performs the reference's
+operations one by one, rounded once each, so the FP64 kernels agree with
+the FP64 restatement to the last bits and the FP32 kernels round the field
+updates as the reference's own float build does (the kernels are bound by
+memory traffic, so fused multiply-adds would buy no time).
 """
 import os
 import shutil
@@ -58,7 +60,7 @@ def build(verbose=False, force=True):
     hdrs = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")] + \
         [os.path.join(ROOT, "include", "vgpu.h")]
     units = [("vgpu.hip", ["-ffp-contract=off"]), ("vgpu_k64.hip", ["-ffp-contract=off"]),
-             ("vgpu_k32.hip", ["-ffp-contract=fast"])]
+             ("vgpu_k32.hip", ["-ffp-contract=off"])]
     objs = []
     for src, extra in units:
         s = os.path.join(CSRC, src)

@@ -11,6 +11,13 @@ namespace vgpu {
 
 #define VG_DEV __device__ __forceinline__
 #define VG_HD __host__ __device__ __forceinline__
+#define VG_GLOBAL __attribute__((address_space(1)))
+
+// A pointer read from an argument block in memory is generic to the
+// compiler, which then emits flat loads; each wait on a flat load drains the
+// vector and LDS counters together.  Device arrays are global memory: saying
+// so turns them into global loads, whose waits stay partial.
+template <typename T> VG_DEV VG_GLOBAL T *gp(T *p) { return (VG_GLOBAL T *)p; }
 
 VG_HD bool in_box(const int *b, int i, int j, int k)
 {
@@ -43,20 +50,24 @@ VG_HD int classify(const A &a, const int (*in)[6], int C, int i, int j, int k, l
 // Debye ADE correction of one site (debye.c:243-307); shared by the fused E
 // sweep (layer 0) and k_debye (later layers).  g is the site, m its medium.
 template <typename T>
-VG_DEV T debye_site(T e, T beta, long long g, int m, T *__restrict__ lastE, T *__restrict__ J, long long stride,
-                    const double *__restrict__ dalpha, const double *__restrict__ dbeta, const int *__restrict__ npoles,
-                    int maxPoles, int cplx)
+VG_DEV T debye_site(T e, T beta, long long g, int m, T *lastE_, T *J_, long long stride, const double *dalpha_,
+                    const double *dbeta_, const int *npoles_, int maxPoles, int cplx)
 {
+  VG_GLOBAL T *__restrict__ lastE = gp(lastE_);
+  VG_GLOBAL T *__restrict__ J = gp(J_);
+  const VG_GLOBAL double *__restrict__ dalpha = gp(dalpha_);
+  const VG_GLOBAL double *__restrict__ dbeta = gp(dbeta_);
+  const VG_GLOBAL int *__restrict__ npoles = gp(npoles_);
   const int np = npoles[m];
-  const double *da = dalpha + (long long)m * maxPoles * 2;
-  const double *db = dbeta + (long long)m * maxPoles * 2;
+  const VG_GLOBAL double *da = dalpha + (long long)m * maxPoles * 2;
+  const VG_GLOBAL double *db = dbeta + (long long)m * maxPoles * 2;
   if (!cplx) {
     T jsum = T(0);
     for (int p = 0; p < np; ++p) jsum = jsum + (T(1) + T(da[2 * p])) * J[p * stride + g];
     e = e - beta * jsum;
     const T de = e - lastE[g];
     for (int p = 0; p < np; ++p) {
-      T *Jp = &J[p * stride + g];
+      VG_GLOBAL T *Jp = &J[p * stride + g];
       *Jp = T(da[2 * p]) * *Jp + T(db[2 * p]) * de;
     }
   } else {
@@ -69,7 +80,7 @@ VG_DEV T debye_site(T e, T beta, long long g, int m, T *__restrict__ lastE, T *_
     e = e - beta * sr;
     const T de = e - lastE[g];
     for (int p = 0; p < np; ++p) {
-      T *Jp = &J[(p * stride + g) * 2];
+      VG_GLOBAL T *Jp = &J[(p * stride + g) * 2];
       const T jr = Jp[0], ji = Jp[1];
       const T dar = T(da[2 * p]), dai = T(da[2 * p + 1]), dbr = T(db[2 * p]), dbi = T(db[2 * p + 1]);
       Jp[0] = (dar * jr - dai * ji) + dbr * de;
@@ -115,16 +126,17 @@ VG_DEV void debye_fused(const FastE<T> &f, int bx, int j, int k, bool vx, unsign
 {
   const EArgs<T> &a = *f.ap;
   const long long row = ((long long)(C * f.NZ + k) * f.NY + j) * f.nchx + bx;
-  const int g0 = a.dbase[row];
-  if (a.dbase[row + 1] == g0) return;
-  const int ds = vx ? a.edslot[cls * 3 + C] : -1;
+  const VG_GLOBAL int *dbase = gp(a.dbase);
+  const int g0 = dbase[row];
+  if (dbase[row + 1] == g0) return;
+  const int ds = vx ? gp(a.edslot)[cls * 3 + C] : -1;
   const unsigned long long m = __ballot(ds >= 0);
   if (ds < 0) return;
   const long long g = g0 + __popcll(m & ((1ull << __lane_id()) - 1ull));
   e = debye_site<T>(e, be, g, ds, a.dlast, a.dJ, a.dstride, a.dalpha, a.dbeta, a.dnp, a.dmaxP, a.dcplx);
 }
 
-template <typename T, bool IDX, int C>
+template <typename T, bool IDX, bool FUSE, int C>
 VG_DEV void e_fast(const FastE<T> &f, int bx, int j, int k, bool vx, long long id, unsigned cls, T curl)
 {
   T *__restrict__ E = f.E[C];
@@ -137,12 +149,13 @@ VG_DEV void e_fast(const FastE<T> &f, int bx, int j, int k, bool vx, long long i
     be = f.beta[C][id];
   }
   T e = al * E[id] + be * curl;
-  if (IDX && f.dfuse) debye_fused<T, C>(f, bx, j, k, vx, cls, be, e);
+  if (FUSE) debye_fused<T, C>(f, bx, j, k, vx, cls, be, e);
   if (vx) E[id] = e;
 }
 
-// Fast E kernel: inner grid (grid.c:709-797) and the first Debye layer.
-template <typename T, bool IDX> __global__ __launch_bounds__(256) void k_e_fast(FastE<T> f)
+// Fast E kernel: inner grid (grid.c:709-797) and, with FUSE, the first Debye
+// layer.
+template <typename T, bool IDX, bool FUSE> __global__ __launch_bounds__(256) void k_e_fast(FastE<T> f)
 {
   const unsigned L = xcd_block();
   if (L >= (unsigned)(f.nchx * f.gy * f.gz)) return;
@@ -169,9 +182,9 @@ template <typename T, bool IDX> __global__ __launch_bounds__(256) void k_e_fast(
     const T hzy = Hz[id - SJ], hzx = Hz[id - 1], hyx = Hy[id - 1], hxy = Hx[id - SJ];
     const unsigned cls = IDX ? (unsigned)f.ecls[id] : 0u;
     const T idhz = f.idh[2][k];
-    e_fast<T, IDX, 0>(f, bx, j, k, vx, id, cls, idhy * (hz - hzy) + idhz * (hym - hy));
-    e_fast<T, IDX, 1>(f, bx, j, k, vx, id, cls, idhz * (hx - hxm) + idhx * (hzx - hz));
-    e_fast<T, IDX, 2>(f, bx, j, k, vx, id, cls, idhx * (hy - hyx) + idhy * (hxy - hx));
+    e_fast<T, IDX, FUSE, 0>(f, bx, j, k, vx, id, cls, idhy * (hz - hzy) + idhz * (hym - hy));
+    e_fast<T, IDX, FUSE, 1>(f, bx, j, k, vx, id, cls, idhz * (hx - hxm) + idhx * (hzx - hz));
+    e_fast<T, IDX, FUSE, 2>(f, bx, j, k, vx, id, cls, idhx * (hy - hyx) + idhy * (hxy - hx));
     hxm = hx;
     hym = hy;
   }
@@ -271,44 +284,68 @@ template <int C> VG_HD ColRegion shell_region(const ShellPiece *pc, int i, int d
   return r;
 }
 
+// The E shell's arrays as global pointers, read from the argument block
+// once, before the plane loop.
+template <typename T> struct EShellPtrs {
+  VG_GLOBAL T *E[3];
+  VG_GLOBAL const T *H[3], *alpha[3], *beta[3], *ad[3], *bd[3], *ah[3], *ibh[3], *idh[3], *etab;
+  VG_GLOBAL const uint16_t *ecls;
+  VG_DEV explicit EShellPtrs(const EArgs<T> &a)
+  {
+    for (int d = 0; d < 3; ++d) {
+      E[d] = gp(a.E[d]);
+      H[d] = gp(a.H[d]);
+      alpha[d] = gp(a.alpha[d]);
+      beta[d] = gp(a.beta[d]);
+      ad[d] = gp(a.ad[d]);
+      bd[d] = gp(a.bd[d]);
+      ah[d] = gp(a.ah[d]);
+      ibh[d] = gp(a.ibh[d]);
+      idh[d] = gp(a.idh[d]);
+    }
+    etab = gp(a.etab);
+    ecls = gp(a.ecls);
+  }
+};
+
 // One E component of a shell cell (grid.c:709-795 ordinary update;
 // pml.c:859-1023 UPML split update with the axis roles of component C: P
 // along axis C+1, E along C+2): loads, then the stores.
 template <typename T, bool IDX, int C> struct EShellCell {
   T e, al, be, opp, op, cf[6];
-  VG_DEV void load(const EArgs<T> &a, const ColRegion &rg, const int n3[3], long long id, unsigned cls)
+  VG_DEV void load(const EShellPtrs<T> &g, const ColRegion &rg, const int n3[3], long long id, unsigned cls)
   {
     if (rg.reg < 0) return;
-    e = a.E[C][id];
+    e = g.E[C][id];
     if (IDX) {
-      al = a.etab[cls * 6 + 2 * C];
-      be = a.etab[cls * 6 + 2 * C + 1];
+      al = g.etab[cls * 6 + 2 * C];
+      be = g.etab[cls * 6 + 2 * C + 1];
     } else {
-      al = a.alpha[C][id];
-      be = a.beta[C][id];
+      al = g.alpha[C][id];
+      be = g.beta[C][id];
     }
     if (rg.reg < 6) {
       constexpr int A1 = (C + 1) % 3, A2 = (C + 2) % 3;
-      opp = static_cast<const T *>(rg.a)[rg.p];
-      op = static_cast<const T *>(rg.b)[rg.p];
-      cf[0] = a.ad[A1][n3[A1]];
-      cf[1] = a.bd[A1][n3[A1]];
-      cf[2] = a.ad[A2][n3[A2]];
-      cf[3] = a.bd[A2][n3[A2]];
-      cf[4] = a.ibh[C][n3[C]];
-      cf[5] = a.ah[C][n3[C]];
+      opp = gp(static_cast<const T *>(rg.a))[rg.p];
+      op = gp(static_cast<const T *>(rg.b))[rg.p];
+      cf[0] = g.ad[A1][n3[A1]];
+      cf[1] = g.bd[A1][n3[A1]];
+      cf[2] = g.ad[A2][n3[A2]];
+      cf[3] = g.bd[A2][n3[A2]];
+      cf[4] = g.ibh[C][n3[C]];
+      cf[5] = g.ah[C][n3[C]];
     }
   }
-  VG_DEV void store(const EArgs<T> &a, const ColRegion &rg, long long id, T curl) const
+  VG_DEV void store(const EShellPtrs<T> &g, const ColRegion &rg, long long id, T curl) const
   {
     if (rg.reg == 6) {
-      a.E[C][id] = al * e + be * curl;
+      g.E[C][id] = al * e + be * curl;
     } else if (rg.reg >= 0) {
       const T ppn = al * opp + be * curl;
-      static_cast<T *>(rg.a)[rg.p] = ppn;
+      gp(static_cast<T *>(rg.a))[rg.p] = ppn;
       const T pn = cf[0] * op + cf[1] * (ppn - opp);
-      static_cast<T *>(rg.b)[rg.p] = pn;
-      a.E[C][id] = cf[2] * e + cf[3] * cf[4] * (pn - cf[5] * op);
+      gp(static_cast<T *>(rg.b))[rg.p] = pn;
+      g.E[C][id] = cf[2] * e + cf[3] * cf[4] * (pn - cf[5] * op);
     }
   }
 };
@@ -320,35 +357,37 @@ __global__ __launch_bounds__(256) void k_e_shell(const ShellPiece *pcs, const Sh
   int di, dj, dk0, dk1;
   if (!shell_column(pcs, work, pc, di, dj, dk0, dk1)) return;
   const EArgs<T> &a = *ap;
+  const EShellPtrs<T> g(a);
+  const int NY = a.NY;
+  const long long P = a.plane, SJ = a.PX;
   const int i = pc->box[0] + di, j = pc->box[2] + dj;
   ColRegion rg[3] = {shell_region<0>(pc, i, dj, dk0), shell_region<1>(pc, i, dj, dk0),
                      shell_region<2>(pc, i, dj, dk0)};
   if (rg[0].reg < 0 && rg[1].reg < 0 && rg[2].reg < 0) return;
-  const long long P = a.plane, SJ = a.PX;
-  const T *Hx = a.H[0], *Hy = a.H[1], *Hz = a.H[2];
+  const VG_GLOBAL T *Hx = g.H[0], *Hy = g.H[1], *Hz = g.H[2];
   // (the sweep never updates index 0 of any axis; the guards keep every read
   // inside the allocation regardless)
   const bool xm = i > 0, ym = j > 0;
-  const T idh0 = a.idh[0][i], idh1 = a.idh[1][j];
+  const T idh0 = g.idh[0][i], idh1 = g.idh[1][j];
   int k = pc->box[4] + dk0;
-  long long id = ((long long)k * a.NY + j) * SJ + i;
+  long long id = ((long long)k * NY + j) * SJ + i;
   T hxm = k > 0 ? Hx[id - P] : T(0), hym = k > 0 ? Hy[id - P] : T(0);  // plane k-1, carried
   for (int dk = dk0; dk < dk1; ++dk, ++k, id += P) {
     const T hx = Hx[id], hy = Hy[id], hz = Hz[id];
     const T hzy = ym ? Hz[id - SJ] : T(0), hzx = xm ? Hz[id - 1] : T(0);
     const T hyx = xm ? Hy[id - 1] : T(0), hxy = ym ? Hx[id - SJ] : T(0);
-    const T idh2 = a.idh[2][k];
-    const unsigned cls = IDX ? (unsigned)a.ecls[id] : 0u;
+    const T idh2 = g.idh[2][k];
+    const unsigned cls = IDX ? (unsigned)g.ecls[id] : 0u;
     const int n3[3] = {i, j, k};
     EShellCell<T, IDX, 0> c0;
     EShellCell<T, IDX, 1> c1;
     EShellCell<T, IDX, 2> c2;
-    c0.load(a, rg[0], n3, id, cls);
-    c1.load(a, rg[1], n3, id, cls);
-    c2.load(a, rg[2], n3, id, cls);
-    c0.store(a, rg[0], id, idh1 * (hz - hzy) + idh2 * (hym - hy));
-    c1.store(a, rg[1], id, idh2 * (hx - hxm) + idh0 * (hzx - hz));
-    c2.store(a, rg[2], id, idh0 * (hy - hyx) + idh1 * (hxy - hx));
+    c0.load(g, rg[0], n3, id, cls);
+    c1.load(g, rg[1], n3, id, cls);
+    c2.load(g, rg[2], n3, id, cls);
+    c0.store(g, rg[0], id, idh1 * (hz - hzy) + idh2 * (hym - hy));
+    c1.store(g, rg[1], id, idh2 * (hx - hxm) + idh0 * (hzx - hz));
+    c2.store(g, rg[2], id, idh0 * (hy - hyx) + idh1 * (hxy - hx));
     hxm = hx;
     hym = hy;
     rg[0].p += rg[0].psz;
@@ -357,35 +396,57 @@ __global__ __launch_bounds__(256) void k_e_shell(const ShellPiece *pcs, const Sh
   }
 }
 
+// The H shell's arrays as global pointers.
+template <typename T> struct HShellPtrs {
+  VG_GLOBAL T *H[3];
+  VG_GLOBAL const T *E[3], *gamma[3], *ad[3], *ah[3], *bh[3], *ibd[3], *ide[3], *htab;
+  VG_GLOBAL const uint16_t *hcls;
+  VG_DEV explicit HShellPtrs(const HArgs<T> &a)
+  {
+    for (int d = 0; d < 3; ++d) {
+      H[d] = gp(a.H[d]);
+      E[d] = gp(a.E[d]);
+      gamma[d] = gp(a.gamma[d]);
+      ad[d] = gp(a.ad[d]);
+      ah[d] = gp(a.ah[d]);
+      bh[d] = gp(a.bh[d]);
+      ibd[d] = gp(a.ibd[d]);
+      ide[d] = gp(a.ide[d]);
+    }
+    htab = gp(a.htab);
+    hcls = gp(a.hcls);
+  }
+};
+
 // One H component of a shell cell: ordinary update (inner grid or Mur normal
 // H, grid.c:797-888) or UPML split update (pml.c:1026-1176): B along axis
 // C+1, H along C+2.
 template <typename T, bool IDX, int C> struct HShellCell {
   T h, g, ob, cf[6];
-  VG_DEV void load(const HArgs<T> &a, const ColRegion &rg, const int n3[3], long long id, unsigned cls)
+  VG_DEV void load(const HShellPtrs<T> &q, const ColRegion &rg, const int n3[3], long long id, unsigned cls)
   {
     if (rg.reg < 0) return;
-    h = a.H[C][id];
-    g = IDX ? a.htab[cls * 3 + C] : a.gamma[C][id];
+    h = q.H[C][id];
+    g = IDX ? q.htab[cls * 3 + C] : q.gamma[C][id];
     if (rg.reg < 6) {
       constexpr int A1 = (C + 1) % 3, A2 = (C + 2) % 3;
-      ob = static_cast<const T *>(rg.a)[rg.p];
-      cf[0] = a.ah[A1][n3[A1]];
-      cf[1] = a.bh[A1][n3[A1]];
-      cf[2] = a.ah[A2][n3[A2]];
-      cf[3] = a.bh[A2][n3[A2]];
-      cf[4] = a.ibd[C][n3[C]];
-      cf[5] = a.ad[C][n3[C]];
+      ob = gp(static_cast<const T *>(rg.a))[rg.p];
+      cf[0] = q.ah[A1][n3[A1]];
+      cf[1] = q.bh[A1][n3[A1]];
+      cf[2] = q.ah[A2][n3[A2]];
+      cf[3] = q.bh[A2][n3[A2]];
+      cf[4] = q.ibd[C][n3[C]];
+      cf[5] = q.ad[C][n3[C]];
     }
   }
-  VG_DEV void store(const HArgs<T> &a, const ColRegion &rg, long long id, T curl) const
+  VG_DEV void store(const HShellPtrs<T> &q, const ColRegion &rg, long long id, T curl) const
   {
     if (rg.reg == 6) {
-      a.H[C][id] = h + g * curl;
+      q.H[C][id] = h + g * curl;
     } else if (rg.reg >= 0) {
       const T bn = cf[0] * ob + g * cf[1] * curl;
-      static_cast<T *>(rg.a)[rg.p] = bn;
-      a.H[C][id] = cf[2] * h + cf[3] * cf[4] * (bn - cf[5] * ob);
+      gp(static_cast<T *>(rg.a))[rg.p] = bn;
+      q.H[C][id] = cf[2] * h + cf[3] * cf[4] * (bn - cf[5] * ob);
     }
   }
 };
@@ -397,34 +458,36 @@ __global__ __launch_bounds__(256) void k_h_shell(const ShellPiece *pcs, const Sh
   int di, dj, dk0, dk1;
   if (!shell_column(pcs, work, pc, di, dj, dk0, dk1)) return;
   const HArgs<T> &a = *ap;
+  const HShellPtrs<T> q(a);
+  const int NX = a.NX, NY = a.NY, NZ = a.NZ;
+  const long long P = a.plane, SJ = a.PX;
   const int i = pc->box[0] + di, j = pc->box[2] + dj;
   ColRegion rg[3] = {shell_region<0>(pc, i, dj, dk0), shell_region<1>(pc, i, dj, dk0),
                      shell_region<2>(pc, i, dj, dk0)};
   if (rg[0].reg < 0 && rg[1].reg < 0 && rg[2].reg < 0) return;
-  const long long P = a.plane, SJ = a.PX;
-  const T *Ex = a.E[0], *Ey = a.E[1], *Ez = a.E[2];
-  const bool xin = i + 1 < a.NX, yin = j + 1 < a.NY;
-  const T ide0 = a.ide[0][i], ide1 = a.ide[1][j];
+  const VG_GLOBAL T *Ex = q.E[0], *Ey = q.E[1], *Ez = q.E[2];
+  const bool xin = i + 1 < NX, yin = j + 1 < NY;
+  const T ide0 = q.ide[0][i], ide1 = q.ide[1][j];
   int k = pc->box[4] + dk0;
-  long long id = ((long long)k * a.NY + j) * SJ + i;
+  long long id = ((long long)k * NY + j) * SJ + i;
   T exc = Ex[id], eyc = Ey[id];  // this plane's Ex, Ey: carried from the previous plane's k+1 loads
   for (int dk = dk0; dk < dk1; ++dk, ++k, id += P) {
-    const bool zin = k + 1 < a.NZ;
+    const bool zin = k + 1 < NZ;
     const T exp1 = zin ? Ex[id + P] : T(0), eyp1 = zin ? Ey[id + P] : T(0);
     const T ez = Ez[id], ezy = yin ? Ez[id + SJ] : T(0), ezx = xin ? Ez[id + 1] : T(0);
     const T eyx = xin ? Ey[id + 1] : T(0), exy = yin ? Ex[id + SJ] : T(0);
-    const T ide2 = a.ide[2][k];
-    const unsigned cls = IDX ? (unsigned)a.hcls[id] : 0u;
+    const T ide2 = q.ide[2][k];
+    const unsigned cls = IDX ? (unsigned)q.hcls[id] : 0u;
     const int n3[3] = {i, j, k};
     HShellCell<T, IDX, 0> c0;
     HShellCell<T, IDX, 1> c1;
     HShellCell<T, IDX, 2> c2;
-    c0.load(a, rg[0], n3, id, cls);
-    c1.load(a, rg[1], n3, id, cls);
-    c2.load(a, rg[2], n3, id, cls);
-    c0.store(a, rg[0], id, ide2 * (eyp1 - eyc) + ide1 * (ez - ezy));
-    c1.store(a, rg[1], id, ide0 * (ezx - ez) + ide2 * (exc - exp1));
-    c2.store(a, rg[2], id, ide1 * (exy - exc) + ide0 * (eyc - eyx));
+    c0.load(q, rg[0], n3, id, cls);
+    c1.load(q, rg[1], n3, id, cls);
+    c2.load(q, rg[2], n3, id, cls);
+    c0.store(q, rg[0], id, ide2 * (eyp1 - eyc) + ide1 * (ez - ezy));
+    c1.store(q, rg[1], id, ide0 * (ezx - ez) + ide2 * (exc - exp1));
+    c2.store(q, rg[2], id, ide1 * (exy - exc) + ide0 * (eyc - eyx));
     exc = exp1;
     eyc = eyp1;
     rg[0].p += rg[0].psz;

@@ -410,8 +410,9 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
   const ShellWork *ewk = P<ShellWork>(c->d_ework), *hwk = P<ShellWork>(c->d_hwork);
   auto fast_e = [&]() {
     if (egrid.x) {
-      if (c->indexed) hipLaunchKernelGGL((k_e_fast<T, true>), eg1, vblk, 0, s, fe);
-      else hipLaunchKernelGGL((k_e_fast<T, false>), eg1, vblk, 0, s, fe);
+      if (c->indexed && c->debyeFused) hipLaunchKernelGGL((k_e_fast<T, true, true>), eg1, vblk, 0, s, fe);
+      else if (c->indexed) hipLaunchKernelGGL((k_e_fast<T, true, false>), eg1, vblk, 0, s, fe);
+      else hipLaunchKernelGGL((k_e_fast<T, false, false>), eg1, vblk, 0, s, fe);
     }
   };
   auto shell_e = [&]() {
