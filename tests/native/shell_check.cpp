@@ -2,7 +2,8 @@
 // set-up of each mesh against the stand-in device library, replays the
 // library's own planning code (fast_boxes, fill_volume_args, build_shell)
 // on the captured set-up, and checks every cell of the allocation:
-//   - inside a fast box every component takes the ordinary update;
+//   - inside a fast box (exact, or cut to whole lines of 16 or 32 elements)
+//     every component takes the ordinary update;
 //   - every other cell lies in at most one shell piece, whose x runs give
 //     the cell's own regions (and UPML slab indices), and a cell in no piece
 //     has no component updated;
@@ -168,9 +169,13 @@ int main(int argc, char **argv)
     for (int C = 0; C < 3; ++C)
       for (int q = 0; q < 6; ++q) ein[C][0][q] = ea.in[C][q];
     long long pe = 0, ph = 0, se = 0, sh = 0;
-    fast_boxes(c->grid.gfilim, N, c->fastE, c->fastH);
-    if (check_sweep<1>(argv[m], ea, ein, c->fastE, N, pe, se) || check_sweep<3>(argv[m], ha, ha.in, c->fastH, N, ph, sh))
-      return 1;
+    // exact fast boxes, and boxes cut to whole lines of FP64 and FP32 elements
+    for (int lw : {0, 16, 32}) {
+      fast_boxes(c->grid.gfilim, N, c->fastE, c->fastH, lw);
+      if (check_sweep<1>(argv[m], ea, ein, c->fastE, N, pe, se) ||
+          check_sweep<3>(argv[m], ha, ha.in, c->fastH, N, ph, sh))
+        return 1;
+    }
     printf("%s OK cells %lld E pieces %lld shell %lld H pieces %lld shell %lld\n", argv[m],
            (long long)N[0] * N[1] * N[2], pe, se, ph, sh);
     delete c;

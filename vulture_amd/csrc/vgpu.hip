@@ -211,7 +211,9 @@ int vgpu_create(const vgpu_grid_desc *g, vgpu_prec prec, int device, vgpu_ctx **
   c->grid.idhx = c->grid.idhy = c->grid.idhz = nullptr;
   {
     const int N[3] = {c->NX, c->NY, c->NZ};
-    fast_boxes(g->gfilim, N, c->fastE, c->fastH);
+    // x range of the fast boxes on whole 128-byte lines (VGPU_ALIGN_FAST=0 keeps the exact boxes)
+    const char *al = getenv("VGPU_ALIGN_FAST");
+    fast_boxes(g->gfilim, N, c->fastE, c->fastH, al && al[0] == '0' ? 0 : (int)(128 / c->esz));
   }
   c->NCH = c->fastE[0] <= c->fastE[1] ? (c->fastE[1] - c->fastE[0] + 64) / 64 : 0;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {

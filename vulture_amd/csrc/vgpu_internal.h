@@ -320,7 +320,10 @@ struct Region { int lim[6]; };
 // Fast boxes of the two sweeps: where every E (H) component takes the
 // ordinary update (the intersection of the inner-grid limits gfilim), kept
 // one cell inside the allocation so the fast kernels never test bounds.
-inline void fast_boxes(const int gfilim[6][6], const int N[3], int fastE[6], int fastH[6])
+// With lw > 0 the x range is cut to whole lines of lw elements (when that
+// keeps most of it): then no line is shared by the fast kernel and the
+// shell, and the shell's row ends fill their lines.
+inline void fast_boxes(const int gfilim[6][6], const int N[3], int fastE[6], int fastH[6], int lw = 0)
 {
   for (int e = 0; e < 2; ++e) {
     int *fb = e == 0 ? fastE : fastH;
@@ -330,6 +333,13 @@ inline void fast_boxes(const int gfilim[6][6], const int N[3], int fastE[6], int
       for (int d = 0; d < 3; ++d) {
         fb[2 * a] = fb[2 * a] > gfilim[3 * e + d][2 * a] ? fb[2 * a] : gfilim[3 * e + d][2 * a];
         fb[2 * a + 1] = fb[2 * a + 1] < gfilim[3 * e + d][2 * a + 1] ? fb[2 * a + 1] : gfilim[3 * e + d][2 * a + 1];
+      }
+    }
+    if (lw > 0) {
+      const int x0 = (fb[0] + lw - 1) / lw * lw, x1 = (fb[1] + 1) / lw * lw - 1;
+      if (fb[1] - fb[0] + 1 >= 4 * lw && x1 - x0 + 1 >= 2 * lw) {
+        fb[0] = x0;
+        fb[1] = x1;
       }
     }
   }
