@@ -67,7 +67,7 @@ template <typename T> struct FastE {
   long long plane;
   int box[6];            // fast box (every E component inner-grid)
   int kchunk, nchx;      // planes per block; 64-cell x chunks from box[0]
-  int gy, gz;            // 4-row y blocks, z chunks (blocks = nchx*gy*gz, launched in XCD order)
+  int gy, gz;            // y blocks (blockDim.y rows), z chunks (blocks = nchx*gy*gz, launched in XCD order)
   T *E[3];
   const T *H[3];
   const T *idh[3];

@@ -155,13 +155,13 @@ VG_DEV void e_fast(const FastE<T> &f, int bx, int j, int k, bool vx, long long i
 
 // Fast E kernel: inner grid (grid.c:709-797) and, with FUSE, the first Debye
 // layer.
-template <typename T, bool IDX, bool FUSE> __global__ __launch_bounds__(256) void k_e_fast(FastE<T> f)
+template <typename T, bool IDX, bool FUSE> __global__ __launch_bounds__(512) void k_e_fast(FastE<T> f)
 {
   const unsigned L = xcd_block();
   if (L >= (unsigned)(f.nchx * f.gy * f.gz)) return;
   const int bx = (int)(L % f.nchx), by = (int)((L / f.nchx) % f.gy), bz = (int)(L / (f.nchx * f.gy));
   const int i = f.box[0] + bx * 64 + (int)threadIdx.x;
-  const int j = f.box[2] + by * 4 + (int)threadIdx.y;
+  const int j = f.box[2] + by * (int)blockDim.y + (int)threadIdx.y;
   if (j > f.box[3]) return;  // whole wavefront
   const bool vx = i <= f.box[1];
   const int ic = vx ? i : f.box[1];
@@ -191,13 +191,13 @@ template <typename T, bool IDX, bool FUSE> __global__ __launch_bounds__(256) voi
 }
 
 // Fast H kernel: inner grid (grid.c:800-888).
-template <typename T, bool IDX> __global__ __launch_bounds__(256) void k_h_fast(FastH<T> f)
+template <typename T, bool IDX> __global__ __launch_bounds__(512) void k_h_fast(FastH<T> f)
 {
   const unsigned L = xcd_block();
   if (L >= (unsigned)(f.nchx * f.gy * f.gz)) return;
   const int bx = (int)(L % f.nchx), by = (int)((L / f.nchx) % f.gy), bz = (int)(L / (f.nchx * f.gy));
   const int i = f.box[0] + bx * 64 + (int)threadIdx.x;
-  const int j = f.box[2] + by * 4 + (int)threadIdx.y;
+  const int j = f.box[2] + by * (int)blockDim.y + (int)threadIdx.y;
   if (j > f.box[3]) return;
   const bool vx = i <= f.box[1];
   const int ic = vx ? i : f.box[1];

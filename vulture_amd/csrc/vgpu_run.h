@@ -374,12 +374,15 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
   EArgs<T> ea{};
   HArgs<T> ha{};
   fill_volume_args<T>(c, ea, ha);
-  // Fast kernels: 64x4 tiles over the fast boxes, marching z in chunks sized
+  // Fast kernels: 64x4 tiles (64 x ty) over the fast boxes, marching z in chunks sized
   // for about 16k blocks at 272^3 (at least 4 planes so the carry pays).
-  const dim3 vblk(64, 4, 1);
+  // rows per block (tuning override VGPU_FAST_TY: 1..8)
+  int ty = 4;
+  if (const char *ev = getenv("VGPU_FAST_TY")) ty = std::max(1, std::min(atoi(ev), 8));
+  const dim3 vblk(64, ty, 1);
   auto fast_grid = [&](const int *fb, int &kchunk) {
     if (fb[0] > fb[1] || fb[2] > fb[3] || fb[4] > fb[5]) return dim3(0, 0, 0);
-    const int gx = (fb[1] - fb[0] + 64) / 64, gy = (fb[3] - fb[2] + 4) / 4, nz = fb[5] - fb[4] + 1;
+    const int gx = (fb[1] - fb[0] + 64) / 64, gy = (fb[3] - fb[2] + ty) / ty, nz = fb[5] - fb[4] + 1;
     kchunk = (int)((long long)nz * gx * gy / 16384);
     kchunk = std::max(4, std::min(kchunk, 32));
     if (const char *ev = getenv("VGPU_FAST_KCHUNK"))  // tuning override: planes per block
