@@ -133,6 +133,7 @@ def sibc_box(work, steps, warmup=5):
     return {"workload": "SIBC + Mur box 384x384x256 cubic 1.5 mm, six fitted conductor walls, 64 dielectric "
                         "blocks, current-moment dipole (seed 4)", "dtype": "fp32", "steps": steps,
             "cells": st["cells"], "sibc_faces": st["sibc_faces"],
+            "sibc_state": ["complex per filter", "real per filter", "real per input"][st["sibc_state"]],
             "value": round(st["cells"] * steps / (total_ms * 1e-3) / 1e6, 1),
             "ms_per_step": round(total_ms / steps, 4), "fast_e_ms": round(e_ms, 5), "fast_h_ms": round(h_ms, 5),
             "setup_s": round(setup_s, 2)}

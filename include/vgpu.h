@@ -187,6 +187,8 @@ typedef struct {
   uint64_t debye_fused_sites;  /* Debye sites applied inside the fast E kernel */
   int indexed_media;           /* 1: coefficients read as 16-bit classes */
   int debye_fused;             /* 1: first Debye layer fused into the E sweep */
+  int sibc_state;              /* SIBC filter state: 0 complex per filter, 1 real per filter, 2 real per input */
+  int pad_;
 } vgpu_stats;
 int vgpu_get_stats(const vgpu_ctx *c, vgpu_stats *out);
 

@@ -203,3 +203,16 @@ def test_fused_debye_spheres(tmp_path, monkeypatch):
     monkeypatch.setenv("VGPU_DEBYE_FUSION", "1")
     mesh = meshgen.write("debye", str(tmp_path), n=40, spheres=6, rmin=3, rmax=8, steps=150, seed=7)
     _check(_run_pair(mesh, tmp_path, "fp64"), "fp64")
+
+
+def test_sibc_state_forms(tmp_path):
+    """The SIBC filter state takes the form the coefficients allow: complex
+    per filter for the reference's complex-pole model (closedbox_go04), real
+    per filter for fitted slabs in FP64, real per (input, pole) for fitted
+    slabs in FP32 -- and the SIBC box meets its bars in each (test_workload)."""
+    box = meshgen.write("sibc_box", str(tmp_path / "box"), nx=24, ny=24, nz=20, blocks=2, steps=10, observers=1,
+                        nfreq=4)
+    go04 = _stage("closedbox_go04", tmp_path / "go04")
+    for mesh, prec, form in ((go04, "fp64", 0), (go04, "fp32", 0), (box, "fp64", 1), (box, "fp32", 2)):
+        with vulture_amd.DeviceSolve(mesh, precision=prec) as ds:
+            assert ds.stats()["sibc_state"] == form, (mesh, prec)

@@ -107,7 +107,8 @@ class Stats(ctypes.Structure):
                 ("debye_poles", ctypes.c_int), ("debye_complex", ctypes.c_int), ("sibc_faces", ctypes.c_uint64),
                 ("correction_rows", ctypes.c_uint64 * 2), ("bytes", ctypes.c_uint64),
                 ("fast_cells", ctypes.c_uint64 * 2), ("debye_fused_sites", ctypes.c_uint64),
-                ("indexed_media", ctypes.c_int), ("debye_fused", ctypes.c_int)]
+                ("indexed_media", ctypes.c_int), ("debye_fused", ctypes.c_int), ("sibc_state", ctypes.c_int),
+                ("pad_", ctypes.c_int)]
 
 
 class DeviceSolve:

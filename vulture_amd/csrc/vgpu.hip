@@ -648,8 +648,13 @@ static int build_sibc(Ctx *c)
   std::vector<long long> zeroKeys;
   long long f = 0, seq = 0;
   const std::vector<double> *ide[3] = {&c->idex, &c->idey, &c->idez};
+  std::vector<SibcBlock> blocks;
   for (int s = 0; s < ns; ++s) {
     const vgpu_sibc_desc &d = c->sibc[s].d;
+    {
+      const long long nfs = (long long)(d.gbbox[1] - d.gbbox[0]) * (d.gbbox[3] - d.gbbox[2]) * (d.gbbox[5] - d.gbbox[4]);
+      for (long long f0 = 0; f0 < nfs; f0 += 128) blocks.push_back({s, 0, f + f0, f + nfs});
+    }
     normal[s] = d.normal;
     set[s] = s;
     for (int p = 0; p < 16; ++p) {
@@ -669,12 +674,20 @@ static int build_sibc(Ctx *c)
       return (d.pmc_index[dir][0] >= 0 && idx == d.pmc_index[dir][0]) ||
              (d.pmc_index[dir][1] >= 0 && idx == d.pmc_index[dir][1]);
     };
+    // Faces are visited in the reference's order (k fastest), which fixes
+    // the order of the H terms; they are stored x-fastest (then y, then z),
+    // so a wavefront of faces of a y- or z-normal wall reads and writes
+    // consecutive cells.
     long long lf = 0;
+    const long long f0 = f;
+    const int ni = d.gbbox[1] - d.gbbox[0], nj = d.gbbox[3] - d.gbbox[2];
     for (int i = d.gbbox[0]; i < d.gbbox[1]; ++i)
       for (int j = d.gbbox[2]; j < d.gbbox[3]; ++j)
-        for (int k = d.gbbox[4]; k < d.gbbox[5]; ++k, ++lf, ++f) {
+        for (int k = d.gbbox[4]; k < d.gbbox[5]; ++k, ++lf) {
           if (i < 1 || j < 1 || k < 1 || i + 1 >= c->NX || j + 1 >= c->NY || k + 1 >= c->NZ)
             return set_error(c, VGPU_EINVAL, "SIBC face too close to the grid edge");
+          const long long f =
+              f0 + ((long long)(k - d.gbbox[4]) * nj + (j - d.gbbox[2])) * ni + (i - d.gbbox[0]);
           ijk[3 * f] = i; ijk[3 * f + 1] = j; ijk[3 * f + 2] = k;
           surf[f] = s;
           const unsigned char *a = &c->sibc[s].adjA[4 * lf], *b = &c->sibc[s].adjB[4 * lf];
@@ -749,6 +762,7 @@ static int build_sibc(Ctx *c)
             adj(5, i, j - 1, k, -1, wxl, 0, 1, 2, k - 1, k, 3);
           }
         }
+    f = f0 + lf;
   }
   std::stable_sort(hents.begin(), hents.end(), [](const HE &a, const HE &b) { return a.key < b.key; });
   std::sort(zeroKeys.begin(), zeroKeys.end());
@@ -783,8 +797,61 @@ static int build_sibc(Ctx *c)
   c->sibcReal = true;
   for (size_t e = 1; e < q.size(); e += 2)
     if (q[e] != 0.0) c->sibcReal = false;
-  c->d_szeta = dev_alloc(c, (size_t)16 * maxP * NF * (c->sibcReal ? 1 : 2) * sizeof(double));
-  c->d_sold = dev_alloc(c, (size_t)16 * NF * sizeof(double));
+  // FP32 form with real coefficients: the filters fed by one input usually
+  // share their poles (a fitted slab's Z11 and Z12).  A filter's recursive-
+  // convolution coefficients q1, q2 are its residue times a pole-only factor,
+  // so its state is its residue ratio to a base filter of the same input
+  // times the base filter's state: one state per (input, pole) serves every
+  // filter of that input.  The reordered rounding stays far inside the FP32
+  // bar; the FP64 form keeps the reference's per-filter recursion.
+  std::vector<int> fbase(4 * ns, -1);
+  std::vector<double> fratio((size_t)16 * ns * maxP, 0.0);
+  c->sibcFactored = c->prec == VGPU_FP32 && c->sibcReal;
+  for (int st = 0; st < ns && c->sibcFactored; ++st)
+    for (int in = 0; in < 4 && c->sibcFactored; ++in) {
+      // base: the diagonal filter when every q1, q2 of it is non-zero (a
+      // slab's Z11), else the first such filter of the input
+      auto usable = [&](int flt) {
+        const int np = npl[16 * st + flt];
+        if (np == 0) return false;
+        for (int kp = 0; kp < np; ++kp) {
+          const double *qf = &q[(((size_t)16 * st + flt) * maxP + kp) * 6];
+          if (qf[2] == 0.0 || qf[4] == 0.0) return false;
+        }
+        return true;
+      };
+      if (usable(5 * in)) fbase[4 * st + in] = 5 * in;
+      for (int p = 0; p < 4 && fbase[4 * st + in] < 0; ++p)
+        if (usable(4 * p + in)) fbase[4 * st + in] = 4 * p + in;
+      for (int p = 0; p < 4; ++p) {
+        const int flt = 4 * p + in, np = npl[16 * st + flt];
+        if (np == 0) continue;
+        const double *qf = &q[((size_t)16 * st + flt) * maxP * 6];
+        if (fbase[4 * st + in] < 0) { c->sibcFactored = false; break; }
+        const int b = fbase[4 * st + in];
+        const double *qb = &q[((size_t)16 * st + b) * maxP * 6];
+        if (np != npl[16 * st + b]) { c->sibcFactored = false; break; }
+        for (int kp = 0; kp < np && c->sibcFactored; ++kp) {
+          const double q0 = qf[6 * kp], q1 = qf[6 * kp + 2], q2 = qf[6 * kp + 4];
+          const double b0 = qb[6 * kp], b1 = qb[6 * kp + 2], b2 = qb[6 * kp + 4];
+          if (q0 != b0 || b1 == 0.0 || b2 == 0.0) { c->sibcFactored = false; break; }
+          const double r1 = q1 / b1, r2 = q2 / b2;
+          if (std::fabs(r1 - r2) > 1e-12 * std::max(std::max(std::fabs(r1), std::fabs(r2)), 1e-300)) {
+            c->sibcFactored = false;
+            break;
+          }
+          fratio[((size_t)16 * st + flt) * maxP + kp] = r1;
+        }
+      }
+    }
+  const size_t states = (size_t)(c->sibcFactored ? 4 : 16) * maxP * NF * (c->sibcReal ? 1 : 2);
+  c->d_szeta = dev_alloc(c, states * sizeof(double));
+  c->d_sold = dev_alloc(c, (size_t)4 * NF * sizeof(double));
+  c->d_sfbase = upload_vec(c, fbase);
+  c->d_sfratio = upload_vec(c, fratio);
+  if (f != NF) return set_error(c, VGPU_EINVAL, "SIBC face count");
+  c->d_sblocks = upload_vec(c, blocks);
+  c->sibcBlocks = (int)blocks.size();
   c->d_setan = dev_alloc(c, (size_t)4 * NF * c->esz);
   c->d_shkey = upload_vec(c, rkey);
   c->d_shstart = upload_vec(c, rstart);
@@ -1196,6 +1263,7 @@ int vgpu_get_stats(const vgpu_ctx *c, vgpu_stats *out)
   out->debye_fused_sites = c->debyeFused ? (uint64_t)c->debyeL0 : 0;
   out->indexed_media = c->indexed ? 1 : 0;
   out->debye_fused = c->debyeFused ? 1 : 0;
+  out->sibc_state = c->sibcFactored ? 2 : c->sibcReal ? 1 : 0;
   return VGPU_OK;
 }
 

@@ -153,9 +153,17 @@ template <typename T> struct DebyeArgs {
   int cplx;
 };
 
+// A block of the SIBC E launch: up to 128 faces of one surface, so the
+// surface, its filters and its matrices are uniform across the block.
+struct SibcBlock {
+  int surf, pad_;
+  long long f0, f1;      // first face of the block, end of the surface's faces
+};
+
 // SIBC faces (sibc.c:443-586): per-face geometry, per-surface constants.
 template <typename T> struct SibcEArgs {
   long long nfaces;
+  const SibcBlock *blocks;
   const int *ijk;        // [face][3]
   const int *surf;       // [face]
   const unsigned char *adjA, *adjB;  // [face][4]
@@ -167,8 +175,11 @@ template <typename T> struct SibcEArgs {
   const double *q;       // [set][16][maxPoles][3][2]
   int maxPoles;
   int real;              // all coefficients real: zeta holds real states only
+  int factored;          // one state per (input, pole): zeta [(input*maxPoles + pole)][face]
+  const int *fbase;      // factored: [set][input] filter whose coefficients drive the input's states, or -1
+  const double *fratio;  // factored: [set][16][maxPoles] each filter's residue over its base filter's
   double *zeta;          // [(filter*maxPoles + pole)][face][2], or [..][face] when real
-  double *old;           // [filter][face]
+  double *old;           // [input][face]: the previous step's filter input
   T *Etan;               // [face][4]
   T *E[3];
   const T *H[3];
@@ -424,6 +435,9 @@ struct Ctx {
   long long sibcFaces = 0;
   int sibcMaxPoles = 1;
   bool sibcReal = false;                     // all filter coefficients real: real filter state
+  bool sibcFactored = false;                 // FP32 form: one state per (input, pole)
+  void *d_sfbase = nullptr, *d_sfratio = nullptr, *d_sblocks = nullptr;
+  int sibcBlocks = 0;
   void *d_sijk = nullptr, *d_ssurf = nullptr, *d_sadjA = nullptr, *d_sadjB = nullptr, *d_snormal = nullptr;
   void *d_sA = nullptr, *d_sB = nullptr, *d_sset = nullptr, *d_snp = nullptr, *d_sasymp = nullptr, *d_sq = nullptr;
   void *d_szeta = nullptr, *d_sold = nullptr, *d_setan = nullptr;

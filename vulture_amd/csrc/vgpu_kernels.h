@@ -566,12 +566,13 @@ __global__ void k_debye_gather(long long n, const long long *key, const T *const
 // SIBC: E pass per face (sibc.c:443-586) with FP64 recursive convolution
 // (filter.c:309-327).
 // ---------------------------------------------------------------------------
-template <typename T> __global__ void k_sibc_e(SibcEArgs<T> a)
+template <typename T> __global__ __launch_bounds__(128) void k_sibc_e(SibcEArgs<T> a)
 {
-  const long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (f >= a.nfaces) return;
+  const SibcBlock w = a.blocks[blockIdx.x];  // one surface per block: s is uniform
+  const long long f = w.f0 + threadIdx.x;
+  if (f >= w.f1) return;
   const int i = a.ijk[3 * f], j = a.ijk[3 * f + 1], k = a.ijk[3 * f + 2];
-  const int s = a.surf[f];
+  const int s = w.surf;
   const int nrm = a.normal[s];
   const long long SJ = a.PX, P = a.plane;
   const long long id = ((long long)k * a.NY + j) * SJ + i;
@@ -606,6 +607,53 @@ template <typename T> __global__ void k_sibc_e(SibcEArgs<T> a)
   }
   const int set = a.filtset[s];
   const long long NF = a.nfaces;
+  // every filter fed by input q saw the same input last step: one stored
+  // previous input per (input, face), read before and written after all
+  double xo4[4];
+  bool fed[4] = {false, false, false, false};
+  for (int q = 0; q < 4; ++q) {
+    xo4[q] = a.old[q * NF + f];
+    for (int p = 0; p < 4; ++p) fed[q] = fed[q] || a.npoles[16 * set + 4 * p + q] > 0;
+  }
+  if (a.factored) {
+    // one state per (input, pole), scaled by each filter's residue ratio
+    double y16[16];
+    for (int flt = 0; flt < 16; ++flt) y16[flt] = a.asymp[16 * set + flt] * (double)Hin[flt & 3];
+    for (int q = 0; q < 4; ++q) {
+      const int b = a.fbase[4 * set + q];
+      if (b < 0) continue;
+      const double x = (double)Hin[q], xo = xo4[q];
+      const int np = a.npoles[16 * set + b];
+      const double *qq = a.q + ((long long)(16 * set + b) * a.maxPoles) * 6;
+      VG_GLOBAL double *zq = gp(a.zeta) + (long long)q * a.maxPoles * NF + f;
+      // states in groups of eight: all loads of a group issued before its math
+      for (int k0 = 0; k0 < np; k0 += 8) {
+        double zs[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (k0 + u < np) zs[u] = zq[(long long)(k0 + u) * NF];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int kp = k0 + u;
+          if (kp >= np) break;
+          const double xi = (qq[6 * kp] * zs[u] + qq[6 * kp + 2] * xo) + qq[6 * kp + 4] * x;
+          zq[(long long)kp * NF] = xi;
+          for (int p = 0; p < 4; ++p) {
+            const int flt = 4 * p + q;
+            if (a.npoles[16 * set + flt] > 0) y16[flt] += a.fratio[((long long)(16 * set + flt)) * a.maxPoles + kp] * xi;
+          }
+        }
+      }
+    }
+    for (int p = 0; p < 4; ++p) {
+      Eout[p] = T(0);
+      for (int q = 0; q < 4; ++q) {
+        const int flt = 4 * p + q;
+        if (a.npoles[16 * set + flt] == 0 && a.asymp[16 * set + flt] == 0.0) continue;
+        Eout[p] = T((double)Eout[p] + y16[flt]);
+      }
+    }
+  } else
   for (int p = 0; p < 4; ++p) {
     Eout[p] = T(0);
     for (int q = 0; q < 4; ++q) {
@@ -616,17 +664,25 @@ template <typename T> __global__ void k_sibc_e(SibcEArgs<T> a)
       const double x = (double)Hin[q];
       double y = asy * x;
       if (np > 0) {
-        double *old = &a.old[flt * NF + f];
-        const double xo = *old;
+        const double xo = xo4[q];
         const double *qq = a.q + ((long long)(16 * set + flt) * a.maxPoles) * 6;
         if (a.real) {
           // real coefficients: the complex step below with zero imaginary
           // parts, which leave the same real part bit for bit
-          for (int kp = 0; kp < np; ++kp) {
-            double *z = &a.zeta[((long long)flt * a.maxPoles + kp) * NF + f];
-            const double nr = (qq[6 * kp] * *z + qq[6 * kp + 2] * xo) + qq[6 * kp + 4] * x;
-            *z = nr;
-            y = y + nr;
+          VG_GLOBAL double *zf = gp(a.zeta) + (long long)flt * a.maxPoles * NF + f;
+          for (int k0 = 0; k0 < np; k0 += 8) {
+            double zs[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+              if (k0 + u < np) zs[u] = zf[(long long)(k0 + u) * NF];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+              const int kp = k0 + u;
+              if (kp >= np) break;
+              const double nr = (qq[6 * kp] * zs[u] + qq[6 * kp + 2] * xo) + qq[6 * kp + 4] * x;
+              zf[(long long)kp * NF] = nr;
+              y = y + nr;
+            }
           }
         } else
         for (int kp = 0; kp < np; ++kp) {
@@ -640,11 +696,12 @@ template <typename T> __global__ void k_sibc_e(SibcEArgs<T> a)
           z[1] = ni;
           y = y + nr;
         }
-        *old = x;
       }
       Eout[p] = T((double)Eout[p] + y);
     }
   }
+  for (int q = 0; q < 4; ++q)
+    if (fed[q]) a.old[q * NF + f] = (double)Hin[q];
   T *Et = a.Etan + 4 * f;
   for (int p = 0; p < 4; ++p) {
     T v = T(0);

@@ -479,11 +479,14 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
   SibcHArgs<T> sh{};
   if (c->sibcFaces) {
     se.nfaces = c->sibcFaces; se.ijk = P<int>(c->d_sijk); se.surf = P<int>(c->d_ssurf);
+    se.blocks = P<SibcBlock>(c->d_sblocks);
     se.adjA = P<unsigned char>(c->d_sadjA); se.adjB = P<unsigned char>(c->d_sadjB);
     se.normal = P<int>(c->d_snormal); se.A = P<double>(c->d_sA); se.B = P<double>(c->d_sB);
     se.filtset = P<int>(c->d_sset); se.npoles = P<int>(c->d_snp); se.asymp = P<double>(c->d_sasymp);
     se.q = P<double>(c->d_sq); se.maxPoles = c->sibcMaxPoles; se.zeta = P<double>(c->d_szeta);
     se.real = c->sibcReal ? 1 : 0;
+    se.factored = c->sibcFactored ? 1 : 0;
+    se.fbase = P<int>(c->d_sfbase); se.fratio = P<double>(c->d_sfratio);
     se.old = P<double>(c->d_sold); se.Etan = P<T>(c->d_setan);
     for (int d = 0; d < 3; ++d) { se.E[d] = P<T>(c->F[d]); se.H[d] = P<T>(c->F[3 + d]); }
     se.NY = c->NY; se.PX = c->PX; se.plane = c->plane;
@@ -595,7 +598,7 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
       hipLaunchKernelGGL(k_debye<T>, dim3((unsigned)((da.n + 255) / 256)), dim3(256), 0, s, da);
     }
     if (c->sibcFaces)
-      hipLaunchKernelGGL(k_sibc_e<T>, dim3((unsigned)((se.nfaces + 127) / 128)), dim3(128), 0, s, se);
+      hipLaunchKernelGGL(k_sibc_e<T>, dim3((unsigned)c->sibcBlocks), dim3(128), 0, s, se);
     if (anyAux) {
       xa.soff = soff;
       hipLaunchKernelGGL(k_aux_e<T>, dim3(xa.npw), dim3(256), 0, s, xa, pi);
