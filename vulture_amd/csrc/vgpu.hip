@@ -615,8 +615,14 @@ static int build_class_tables(Ctx *c)
     for (size_t q = 0; q < c->ncell; ++q) { e[q] = (uint16_t)c->ecls[q]; h[q] = (uint16_t)c->hcls[q]; }
     c->d_ecls = upload_vec(c, e);
     c->d_hcls = upload_vec(c, h);
-    c->d_etab = upload_reals(c, c->etab);
-    c->d_htab = upload_reals(c, c->htab);
+    // padded to ETAB_STRIDE / HTAB_STRIDE entries per class (vector loads)
+    std::vector<double> e8((size_t)c->nEcls * ETAB_STRIDE, 0.0), h4((size_t)c->nHcls * HTAB_STRIDE, 0.0);
+    for (int k = 0; k < c->nEcls; ++k)
+      for (int q = 0; q < 6; ++q) e8[(size_t)k * ETAB_STRIDE + q] = c->etab[(size_t)k * 6 + q];
+    for (int k = 0; k < c->nHcls; ++k)
+      for (int q = 0; q < 3; ++q) h4[(size_t)k * HTAB_STRIDE + q] = c->htab[(size_t)k * 3 + q];
+    c->d_etab = upload_reals(c, e8);
+    c->d_htab = upload_reals(c, h4);
     c->d_edslot = upload_vec(c, c->edslot);
     if (!c->d_ecls || !c->d_hcls || !c->d_etab || !c->d_htab || !c->d_edslot)
       return set_error(c, VGPU_ENOMEM, "coefficient class allocation");

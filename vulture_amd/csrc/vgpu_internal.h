@@ -31,7 +31,7 @@ template <typename T> struct EArgs {
   int pl[6][3][6];       // UPML slab limits [region][component][face]
   T *PP[6][3], *P[6][3]; // UPML auxiliaries, x-fastest within each slab box
   const T *ad[3], *bd[3], *ah[3], *ibh[3];
-  // coefficient classes: per-cell class, [class][6] alpha/beta, [class][3] Debye medium
+  // coefficient classes: per-cell class, [class][8] alpha/beta (6 used), [class][3] Debye medium
   const uint16_t *ecls;
   const T *etab;
   const int *edslot;
@@ -57,7 +57,7 @@ template <typename T> struct HArgs {
   int pl[6][3][6];
   T *B[6][3];
   const T *ad[3], *ah[3], *bh[3], *ibd[3];
-  const uint16_t *hcls;  // coefficient classes: per-cell class, [class][3] gamma
+  const uint16_t *hcls;  // coefficient classes: per-cell class, [class][4] gamma (3 used)
   const T *htab;
 };
 
@@ -387,7 +387,7 @@ struct Ctx {
   // E component in vgpu_finalize, uploaded as uint16 per cell when both
   // tables fit.
   std::vector<uint32_t> ecls, hcls;          // host, per device cell
-  std::vector<double> etab, htab;            // [class][6] alpha/beta, [class][3] gamma
+  std::vector<double> etab, htab;            // host: [class][6] alpha/beta, [class][3] gamma (device: padded to 8, 4)
   std::vector<int> edslot;                   // [class][3] layer-0 Debye medium or -1
   bool indexed = false;
   void *d_ecls = nullptr, *d_hcls = nullptr, *d_etab = nullptr, *d_htab = nullptr, *d_edslot = nullptr;

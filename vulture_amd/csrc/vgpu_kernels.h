@@ -19,6 +19,38 @@ namespace vgpu {
 // so turns them into global loads, whose waits stay partial.
 template <typename T> VG_DEV VG_GLOBAL T *gp(T *p) { return (VG_GLOBAL T *)p; }
 
+// Coefficient class tables on the device are padded to 8 (E: alpha, beta
+// per component) and 4 (H: gamma per component) entries per class, so a
+// component's (alpha, beta) pair is one aligned 2-vector load and a class's
+// three gammas one 4-vector load (two 2-vectors in FP64).
+constexpr int ETAB_STRIDE = 8, HTAB_STRIDE = 4;
+typedef float vf2 __attribute__((ext_vector_type(2)));
+typedef float vf4 __attribute__((ext_vector_type(4)));
+typedef double vd2 __attribute__((ext_vector_type(2)));
+template <typename T> struct Vec2 { using type = vd2; };
+template <> struct Vec2<float> { using type = vf2; };
+
+template <typename T> VG_DEV void load_pair(const T *p, T &a, T &b)
+{
+  const typename Vec2<T>::type v = *reinterpret_cast<const VG_GLOBAL typename Vec2<T>::type *>(gp(p));
+  a = v.x;
+  b = v.y;
+}
+
+template <typename T> VG_DEV void load_gammas(const T *p, T &g0, T &g1, T &g2)
+{
+  T u;
+  load_pair(p, g0, g1);
+  load_pair(p + 2, g2, u);
+}
+template <> VG_DEV void load_gammas<float>(const float *p, float &g0, float &g1, float &g2)
+{
+  const vf4 v = *reinterpret_cast<const VG_GLOBAL vf4 *>(gp(p));
+  g0 = v.x;
+  g1 = v.y;
+  g2 = v.z;
+}
+
 VG_HD bool in_box(const int *b, int i, int j, int k)
 {
   return i >= b[0] && i <= b[1] && j >= b[2] && j <= b[3] && k >= b[4] && k <= b[5];
@@ -136,21 +168,23 @@ VG_DEV void debye_fused(const FastE<T> &f, int bx, int j, int k, bool vx, unsign
   e = debye_site<T>(e, be, g, ds, a.dlast, a.dJ, a.dstride, a.dalpha, a.dbeta, a.dnp, a.dmaxP, a.dcplx);
 }
 
+// One E component of a fast-box cell: its coefficients and old value, then
+// the ordinary update (and, with FUSE, the first Debye layer).  All three
+// components are computed before any is stored, so the stores do not hold
+// back the next component's loads.
 template <typename T, bool IDX, bool FUSE, int C>
-VG_DEV void e_fast(const FastE<T> &f, int bx, int j, int k, bool vx, long long id, unsigned cls, T curl)
+VG_DEV T e_fast(const FastE<T> &f, int bx, int j, int k, bool vx, long long id, unsigned cls, T curl)
 {
-  T *__restrict__ E = f.E[C];
   T al, be;
   if (IDX) {
-    al = f.etab[cls * 6 + 2 * C];
-    be = f.etab[cls * 6 + 2 * C + 1];
+    load_pair(f.etab + cls * ETAB_STRIDE + 2 * C, al, be);
   } else {
     al = f.alpha[C][id];
     be = f.beta[C][id];
   }
-  T e = al * E[id] + be * curl;
+  T e = al * f.E[C][id] + be * curl;
   if (FUSE) debye_fused<T, C>(f, bx, j, k, vx, cls, be, e);
-  if (vx) E[id] = e;
+  return e;
 }
 
 // Fast E kernel: inner grid (grid.c:709-797) and, with FUSE, the first Debye
@@ -182,9 +216,14 @@ template <typename T, bool IDX, bool FUSE> __global__ __launch_bounds__(512) voi
     const T hzy = Hz[id - SJ], hzx = Hz[id - 1], hyx = Hy[id - 1], hxy = Hx[id - SJ];
     const unsigned cls = IDX ? (unsigned)f.ecls[id] : 0u;
     const T idhz = f.idh[2][k];
-    e_fast<T, IDX, FUSE, 0>(f, bx, j, k, vx, id, cls, idhy * (hz - hzy) + idhz * (hym - hy));
-    e_fast<T, IDX, FUSE, 1>(f, bx, j, k, vx, id, cls, idhz * (hx - hxm) + idhx * (hzx - hz));
-    e_fast<T, IDX, FUSE, 2>(f, bx, j, k, vx, id, cls, idhx * (hy - hyx) + idhy * (hxy - hx));
+    const T e0 = e_fast<T, IDX, FUSE, 0>(f, bx, j, k, vx, id, cls, idhy * (hz - hzy) + idhz * (hym - hy));
+    const T e1 = e_fast<T, IDX, FUSE, 1>(f, bx, j, k, vx, id, cls, idhz * (hx - hxm) + idhx * (hzx - hz));
+    const T e2 = e_fast<T, IDX, FUSE, 2>(f, bx, j, k, vx, id, cls, idhx * (hy - hyx) + idhy * (hxy - hx));
+    if (vx) {
+      f.E[0][id] = e0;
+      f.E[1][id] = e1;
+      f.E[2][id] = e2;
+    }
     hxm = hx;
     hym = hy;
   }
@@ -220,9 +259,14 @@ template <typename T, bool IDX> __global__ __launch_bounds__(512) void k_h_fast(
     const T ez = Ez[id], ezy = Ez[id + SJ], ezx = Ez[id + 1], exy = Ex[id + SJ], eyx = Ey[id + 1];
     const unsigned cls = IDX ? (unsigned)f.hcls[id] : 0u;
     const T idez = f.ide[2][k];
-    const T g0 = IDX ? f.htab[cls * 3] : f.gamma[0][id];
-    const T g1 = IDX ? f.htab[cls * 3 + 1] : f.gamma[1][id];
-    const T g2 = IDX ? f.htab[cls * 3 + 2] : f.gamma[2][id];
+    T g0, g1, g2;
+    if (IDX) {
+      load_gammas(f.htab + cls * HTAB_STRIDE, g0, g1, g2);
+    } else {
+      g0 = f.gamma[0][id];
+      g1 = f.gamma[1][id];
+      g2 = f.gamma[2][id];
+    }
     const T h0 = H0[id] + g0 * (idez * (eyp1 - ey) + idey * (ez - ezy));
     const T h1 = H1[id] + g1 * (idex * (ezx - ez) + idez * (ex - exp1));
     const T h2 = H2[id] + g2 * (idey * (exy - ex) + idex * (ey - eyx));
@@ -318,8 +362,7 @@ template <typename T, bool IDX, int C> struct EShellCell {
     if (rg.reg < 0) return;
     e = g.E[C][id];
     if (IDX) {
-      al = g.etab[cls * 6 + 2 * C];
-      be = g.etab[cls * 6 + 2 * C + 1];
+      load_pair((const T *)(g.etab + cls * ETAB_STRIDE + 2 * C), al, be);
     } else {
       al = g.alpha[C][id];
       be = g.beta[C][id];
@@ -427,7 +470,7 @@ template <typename T, bool IDX, int C> struct HShellCell {
   {
     if (rg.reg < 0) return;
     h = q.H[C][id];
-    g = IDX ? q.htab[cls * 3 + C] : q.gamma[C][id];
+    g = IDX ? q.htab[cls * HTAB_STRIDE + C] : q.gamma[C][id];
     if (rg.reg < 6) {
       constexpr int A1 = (C + 1) % 3, A2 = (C + 2) % 3;
       ob = gp(static_cast<const T *>(rg.a))[rg.p];
@@ -559,7 +602,7 @@ __global__ void k_debye_gather(long long n, const long long *key, const T *const
   if (g >= n) return;
   const int c = (int)(key[g] & 3);
   const long long id = (key[g] >> 2) & ((1LL << 40) - 1);
-  sbeta[g] = etab ? etab[(unsigned)ecls[id] * 6 + 2 * c + 1] : beta[c][id];
+  sbeta[g] = etab ? etab[(unsigned)ecls[id] * ETAB_STRIDE + 2 * c + 1] : beta[c][id];
 }
 
 // ---------------------------------------------------------------------------
