@@ -23,8 +23,10 @@ step with the compulsory bytes of one pass in the same representation
 (step_bytes: 12 field words and two 2-byte classes per cell, 18 more words
 per UPML cell, the Debye state), over ms_per_step.
 
-`cpu_baseline` times the CPU restatement of the same loop (-O3 timing build,
-OpenMP) on a bounded number of steps of the same mesh on this host.
+`cpu_baseline` times the reference itself (its float build from its own
+sources with its OpenMP option, built by oracle/Makefile) on a bounded number
+of steps of the same mesh on this host, when that build is present; the CPU
+restatement of the loop is timed beside it (`cpu_port`).
 """
 import argparse
 import json
@@ -139,14 +141,44 @@ def sibc_box(work, steps, warmup=5):
             "setup_s": round(setup_s, 2)}
 
 
-def cpu_baseline(mesh, steps):
+def cpu_baseline(mesh, steps, cells):
+    """The reference itself (its float build from its own sources, -O3 with
+    its OpenMP option, oracle/_ref/vulture_ref_omp) on `steps` steps of the
+    same mesh: wall time of the full run minus wall time of its set-up-only
+    run (-p).  The CPU restatement of the loop is timed beside it (cpu_port).
+    Both on up to 16 host threads."""
+    import re
+    import subprocess
     import oracle
     threads = min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
-    sec, cells, n = oracle.time_mesh(mesh, steps=steps, warmup=2, threads=threads, precision="fp32",
-                                     outdir=os.path.join(os.path.dirname(mesh), "cpu"))
-    return {"value": round(cells * n / sec / 1e6, 2), "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
+    sec, pcells, n = oracle.time_mesh(mesh, steps=steps, warmup=2, threads=threads, precision="fp32",
+                                      outdir=os.path.join(os.path.dirname(mesh), "cpu"))
+    port = {"value": round(pcells * n / sec / 1e6, 2), "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
             "sample": f"{n} timed steps (after 2 untimed) of the same 272^3 Debye mesh, FP32 CPU restatement "
                       f"(-O3, OpenMP, {threads} threads): {sec:.2f} s"}
+    ref = os.path.join(HERE, "oracle", "_ref", "vulture_ref_omp")
+    if not os.path.exists(ref):
+        return port, None
+    d = os.path.join(os.path.dirname(mesh), "ref")
+    os.makedirs(d, exist_ok=True)
+    text = re.sub(r"^NT \d+$", f"NT {steps}", open(mesh).read(), flags=re.M)
+    rmesh = os.path.join(d, "ref.mesh")
+    with open(rmesh, "w") as f:
+        f.write(text)
+    walls = []
+    for opt in (["-p"], []):
+        t0 = time.monotonic()
+        r = subprocess.run([ref, "-n", str(threads)] + opt + [rmesh], cwd=d, capture_output=True, text=True,
+                           timeout=600)
+        walls.append(time.monotonic() - t0)
+        if r.returncode != 0:
+            return port, None
+    loop = walls[1] - walls[0]
+    return {"value": round(cells * steps / loop / 1e6, 2), "unit": "Mcell-updates/s", "cores": threads,
+            "kind": "reference",
+            "sample": f"the reference's own float build (-O3, its OpenMP option, {threads} threads) on {steps} "
+                      f"steps of the same 272^3 Debye mesh: full run {walls[1]:.2f} s minus set-up-only run "
+                      f"(-p) {walls[0]:.2f} s = {loop:.2f} s"}, port
 
 
 def main():
@@ -199,7 +231,12 @@ def main():
         out["fp64"] = {"value": round(f64["value"], 1), "ms_per_step": round(f64["ms_per_step"], 4),
                        "roofline": f64["roofline"], "step_roofline": f64["step_roofline"]}
         out["fp32_over_fp64"] = round(res["fp32"]["value"] / f64["value"], 3)
-    out["cpu_baseline"] = cpu_baseline(mesh, args.cpu_steps) if args.cpu_steps > 0 else None
+    out["cpu_baseline"] = None
+    if args.cpu_steps > 0:
+        base, port = cpu_baseline(mesh, args.cpu_steps, st["cells"])
+        out["cpu_baseline"] = base
+        if port is not None:
+            out["cpu_port"] = port
     if args.sibc_steps > 0:
         out["sibc_box"] = sibc_box(work, args.sibc_steps)
         print(f"# sibc_box: {out['sibc_box']['value']:.1f} Mcell-updates/s, {out['sibc_box']['ms_per_step']:.3f} "

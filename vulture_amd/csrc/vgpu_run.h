@@ -242,7 +242,8 @@ static int build_shell(const A &a, const int (*in)[NB][6], const int *fb, const 
   long long rawCells = 0;
   for (const ShellPiece &pc : raw)
     rawCells += (long long)(pc.box[1] - pc.box[0] + 1) * (pc.box[3] - pc.box[2] + 1) * (pc.box[5] - pc.box[4] + 1);
-  const int kcmax = (int)std::max(1LL, std::min((long long)SHELL_KC, rawCells / (256LL * 2048)));
+  int kcmax = (int)std::max(1LL, std::min((long long)SHELL_KC, rawCells / (256LL * 2048)));
+  if (const char *ev = getenv("VGPU_SHELL_KC")) kcmax = std::max(1, std::min(atoi(ev), 64));  // tuning override
   cells = 0;
   pcs.clear();
   for (ShellPiece pc : raw) {
