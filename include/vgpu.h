@@ -196,6 +196,27 @@ int vgpu_get_stats(const vgpu_ctx *c, vgpu_stats *out);
  * and the average duration of the E and H volume kernels in it (ms). */
 int vgpu_last_run_timing(const vgpu_ctx *c, double *total_ms, double *volume_e_ms, double *volume_h_ms);
 
+/* Update-once check of the field sweeps.  Replaces the reference's
+ * limit-checking build (CMakeLists.txt:40,89-95: fields start at -1, each
+ * update asserts the old value and marks the cell, grid.h:181-191, and
+ * checkGrid, grid.c:951-1025, reports fields of the outer limits left
+ * unmarked after the run).  Here one step of the field sweeps (Mur E, the E
+ * sweep over inner grid and UPML, the H sweep over inner grid, Mur normal H
+ * and UPML) runs in a counting form that adds one to a per-cell count for
+ * each update instead of storing it, and each component's counts are
+ * tallied against folim[component][face] (the outer field limits, gfolim of
+ * grid.c:404).  Fields, UPML auxiliaries and Debye state are unchanged.
+ * counts, when not NULL, receives the visits per cell (saturated at 255) as
+ * 6 * nx * ny * nz bytes, index ((component * nz + k) * ny + j) * nx + i.
+ * Synchronizes the device; call it between runs. */
+typedef struct {
+  uint64_t missed[6];          /* cells inside folim that no sweep updated */
+  uint64_t repeated[6];        /* cells updated more than once */
+  uint64_t outside[6];         /* cells outside folim that a sweep updated */
+  int64_t first[6];            /* lowest (k * ny + j) * nx + i of an offending cell, -1 if none */
+} vgpu_limits_report;
+int vgpu_check_limits(vgpu_ctx *c, const int folim[6][6], vgpu_limits_report *out, unsigned char *counts);
+
 const char *vgpu_last_error(const vgpu_ctx *c);
 void vgpu_destroy(vgpu_ctx *c);
 

@@ -41,7 +41,7 @@ def test_libraries_load_without_gpu():
     ctypes.CDLL(os.path.join(LIB, "libvgpu.so"), mode=ctypes.RTLD_GLOBAL)
     host = ctypes.CDLL(os.path.join(LIB, "libvhost.so"))
     for n in ("vh_run_mesh_gpu", "vh_gpu_open", "vh_gpu_close", "vh_gpu_setup", "vh_solve_sizes", "vh_read_mesh",
-              "vh_init"):
+              "vh_init", "vh_solve_field_limits"):
         assert hasattr(host, n), n
 
 

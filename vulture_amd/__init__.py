@@ -62,6 +62,8 @@ def load_host():
         _host.vh_solve_sizes.restype = None
         _host.vh_solve_shape.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)]
         _host.vh_solve_shape.restype = None
+        _host.vh_solve_field_limits.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+        _host.vh_solve_field_limits.restype = None
         v = _vgpu
         v.vgpu_run.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_void_p]
         v.vgpu_run.restype = ctypes.c_int
@@ -77,6 +79,9 @@ def load_host():
         v.vgpu_get_stats.restype = ctypes.c_int
         v.vgpu_download_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
         v.vgpu_download_field.restype = ctypes.c_int
+        v.vgpu_check_limits.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(LimitsReport),
+                                        ctypes.c_void_p]
+        v.vgpu_check_limits.restype = ctypes.c_int
     return _host
 
 
@@ -109,6 +114,12 @@ class Stats(ctypes.Structure):
                 ("fast_cells", ctypes.c_uint64 * 2), ("debye_fused_sites", ctypes.c_uint64),
                 ("indexed_media", ctypes.c_int), ("debye_fused", ctypes.c_int), ("sibc_state", ctypes.c_int),
                 ("pad_", ctypes.c_int)]
+
+
+class LimitsReport(ctypes.Structure):
+    """vgpu_limits_report (include/vgpu.h)."""
+    _fields_ = [("missed", ctypes.c_uint64 * 6), ("repeated", ctypes.c_uint64 * 6),
+                ("outside", ctypes.c_uint64 * 6), ("first", ctypes.c_int64 * 6)]
 
 
 class DeviceSolve:
@@ -179,6 +190,31 @@ class DeviceSolve:
         n = self.grid_shape()
         out = np.empty(n, dtype=np.float64)
         self._check(self._v.vgpu_download_field(self._c, component, out.ctypes.data))
+        return out
+
+    def field_limits(self):
+        """Outer field limits gfolim[component][face] of the solve (6x6 ints)."""
+        lim = (ctypes.c_int * 36)()
+        load_host().vh_solve_field_limits(self._s, lim)
+        return [list(lim[6 * c:6 * c + 6]) for c in range(6)]
+
+    def check_limits(self, counts=True):
+        """Update-once check of the field sweeps (vgpu_check_limits): one step
+        of Mur E, the E sweep and the H sweep counts its updates per cell,
+        tallied against the outer field limits; no field changes.  Returns a
+        dict of per-component lists (missed, repeated, outside, first) and,
+        with counts, the visits per cell as uint8 [6][nx][ny][nz]."""
+        import numpy as np
+        lim = (ctypes.c_int * 36)()
+        load_host().vh_solve_field_limits(self._s, lim)
+        rep = LimitsReport()
+        nx, ny, nz = self.grid_shape()
+        buf = np.zeros(6 * nx * ny * nz, dtype=np.uint8) if counts else None
+        self._check(self._v.vgpu_check_limits(self._c, lim, ctypes.byref(rep),
+                                              buf.ctypes.data if counts else None))
+        out = {k: [int(x) for x in getattr(rep, k)] for k, _ in LimitsReport._fields_}
+        if counts:
+            return out, buf.reshape(6, nz, ny, nx).transpose(0, 3, 2, 1)
         return out
 
     def grid_shape(self):

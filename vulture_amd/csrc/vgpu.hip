@@ -1245,6 +1245,20 @@ int vgpu_download_field(vgpu_ctx *c, int component, double *host)
   return VGPU_OK;
 }
 
+int vgpu_check_limits(vgpu_ctx *c, const int folim[6][6], vgpu_limits_report *out, unsigned char *counts)
+{
+  if (!c || !folim || !out) return VGPU_EINVAL;
+  if (!c->finalized) return VGPU_ESTATE;
+  const int N[3] = {c->NX, c->NY, c->NZ};
+  for (int C = 0; C < 6; ++C)
+    for (int a = 0; a < 3; ++a)
+      if (folim[C][2 * a] < 0 || folim[C][2 * a + 1] >= N[a])
+        return set_error(c, VGPU_EINVAL, "field limits outside the grid");
+  hipSetDevice(c->device);
+  (void)hipGetLastError();
+  return c->prec == VGPU_FP64 ? check_limits<double>(c, folim, out, counts) : check_limits<float>(c, folim, out, counts);
+}
+
 size_t vgpu_footprint_bytes(const vgpu_ctx *c) { return c ? c->bytes : 0; }
 
 int vgpu_get_stats(const vgpu_ctx *c, vgpu_stats *out)

@@ -43,6 +43,7 @@ template <typename T> struct EArgs {
   const double *dalpha, *dbeta;
   const int *dnp;
   int dmaxP, dcplx;
+  unsigned *vis[3];      // update-once check: visit counts per component (null in ordinary runs)
 };
 
 template <typename T> struct HArgs {
@@ -59,6 +60,7 @@ template <typename T> struct HArgs {
   const T *ad[3], *ah[3], *bh[3], *ibd[3];
   const uint16_t *hcls;  // coefficient classes: per-cell class, [class][4] gamma (3 used)
   const T *htab;
+  unsigned *vis[3];      // update-once check (null in ordinary runs)
 };
 
 // Lean by-value arguments of the fast sweep kernels.
@@ -76,6 +78,7 @@ template <typename T> struct FastE {
   const T *alpha[3], *beta[3];  // per-cell arrays when there are no classes
   int dfuse;             // apply layer-0 Debye sites
   const EArgs<T> *ap;    // full argument block in device memory (Debye state)
+  unsigned *vis[3];      // update-once check (null in ordinary runs)
 };
 
 template <typename T> struct FastH {
@@ -90,6 +93,7 @@ template <typename T> struct FastH {
   const uint16_t *hcls;
   const T *htab;
   const T *gamma[3];
+  unsigned *vis[3];      // update-once check (null in ordinary runs)
 };
 
 // A piece of a sweep's shell (the cells outside its fast box): a box whose
@@ -134,6 +138,7 @@ template <typename T> struct MurArgs {
   int box[12][6];        // limits of face f component slot s at [2f+s]
   int comp[12];          // component of each slot (-1 unused)
   T zeta[6];
+  unsigned *vis[3];      // update-once check (null in ordinary runs)
 };
 
 // Debye sites of one layer (debye.c:243-307): a layer holds at most one site
@@ -483,11 +488,23 @@ struct Ctx {
 int set_error(Ctx *c, int code, const char *fmt, ...);
 void *dev_alloc(Ctx *c, size_t bytes);
 
+// Update-once check: visit-count arrays per component (device layout) and
+// sweep argument blocks in device memory that carry them.
+struct VisitRun {
+  unsigned *vis[6];
+  void *eargs, *hargs;
+};
+
 // Per-precision step driver, instantiated for double and float.
 // Enqueues nsteps steps that start at the device step counter, then advances
-// it; with timeKernels the volume sweeps of the first steps are timed.
-template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool timeKernels);
+// it; with timeKernels the volume sweeps of the first steps are timed.  With
+// chk it enqueues one step of the field sweeps only, counting updates.
+template <typename T>
+int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool timeKernels, const VisitRun *chk = nullptr);
 // Uploads the sweeps' argument blocks (once, at finalize).
 template <typename T> int prepare_sweeps(Ctx *c);
+// Update-once check of the field sweeps against the outer field limits.
+template <typename T>
+int check_limits(Ctx *c, const int folim[6][6], vgpu_limits_report *rep, unsigned char *counts);
 
 }  // namespace vgpu

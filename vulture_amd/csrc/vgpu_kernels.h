@@ -187,9 +187,15 @@ VG_DEV T e_fast(const FastE<T> &f, int bx, int j, int k, bool vx, long long id, 
   return e;
 }
 
+// Update-once check (the reference's CHECK_LIMITS build, grid.h:181-191):
+// a sweep kernel instantiated with CHK counts its updates of each field cell
+// instead of storing them, and leaves every field and auxiliary untouched.
+VG_DEV void visit(unsigned *v, long long id) { atomicAdd(v + id, 1u); }
+
 // Fast E kernel: inner grid (grid.c:709-797) and, with FUSE, the first Debye
 // layer.
-template <typename T, bool IDX, bool FUSE> __global__ __launch_bounds__(512) void k_e_fast(FastE<T> f)
+template <typename T, bool IDX, bool FUSE, bool CHK = false>
+__global__ __launch_bounds__(512) void k_e_fast(FastE<T> f)
 {
   const unsigned L = xcd_block();
   if (L >= (unsigned)(f.nchx * f.gy * f.gz)) return;
@@ -220,9 +226,15 @@ template <typename T, bool IDX, bool FUSE> __global__ __launch_bounds__(512) voi
     const T e1 = e_fast<T, IDX, FUSE, 1>(f, bx, j, k, vx, id, cls, idhz * (hx - hxm) + idhx * (hzx - hz));
     const T e2 = e_fast<T, IDX, FUSE, 2>(f, bx, j, k, vx, id, cls, idhx * (hy - hyx) + idhy * (hxy - hx));
     if (vx) {
-      f.E[0][id] = e0;
-      f.E[1][id] = e1;
-      f.E[2][id] = e2;
+      if (CHK) {
+        visit(f.vis[0], id);
+        visit(f.vis[1], id);
+        visit(f.vis[2], id);
+      } else {
+        f.E[0][id] = e0;
+        f.E[1][id] = e1;
+        f.E[2][id] = e2;
+      }
     }
     hxm = hx;
     hym = hy;
@@ -230,7 +242,7 @@ template <typename T, bool IDX, bool FUSE> __global__ __launch_bounds__(512) voi
 }
 
 // Fast H kernel: inner grid (grid.c:800-888).
-template <typename T, bool IDX> __global__ __launch_bounds__(512) void k_h_fast(FastH<T> f)
+template <typename T, bool IDX, bool CHK = false> __global__ __launch_bounds__(512) void k_h_fast(FastH<T> f)
 {
   const unsigned L = xcd_block();
   if (L >= (unsigned)(f.nchx * f.gy * f.gz)) return;
@@ -271,9 +283,15 @@ template <typename T, bool IDX> __global__ __launch_bounds__(512) void k_h_fast(
     const T h1 = H1[id] + g1 * (idex * (ezx - ez) + idez * (ex - exp1));
     const T h2 = H2[id] + g2 * (idey * (exy - ex) + idex * (ey - eyx));
     if (vx) {
-      H0[id] = h0;
-      H1[id] = h1;
-      H2[id] = h2;
+      if (CHK) {
+        visit(f.vis[0], id);
+        visit(f.vis[1], id);
+        visit(f.vis[2], id);
+      } else {
+        H0[id] = h0;
+        H1[id] = h1;
+        H2[id] = h2;
+      }
     }
     ex = exp1;
     ey = eyp1;
@@ -393,7 +411,7 @@ template <typename T, bool IDX, int C> struct EShellCell {
   }
 };
 
-template <typename T, bool IDX>
+template <typename T, bool IDX, bool CHK = false>
 __global__ __launch_bounds__(256) void k_e_shell(const ShellPiece *pcs, const ShellWork *work, const EArgs<T> *ap)
 {
   const ShellPiece *pc;
@@ -428,9 +446,14 @@ __global__ __launch_bounds__(256) void k_e_shell(const ShellPiece *pcs, const Sh
     c0.load(g, rg[0], n3, id, cls);
     c1.load(g, rg[1], n3, id, cls);
     c2.load(g, rg[2], n3, id, cls);
-    c0.store(g, rg[0], id, idh1 * (hz - hzy) + idh2 * (hym - hy));
-    c1.store(g, rg[1], id, idh2 * (hx - hxm) + idh0 * (hzx - hz));
-    c2.store(g, rg[2], id, idh0 * (hy - hyx) + idh1 * (hxy - hx));
+    if (CHK) {
+      for (int C = 0; C < 3; ++C)
+        if (rg[C].reg >= 0) visit(a.vis[C], id);
+    } else {
+      c0.store(g, rg[0], id, idh1 * (hz - hzy) + idh2 * (hym - hy));
+      c1.store(g, rg[1], id, idh2 * (hx - hxm) + idh0 * (hzx - hz));
+      c2.store(g, rg[2], id, idh0 * (hy - hyx) + idh1 * (hxy - hx));
+    }
     hxm = hx;
     hym = hy;
     rg[0].p += rg[0].psz;
@@ -494,7 +517,7 @@ template <typename T, bool IDX, int C> struct HShellCell {
   }
 };
 
-template <typename T, bool IDX>
+template <typename T, bool IDX, bool CHK = false>
 __global__ __launch_bounds__(256) void k_h_shell(const ShellPiece *pcs, const ShellWork *work, const HArgs<T> *ap)
 {
   const ShellPiece *pc;
@@ -528,9 +551,14 @@ __global__ __launch_bounds__(256) void k_h_shell(const ShellPiece *pcs, const Sh
     c0.load(q, rg[0], n3, id, cls);
     c1.load(q, rg[1], n3, id, cls);
     c2.load(q, rg[2], n3, id, cls);
-    c0.store(q, rg[0], id, ide2 * (eyp1 - eyc) + ide1 * (ez - ezy));
-    c1.store(q, rg[1], id, ide0 * (ezx - ez) + ide2 * (exc - exp1));
-    c2.store(q, rg[2], id, ide1 * (exy - exc) + ide0 * (eyc - eyx));
+    if (CHK) {
+      for (int C = 0; C < 3; ++C)
+        if (rg[C].reg >= 0) visit(a.vis[C], id);
+    } else {
+      c0.store(q, rg[0], id, ide2 * (eyp1 - eyc) + ide1 * (ez - ezy));
+      c1.store(q, rg[1], id, ide0 * (ezx - ez) + ide2 * (exc - exp1));
+      c2.store(q, rg[2], id, ide1 * (exy - exc) + ide0 * (eyc - eyx));
+    }
     exc = exp1;
     eyc = eyp1;
     rg[0].p += rg[0].psz;
@@ -543,7 +571,7 @@ __global__ __launch_bounds__(256) void k_h_shell(const ShellPiece *pcs, const Sh
 // Mur tangential E (mur.c:135-395), before the E sweep.  blockIdx.z selects
 // the (face, component) slot; x/y of the launch span the face.
 // ---------------------------------------------------------------------------
-template <typename T> __global__ void k_mur_e(MurArgs<T> a)
+template <typename T, bool CHK = false> __global__ void k_mur_e(MurArgs<T> a)
 {
   const int slot = blockIdx.z;
   const int c = a.comp[slot];
@@ -573,7 +601,30 @@ template <typename T> __global__ void k_mur_e(MurArgs<T> a)
   else curl = a.idh[0][i] * (Hy[nb] - Hy[nb - 1]) + a.idh[1][j] * (Hx[nb - SJ] - Hx[nb]);
   T *E = a.E[c];
   const T tmp = a.alpha[c][nb] * E[nb] + a.beta[c][nb] * curl;
-  E[id] = E[nb] + a.zeta[face] * (tmp - E[id]);
+  if (CHK) visit(a.vis[c], id);
+  else E[id] = E[nb] + a.zeta[face] * (tmp - E[id]);
+}
+
+// Tally of one component's visit counts against its outer field limits lim
+// (grid.c:951-1025 checks the same limits): cells inside that no sweep
+// updated, cells updated more than once, cells outside that were updated;
+// out[3] keeps the lowest (k*NY + j)*NX + i of an offending cell, and counts
+// (when given) the visits per cell, x fastest, saturated at 255.
+struct Lim6 { int v[6]; };
+static __global__ void k_visit_tally(const unsigned *vis, int NX, int NY, int NZ, long long PX, Lim6 lim,
+                                     unsigned long long *out, unsigned char *counts)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+  if (i >= NX) return;
+  const unsigned v = vis[((long long)k * NY + j) * PX + i];
+  const long long lin = ((long long)k * NY + j) * NX + i;
+  if (counts) counts[lin] = (unsigned char)(v > 255u ? 255u : v);
+  const bool in = i >= lim.v[0] && i <= lim.v[1] && j >= lim.v[2] && j <= lim.v[3] && k >= lim.v[4] && k <= lim.v[5];
+  const bool missed = in && v == 0, repeated = v > 1, outside = !in && v > 0;
+  if (missed) atomicAdd(out + 0, 1ull);
+  if (repeated) atomicAdd(out + 1, 1ull);
+  if (outside) atomicAdd(out + 2, 1ull);
+  if (missed || repeated || outside) atomicMin(out + 3, (unsigned long long)lin);
 }
 
 // ---------------------------------------------------------------------------

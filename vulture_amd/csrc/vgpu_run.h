@@ -367,7 +367,9 @@ template <typename T> int prepare_sweeps(Ctx *c)
   return VGPU_OK;
 }
 
-template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool timeKernels)
+// The loop (simulation.c:136-230).  With chk, one step of the field sweeps
+// only (Mur E, the E sweep, the H sweep), in their update-counting form.
+template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool timeKernels, const VisitRun *chk)
 {
   const double pi = 4.0 * atan(1.0);
   const double dt = c->grid.dt;
@@ -408,12 +410,17 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
   }
   fe.ecls = ea.ecls; fe.etab = ea.etab; fe.dfuse = ea.dfuse; fe.ap = P<EArgs<T>>(c->d_eargs);
   fh.hcls = ha.hcls; fh.htab = ha.htab;
-  const EArgs<T> *eap = P<EArgs<T>>(c->d_eargs);
-  const HArgs<T> *hap = P<HArgs<T>>(c->d_hargs);
+  const EArgs<T> *eap = P<EArgs<T>>(chk ? chk->eargs : c->d_eargs);
+  const HArgs<T> *hap = P<HArgs<T>>(chk ? chk->hargs : c->d_hargs);
+  if (chk)
+    for (int d = 0; d < 3; ++d) { fe.vis[d] = chk->vis[d]; fh.vis[d] = chk->vis[3 + d]; }
   const ShellPiece *epc = P<ShellPiece>(c->d_epcs), *hpc = P<ShellPiece>(c->d_hpcs);
   const ShellWork *ewk = P<ShellWork>(c->d_ework), *hwk = P<ShellWork>(c->d_hwork);
   auto fast_e = [&]() {
-    if (egrid.x) {
+    if (egrid.x && chk) {
+      if (c->indexed) hipLaunchKernelGGL((k_e_fast<T, true, false, true>), eg1, vblk, 0, s, fe);
+      else hipLaunchKernelGGL((k_e_fast<T, false, false, true>), eg1, vblk, 0, s, fe);
+    } else if (egrid.x) {
       if (c->indexed && c->debyeFused) hipLaunchKernelGGL((k_e_fast<T, true, true>), eg1, vblk, 0, s, fe);
       else if (c->indexed) hipLaunchKernelGGL((k_e_fast<T, true, false>), eg1, vblk, 0, s, fe);
       else hipLaunchKernelGGL((k_e_fast<T, false, false>), eg1, vblk, 0, s, fe);
@@ -422,12 +429,17 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
   auto shell_e = [&]() {
     if (c->nEwork) {
       const dim3 g((unsigned)c->nEwork);
-      if (c->indexed) hipLaunchKernelGGL((k_e_shell<T, true>), g, dim3(256), 0, s, epc, ewk, eap);
+      if (chk && c->indexed) hipLaunchKernelGGL((k_e_shell<T, true, true>), g, dim3(256), 0, s, epc, ewk, eap);
+      else if (chk) hipLaunchKernelGGL((k_e_shell<T, false, true>), g, dim3(256), 0, s, epc, ewk, eap);
+      else if (c->indexed) hipLaunchKernelGGL((k_e_shell<T, true>), g, dim3(256), 0, s, epc, ewk, eap);
       else hipLaunchKernelGGL((k_e_shell<T, false>), g, dim3(256), 0, s, epc, ewk, eap);
     }
   };
   auto fast_h = [&]() {
-    if (hgrid.x) {
+    if (hgrid.x && chk) {
+      if (c->indexed) hipLaunchKernelGGL((k_h_fast<T, true, true>), hg1, vblk, 0, s, fh);
+      else hipLaunchKernelGGL((k_h_fast<T, false, true>), hg1, vblk, 0, s, fh);
+    } else if (hgrid.x) {
       if (c->indexed) hipLaunchKernelGGL((k_h_fast<T, true>), hg1, vblk, 0, s, fh);
       else hipLaunchKernelGGL((k_h_fast<T, false>), hg1, vblk, 0, s, fh);
     }
@@ -435,7 +447,9 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
   auto shell_h = [&]() {
     if (c->nHwork) {
       const dim3 g((unsigned)c->nHwork);
-      if (c->indexed) hipLaunchKernelGGL((k_h_shell<T, true>), g, dim3(256), 0, s, hpc, hwk, hap);
+      if (chk && c->indexed) hipLaunchKernelGGL((k_h_shell<T, true, true>), g, dim3(256), 0, s, hpc, hwk, hap);
+      else if (chk) hipLaunchKernelGGL((k_h_shell<T, false, true>), g, dim3(256), 0, s, hpc, hwk, hap);
+      else if (c->indexed) hipLaunchKernelGGL((k_h_shell<T, true>), g, dim3(256), 0, s, hpc, hwk, hap);
       else hipLaunchKernelGGL((k_h_shell<T, false>), g, dim3(256), 0, s, hpc, hwk, hap);
     }
   };
@@ -448,6 +462,8 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
       ma.E[d] = P<T>(c->F[d]); ma.H[d] = P<T>(c->F[3 + d]);
       ma.alpha[d] = P<T>(c->alpha[d]); ma.beta[d] = P<T>(c->beta[d]); ma.idh[d] = P<T>(c->idh[d]);
     }
+    if (chk)
+      for (int d = 0; d < 3; ++d) ma.vis[d] = chk->vis[d];
     static const int order[6][2] = {{1, 2}, {1, 2}, {0, 2}, {0, 2}, {1, 0}, {1, 0}};
     for (int f = 0; f < 6; ++f) {
       ma.zeta[f] = T(c->zeta[f]);
@@ -474,6 +490,19 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
     da.lastE = P<T>(c->d_dlast); da.J = P<T>(c->d_dJ); da.stride = c->debyeSites;
     da.dalpha = P<double>(c->d_dalpha); da.dbeta = P<double>(c->d_dbeta); da.npoles = P<int>(c->d_dnp);
     da.maxPoles = c->debyeMaxPoles; da.cplx = c->debyeCplx;
+  }
+
+  if (chk) {
+    if (c->murmask)
+      hipLaunchKernelGGL((k_mur_e<T, true>), dim3((murmax1 + 15) / 16, (murmax2 + 15) / 16, 12), dim3(16, 16), 0, s,
+                         ma);
+    fast_e();
+    shell_e();
+    fast_h();
+    shell_h();
+    const hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return set_error(c, VGPU_EDEVICE, "kernel launch failed: %s", hipGetErrorString(err));
+    return VGPU_OK;
   }
 
   SibcEArgs<T> se{};
@@ -626,6 +655,77 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return set_error(c, VGPU_EDEVICE, "kernel launch failed: %s", hipGetErrorString(err));
   return VGPU_OK;
+}
+
+// Update-once check (the reference's CHECK_LIMITS build: grid.h:181-191, and
+// checkGrid, grid.c:951-1025).  One step of the field sweeps -- Mur E, the E
+// sweep, the H sweep, as the loop launches them -- runs in its counting form,
+// which adds one to a per-cell count for each update instead of storing it;
+// each component's counts are then tallied against its outer field limits
+// folim.  Fields, UPML auxiliaries and Debye state are left as they were.
+template <typename T>
+int check_limits(Ctx *c, const int folim[6][6], vgpu_limits_report *rep, unsigned char *counts)
+{
+  const size_t cells = (size_t)c->plane * c->NZ;      // device layout, padded rows
+  const size_t ncell = (size_t)c->NX * c->NY * c->NZ; // counts layout, x fastest
+  unsigned *vis = nullptr;
+  char *args = nullptr;
+  unsigned long long *tally = nullptr;
+  unsigned char *dcounts = nullptr;
+  int r = VGPU_OK;
+  const size_t hoff = (sizeof(EArgs<T>) + 255) / 256 * 256;
+  if (hipDeviceSynchronize() != hipSuccess) return set_error(c, VGPU_EDEVICE, "device error before the check");
+  if (hipMalloc((void **)&vis, 6 * cells * sizeof(unsigned)) != hipSuccess ||
+      hipMalloc((void **)&args, hoff + sizeof(HArgs<T>)) != hipSuccess ||
+      hipMalloc((void **)&tally, 24 * sizeof(unsigned long long)) != hipSuccess ||
+      (counts && hipMalloc((void **)&dcounts, 6 * ncell) != hipSuccess)) {
+    r = set_error(c, VGPU_ENOMEM, "update-once check buffers");
+  } else {
+    EArgs<T> ea;
+    HArgs<T> ha;
+    unsigned long long init[24];
+    for (int q = 0; q < 24; ++q) init[q] = (q & 3) == 3 ? ~0ull : 0ull;
+    bool ok = hipMemcpy(&ea, c->d_eargs, sizeof ea, hipMemcpyDeviceToHost) == hipSuccess &&
+              hipMemcpy(&ha, c->d_hargs, sizeof ha, hipMemcpyDeviceToHost) == hipSuccess;
+    VisitRun vr{};
+    for (int q = 0; q < 6; ++q) vr.vis[q] = vis + q * cells;
+    for (int d = 0; d < 3; ++d) { ea.vis[d] = vr.vis[d]; ha.vis[d] = vr.vis[3 + d]; }
+    vr.eargs = args;
+    vr.hargs = args + hoff;
+    ok = ok && hipMemset(vis, 0, 6 * cells * sizeof(unsigned)) == hipSuccess &&
+         hipMemcpy(args, &ea, sizeof ea, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(args + hoff, &ha, sizeof ha, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(tally, init, sizeof init, hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) r = set_error(c, VGPU_EDEVICE, "update-once check set-up");
+    if (!r) r = run_steps<T>(c, 1, c->stream, false, &vr);
+    if (!r) {
+      for (int C = 0; C < 6; ++C) {
+        Lim6 lim;
+        for (int f = 0; f < 6; ++f) lim.v[f] = folim[C][f];
+        hipLaunchKernelGGL(k_visit_tally, dim3((c->NX + 255) / 256, c->NY, c->NZ), dim3(256), 0, c->stream,
+                           vr.vis[C], c->NX, c->NY, c->NZ, (long long)c->PX, lim, tally + 4 * C,
+                           dcounts ? dcounts + C * ncell : nullptr);
+      }
+      unsigned long long out[24];
+      if (hipStreamSynchronize(c->stream) != hipSuccess ||
+          hipMemcpy(out, tally, sizeof out, hipMemcpyDeviceToHost) != hipSuccess ||
+          (counts && hipMemcpy(counts, dcounts, 6 * ncell, hipMemcpyDeviceToHost) != hipSuccess)) {
+        r = set_error(c, VGPU_EDEVICE, "update-once check: %s", hipGetErrorString(hipGetLastError()));
+      } else {
+        for (int C = 0; C < 6; ++C) {
+          rep->missed[C] = out[4 * C];
+          rep->repeated[C] = out[4 * C + 1];
+          rep->outside[C] = out[4 * C + 2];
+          rep->first[C] = out[4 * C + 3] == ~0ull ? -1 : (int64_t)out[4 * C + 3];
+        }
+      }
+    }
+  }
+  (void)hipFree(vis);
+  (void)hipFree(args);
+  (void)hipFree(tally);
+  (void)hipFree(dcounts);
+  return r;
 }
 
 }  // namespace vgpu

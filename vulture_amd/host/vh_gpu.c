@@ -445,3 +445,11 @@ void vh_solve_shape(const vh_solve *s, long long out[3])
 {
   for (int a = 0; a < 3; a++) out[a] = s->numCells[a];
 }
+
+/* Outer field limits gfolim[component][face] (grid.c:404), the limits the
+ * update-once check (vgpu_check_limits) tallies against. */
+void vh_solve_field_limits(const vh_solve *s, int out[6][6])
+{
+  for (int c = 0; c < 6; c++)
+    for (int f = 0; f < 6; f++) out[c][f] = s->gfolim[c][f];
+}

@@ -32,5 +32,6 @@ void vh_dsolve_close(vh_dsolve *d);
 /* out[0] gobox cells, out[1] mesh cells, out[2] time steps; numCells. */
 void vh_solve_sizes(const vh_solve *s, long long out[3]);
 void vh_solve_shape(const vh_solve *s, long long out[3]);
+void vh_solve_field_limits(const vh_solve *s, int out[6][6]);
 
 #endif
