@@ -278,14 +278,33 @@ int vgpu_set_pml(vgpu_ctx *c, const vgpu_pml_desc *p)
   if (!c || c->finalized || !p) return VGPU_ESTATE;
   c->pmlmask = p->region_mask & 0x3Fu;
   memcpy(c->fplim, p->fplim, sizeof(c->fplim));
+  // Slab offsets within the pools, in elements, each slab 256-byte aligned.
+  // The shell kernels index the pools with 32-bit element offsets.
+  const long long al = 256 / (long long)c->esz;
+  long long offE[6][3] = {}, offH[6][3] = {}, totE = 0, totH = 0;
   for (int r = 0; r < 6; ++r) {
     if (!((c->pmlmask >> r) & 1u)) continue;
     for (int d = 0; d < 3; ++d) {
-      const long long ne = box_cells(c->fplim[r][d]), nh = box_cells(c->fplim[r][3 + d]);
-      c->aux1[r][d] = dev_alloc(c, (size_t)ne * c->esz);
-      c->aux2[r][d] = dev_alloc(c, (size_t)ne * c->esz);
-      c->auxB[r][d] = dev_alloc(c, (size_t)nh * c->esz);
-      if (!c->aux1[r][d] || !c->aux2[r][d] || !c->auxB[r][d]) return set_error(c, VGPU_ENOMEM, "UPML allocation");
+      offE[r][d] = totE;
+      offH[r][d] = totH;
+      totE += (box_cells(c->fplim[r][d]) + al - 1) / al * al;
+      totH += (box_cells(c->fplim[r][3 + d]) + al - 1) / al * al;
+    }
+  }
+  if (totE > 0x7fffffffLL || totH > 0x7fffffffLL)
+    return set_error(c, VGPU_EINVAL, "UPML auxiliaries exceed 2^31 elements");
+  if (c->pmlmask) {
+    for (int q = 0; q < 3; ++q) {
+      c->auxPool[q] = dev_alloc(c, (size_t)(q < 2 ? totE : totH) * c->esz);
+      if (!c->auxPool[q]) return set_error(c, VGPU_ENOMEM, "UPML allocation");
+    }
+    for (int r = 0; r < 6; ++r) {
+      if (!((c->pmlmask >> r) & 1u)) continue;
+      for (int d = 0; d < 3; ++d) {
+        c->aux1[r][d] = (char *)c->auxPool[0] + offE[r][d] * c->esz;
+        c->aux2[r][d] = (char *)c->auxPool[1] + offE[r][d] * c->esz;
+        c->auxB[r][d] = (char *)c->auxPool[2] + offH[r][d] * c->esz;
+      }
     }
   }
   const int n[3] = {c->NX, c->NY, c->NZ};

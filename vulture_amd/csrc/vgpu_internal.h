@@ -30,6 +30,7 @@ template <typename T> struct EArgs {
   unsigned pmlmask;
   int pl[6][3][6];       // UPML slab limits [region][component][face]
   T *PP[6][3], *P[6][3]; // UPML auxiliaries, x-fastest within each slab box
+  T *PPpool, *Ppool;     // the pools holding them (same layout: a slab is at the same offset in both)
   const T *ad[3], *bd[3], *ah[3], *ibh[3];
   // coefficient classes: per-cell class, [class][8] alpha/beta (6 used), [class][3] Debye medium
   const uint16_t *ecls;
@@ -57,6 +58,7 @@ template <typename T> struct HArgs {
   unsigned pmlmask;
   int pl[6][3][6];
   T *B[6][3];
+  T *Bpool;              // the pool holding the B slabs
   const T *ad[3], *ah[3], *bh[3], *ibd[3];
   const uint16_t *hcls;  // coefficient classes: per-cell class, [class][4] gamma (3 used)
   const T *htab;
@@ -413,6 +415,9 @@ struct Ctx {
   unsigned pmlmask = 0;
   int fplim[6][6][6];
   void *aux1[6][3] = {}, *aux2[6][3] = {}, *auxB[6][3] = {};
+  // UPML auxiliaries live in three pools (E: PP and P with the same layout,
+  // H: B); aux1/aux2/auxB point into them, each slab 256-byte aligned.
+  void *auxPool[3] = {};
   void *prof[6][3] = {};   // ad, bd, ah, bh, ibd, ibh per axis
 
   // Mur

@@ -1,6 +1,6 @@
-// Kernel templates of the FDTD core, instantiated for double (compiled with
-// -ffp-contract=off so every update performs the reference's operations in
-// the reference's order) and for float (contraction allowed).
+// Kernel templates of the FDTD core, instantiated for double and for float,
+// both compiled with -ffp-contract=off so every update performs the
+// reference's operations in the reference's order.
 //
 // Volume sweeps: a fast kernel over the box where every component takes the
 // ordinary update, and a shell kernel for the rest (see below).
@@ -18,6 +18,14 @@ namespace vgpu {
 // vector and LDS counters together.  Device arrays are global memory: saying
 // so turns them into global loads, whose waits stay partial.
 template <typename T> VG_DEV VG_GLOBAL T *gp(T *p) { return (VG_GLOBAL T *)p; }
+
+// The 1-D edge-length and UPML profile arrays are never written by a kernel.
+// Read through the constant address space, an element indexed by the plane
+// number (uniform across a wavefront) is a scalar load into a scalar
+// register instead of a vector load per lane.
+#define VG_CONST __attribute__((address_space(4)))
+template <typename T> VG_DEV const VG_CONST T *cp(const T *p) { return (const VG_CONST T *)p; }
+template <typename T> VG_DEV const VG_CONST T *cp(const VG_GLOBAL T *p) { return (const VG_CONST T *)(const T *)p; }
 
 // Coefficient class tables on the device are padded to 8 (E: alpha, beta
 // per component) and 4 (H: gamma per component) entries per class, so a
@@ -221,7 +229,7 @@ __global__ __launch_bounds__(512) void k_e_fast(FastE<T> f)
     const T hx = Hx[id], hy = Hy[id], hz = Hz[id];
     const T hzy = Hz[id - SJ], hzx = Hz[id - 1], hyx = Hy[id - 1], hxy = Hx[id - SJ];
     const unsigned cls = IDX ? (unsigned)f.ecls[id] : 0u;
-    const T idhz = f.idh[2][k];
+    const T idhz = cp(f.idh[2])[k];
     const T e0 = e_fast<T, IDX, FUSE, 0>(f, bx, j, k, vx, id, cls, idhy * (hz - hzy) + idhz * (hym - hy));
     const T e1 = e_fast<T, IDX, FUSE, 1>(f, bx, j, k, vx, id, cls, idhz * (hx - hxm) + idhx * (hzx - hz));
     const T e2 = e_fast<T, IDX, FUSE, 2>(f, bx, j, k, vx, id, cls, idhx * (hy - hyx) + idhy * (hxy - hx));
@@ -270,7 +278,7 @@ template <typename T, bool IDX, bool CHK = false> __global__ __launch_bounds__(5
     const T exp1 = Ex[id + P], eyp1 = Ey[id + P];
     const T ez = Ez[id], ezy = Ez[id + SJ], ezx = Ez[id + 1], exy = Ex[id + SJ], eyx = Ey[id + 1];
     const unsigned cls = IDX ? (unsigned)f.hcls[id] : 0u;
-    const T idez = f.ide[2][k];
+    const T idez = cp(f.ide[2])[k];
     T g0, g1, g2;
     if (IDX) {
       load_gammas(f.htab + cls * HTAB_STRIDE, g0, g1, g2);
@@ -329,6 +337,20 @@ struct ColRegion {
   long long p, psz;
   void *a, *b;
 };
+// A lane's region in the kernels: reg as above, and for a UPML slab the
+// 32-bit element offset q of the lane's current cell in the auxiliaries'
+// pools (the slab's offset in the pool plus its slab index), advanced by psz
+// per plane.  Three 32-bit values per component instead of a 64-bit index,
+// stride and two 64-bit slab addresses keep the shell kernels' register
+// count, and so their occupancy, lower.
+struct LaneRegion {
+  int reg, q, psz;
+  template <typename T> VG_DEV LaneRegion(const ColRegion &r, const T *pool)
+      : reg(r.reg), q(r.reg >= 0 && r.reg < 6 ? (int)(static_cast<const T *>(r.a) - pool + r.p) : 0),
+        psz((int)r.psz)
+  {
+  }
+};
 
 template <int C> VG_HD ColRegion shell_region(const ShellPiece *pc, int i, int dj, int dk)
 {
@@ -350,10 +372,13 @@ template <int C> VG_HD ColRegion shell_region(const ShellPiece *pc, int i, int d
 // once, before the plane loop.
 template <typename T> struct EShellPtrs {
   VG_GLOBAL T *E[3];
+  VG_GLOBAL T *PP, *Pa;  // UPML auxiliary pools
   VG_GLOBAL const T *H[3], *alpha[3], *beta[3], *ad[3], *bd[3], *ah[3], *ibh[3], *idh[3], *etab;
   VG_GLOBAL const uint16_t *ecls;
   VG_DEV explicit EShellPtrs(const EArgs<T> &a)
   {
+    PP = gp(a.PPpool);
+    Pa = gp(a.Ppool);
     for (int d = 0; d < 3; ++d) {
       E[d] = gp(a.E[d]);
       H[d] = gp(a.H[d]);
@@ -372,10 +397,32 @@ template <typename T> struct EShellPtrs {
 
 // One E component of a shell cell (grid.c:709-795 ordinary update;
 // pml.c:859-1023 UPML split update with the axis roles of component C: P
-// along axis C+1, E along C+2): loads, then the stores.
+// along axis C+1, E along C+2): loads, then the stores.  The UPML profile
+// coefficients indexed by x or y are fixed over a thread's z march and load
+// once (init); those indexed by z load per plane.
 template <typename T, bool IDX, int C> struct EShellCell {
+  static constexpr int A1 = (C + 1) % 3, A2 = (C + 2) % 3;
   T e, al, be, opp, op, cf[6];
-  VG_DEV void load(const EShellPtrs<T> &g, const ColRegion &rg, const int n3[3], long long id, unsigned cls)
+  VG_DEV void coef(const EShellPtrs<T> &g, const int n3[3], bool zaxis)
+  {
+    if ((A1 == 2) == zaxis) {
+      cf[0] = A1 == 2 ? cp(g.ad[A1])[n3[A1]] : g.ad[A1][n3[A1]];
+      cf[1] = A1 == 2 ? cp(g.bd[A1])[n3[A1]] : g.bd[A1][n3[A1]];
+    }
+    if ((A2 == 2) == zaxis) {
+      cf[2] = A2 == 2 ? cp(g.ad[A2])[n3[A2]] : g.ad[A2][n3[A2]];
+      cf[3] = A2 == 2 ? cp(g.bd[A2])[n3[A2]] : g.bd[A2][n3[A2]];
+    }
+    if ((C == 2) == zaxis) {
+      cf[4] = C == 2 ? cp(g.ibh[C])[n3[C]] : g.ibh[C][n3[C]];
+      cf[5] = C == 2 ? cp(g.ah[C])[n3[C]] : g.ah[C][n3[C]];
+    }
+  }
+  VG_DEV void init(const EShellPtrs<T> &g, const LaneRegion &rg, const int n3[3])
+  {
+    if (rg.reg >= 0 && rg.reg < 6) coef(g, n3, false);
+  }
+  VG_DEV void load(const EShellPtrs<T> &g, const LaneRegion &rg, const int n3[3], long long id, unsigned cls)
   {
     if (rg.reg < 0) return;
     e = g.E[C][id];
@@ -386,26 +433,20 @@ template <typename T, bool IDX, int C> struct EShellCell {
       be = g.beta[C][id];
     }
     if (rg.reg < 6) {
-      constexpr int A1 = (C + 1) % 3, A2 = (C + 2) % 3;
-      opp = gp(static_cast<const T *>(rg.a))[rg.p];
-      op = gp(static_cast<const T *>(rg.b))[rg.p];
-      cf[0] = g.ad[A1][n3[A1]];
-      cf[1] = g.bd[A1][n3[A1]];
-      cf[2] = g.ad[A2][n3[A2]];
-      cf[3] = g.bd[A2][n3[A2]];
-      cf[4] = g.ibh[C][n3[C]];
-      cf[5] = g.ah[C][n3[C]];
+      opp = g.PP[rg.q];
+      op = g.Pa[rg.q];
+      coef(g, n3, true);
     }
   }
-  VG_DEV void store(const EShellPtrs<T> &g, const ColRegion &rg, long long id, T curl) const
+  VG_DEV void store(const EShellPtrs<T> &g, const LaneRegion &rg, long long id, T curl) const
   {
     if (rg.reg == 6) {
       g.E[C][id] = al * e + be * curl;
     } else if (rg.reg >= 0) {
       const T ppn = al * opp + be * curl;
-      gp(static_cast<T *>(rg.a))[rg.p] = ppn;
+      g.PP[rg.q] = ppn;
       const T pn = cf[0] * op + cf[1] * (ppn - opp);
-      gp(static_cast<T *>(rg.b))[rg.p] = pn;
+      g.Pa[rg.q] = pn;
       g.E[C][id] = cf[2] * e + cf[3] * cf[4] * (pn - cf[5] * op);
     }
   }
@@ -422,8 +463,9 @@ __global__ __launch_bounds__(256) void k_e_shell(const ShellPiece *pcs, const Sh
   const int NY = a.NY;
   const long long P = a.plane, SJ = a.PX;
   const int i = pc->box[0] + di, j = pc->box[2] + dj;
-  ColRegion rg[3] = {shell_region<0>(pc, i, dj, dk0), shell_region<1>(pc, i, dj, dk0),
-                     shell_region<2>(pc, i, dj, dk0)};
+  LaneRegion rg[3] = {LaneRegion(shell_region<0>(pc, i, dj, dk0), a.PPpool),
+                      LaneRegion(shell_region<1>(pc, i, dj, dk0), a.PPpool),
+                      LaneRegion(shell_region<2>(pc, i, dj, dk0), a.PPpool)};
   if (rg[0].reg < 0 && rg[1].reg < 0 && rg[2].reg < 0) return;
   const VG_GLOBAL T *Hx = g.H[0], *Hy = g.H[1], *Hz = g.H[2];
   // (the sweep never updates index 0 of any axis; the guards keep every read
@@ -433,16 +475,22 @@ __global__ __launch_bounds__(256) void k_e_shell(const ShellPiece *pcs, const Sh
   int k = pc->box[4] + dk0;
   long long id = ((long long)k * NY + j) * SJ + i;
   T hxm = k > 0 ? Hx[id - P] : T(0), hym = k > 0 ? Hy[id - P] : T(0);  // plane k-1, carried
+  EShellCell<T, IDX, 0> c0;
+  EShellCell<T, IDX, 1> c1;
+  EShellCell<T, IDX, 2> c2;
+  {
+    const int n3[3] = {i, j, k};
+    c0.init(g, rg[0], n3);
+    c1.init(g, rg[1], n3);
+    c2.init(g, rg[2], n3);
+  }
   for (int dk = dk0; dk < dk1; ++dk, ++k, id += P) {
     const T hx = Hx[id], hy = Hy[id], hz = Hz[id];
     const T hzy = ym ? Hz[id - SJ] : T(0), hzx = xm ? Hz[id - 1] : T(0);
     const T hyx = xm ? Hy[id - 1] : T(0), hxy = ym ? Hx[id - SJ] : T(0);
-    const T idh2 = g.idh[2][k];
+    const T idh2 = cp(g.idh[2])[k];
     const unsigned cls = IDX ? (unsigned)g.ecls[id] : 0u;
     const int n3[3] = {i, j, k};
-    EShellCell<T, IDX, 0> c0;
-    EShellCell<T, IDX, 1> c1;
-    EShellCell<T, IDX, 2> c2;
     c0.load(g, rg[0], n3, id, cls);
     c1.load(g, rg[1], n3, id, cls);
     c2.load(g, rg[2], n3, id, cls);
@@ -456,19 +504,21 @@ __global__ __launch_bounds__(256) void k_e_shell(const ShellPiece *pcs, const Sh
     }
     hxm = hx;
     hym = hy;
-    rg[0].p += rg[0].psz;
-    rg[1].p += rg[1].psz;
-    rg[2].p += rg[2].psz;
+    rg[0].q += rg[0].psz;
+    rg[1].q += rg[1].psz;
+    rg[2].q += rg[2].psz;
   }
 }
 
 // The H shell's arrays as global pointers.
 template <typename T> struct HShellPtrs {
   VG_GLOBAL T *H[3];
+  VG_GLOBAL T *B;  // UPML auxiliary pool
   VG_GLOBAL const T *E[3], *gamma[3], *ad[3], *ah[3], *bh[3], *ibd[3], *ide[3], *htab;
   VG_GLOBAL const uint16_t *hcls;
   VG_DEV explicit HShellPtrs(const HArgs<T> &a)
   {
+    B = gp(a.Bpool);
     for (int d = 0; d < 3; ++d) {
       H[d] = gp(a.H[d]);
       E[d] = gp(a.E[d]);
@@ -486,32 +536,46 @@ template <typename T> struct HShellPtrs {
 
 // One H component of a shell cell: ordinary update (inner grid or Mur normal
 // H, grid.c:797-888) or UPML split update (pml.c:1026-1176): B along axis
-// C+1, H along C+2.
+// C+1, H along C+2.  Profile coefficients indexed by x or y load once.
 template <typename T, bool IDX, int C> struct HShellCell {
+  static constexpr int A1 = (C + 1) % 3, A2 = (C + 2) % 3;
   T h, g, ob, cf[6];
-  VG_DEV void load(const HShellPtrs<T> &q, const ColRegion &rg, const int n3[3], long long id, unsigned cls)
+  VG_DEV void coef(const HShellPtrs<T> &q, const int n3[3], bool zaxis)
+  {
+    if ((A1 == 2) == zaxis) {
+      cf[0] = A1 == 2 ? cp(q.ah[A1])[n3[A1]] : q.ah[A1][n3[A1]];
+      cf[1] = A1 == 2 ? cp(q.bh[A1])[n3[A1]] : q.bh[A1][n3[A1]];
+    }
+    if ((A2 == 2) == zaxis) {
+      cf[2] = A2 == 2 ? cp(q.ah[A2])[n3[A2]] : q.ah[A2][n3[A2]];
+      cf[3] = A2 == 2 ? cp(q.bh[A2])[n3[A2]] : q.bh[A2][n3[A2]];
+    }
+    if ((C == 2) == zaxis) {
+      cf[4] = C == 2 ? cp(q.ibd[C])[n3[C]] : q.ibd[C][n3[C]];
+      cf[5] = C == 2 ? cp(q.ad[C])[n3[C]] : q.ad[C][n3[C]];
+    }
+  }
+  VG_DEV void init(const HShellPtrs<T> &q, const LaneRegion &rg, const int n3[3])
+  {
+    if (rg.reg >= 0 && rg.reg < 6) coef(q, n3, false);
+  }
+  VG_DEV void load(const HShellPtrs<T> &q, const LaneRegion &rg, const int n3[3], long long id, unsigned cls)
   {
     if (rg.reg < 0) return;
     h = q.H[C][id];
     g = IDX ? q.htab[cls * HTAB_STRIDE + C] : q.gamma[C][id];
     if (rg.reg < 6) {
-      constexpr int A1 = (C + 1) % 3, A2 = (C + 2) % 3;
-      ob = gp(static_cast<const T *>(rg.a))[rg.p];
-      cf[0] = q.ah[A1][n3[A1]];
-      cf[1] = q.bh[A1][n3[A1]];
-      cf[2] = q.ah[A2][n3[A2]];
-      cf[3] = q.bh[A2][n3[A2]];
-      cf[4] = q.ibd[C][n3[C]];
-      cf[5] = q.ad[C][n3[C]];
+      ob = q.B[rg.q];
+      coef(q, n3, true);
     }
   }
-  VG_DEV void store(const HShellPtrs<T> &q, const ColRegion &rg, long long id, T curl) const
+  VG_DEV void store(const HShellPtrs<T> &q, const LaneRegion &rg, long long id, T curl) const
   {
     if (rg.reg == 6) {
       q.H[C][id] = h + g * curl;
     } else if (rg.reg >= 0) {
       const T bn = cf[0] * ob + g * cf[1] * curl;
-      gp(static_cast<T *>(rg.a))[rg.p] = bn;
+      q.B[rg.q] = bn;
       q.H[C][id] = cf[2] * h + cf[3] * cf[4] * (bn - cf[5] * ob);
     }
   }
@@ -528,8 +592,9 @@ __global__ __launch_bounds__(256) void k_h_shell(const ShellPiece *pcs, const Sh
   const int NX = a.NX, NY = a.NY, NZ = a.NZ;
   const long long P = a.plane, SJ = a.PX;
   const int i = pc->box[0] + di, j = pc->box[2] + dj;
-  ColRegion rg[3] = {shell_region<0>(pc, i, dj, dk0), shell_region<1>(pc, i, dj, dk0),
-                     shell_region<2>(pc, i, dj, dk0)};
+  LaneRegion rg[3] = {LaneRegion(shell_region<0>(pc, i, dj, dk0), a.Bpool),
+                      LaneRegion(shell_region<1>(pc, i, dj, dk0), a.Bpool),
+                      LaneRegion(shell_region<2>(pc, i, dj, dk0), a.Bpool)};
   if (rg[0].reg < 0 && rg[1].reg < 0 && rg[2].reg < 0) return;
   const VG_GLOBAL T *Ex = q.E[0], *Ey = q.E[1], *Ez = q.E[2];
   const bool xin = i + 1 < NX, yin = j + 1 < NY;
@@ -537,17 +602,23 @@ __global__ __launch_bounds__(256) void k_h_shell(const ShellPiece *pcs, const Sh
   int k = pc->box[4] + dk0;
   long long id = ((long long)k * NY + j) * SJ + i;
   T exc = Ex[id], eyc = Ey[id];  // this plane's Ex, Ey: carried from the previous plane's k+1 loads
+  HShellCell<T, IDX, 0> c0;
+  HShellCell<T, IDX, 1> c1;
+  HShellCell<T, IDX, 2> c2;
+  {
+    const int n3[3] = {i, j, k};
+    c0.init(q, rg[0], n3);
+    c1.init(q, rg[1], n3);
+    c2.init(q, rg[2], n3);
+  }
   for (int dk = dk0; dk < dk1; ++dk, ++k, id += P) {
     const bool zin = k + 1 < NZ;
     const T exp1 = zin ? Ex[id + P] : T(0), eyp1 = zin ? Ey[id + P] : T(0);
     const T ez = Ez[id], ezy = yin ? Ez[id + SJ] : T(0), ezx = xin ? Ez[id + 1] : T(0);
     const T eyx = xin ? Ey[id + 1] : T(0), exy = yin ? Ex[id + SJ] : T(0);
-    const T ide2 = q.ide[2][k];
+    const T ide2 = cp(q.ide[2])[k];
     const unsigned cls = IDX ? (unsigned)q.hcls[id] : 0u;
     const int n3[3] = {i, j, k};
-    HShellCell<T, IDX, 0> c0;
-    HShellCell<T, IDX, 1> c1;
-    HShellCell<T, IDX, 2> c2;
     c0.load(q, rg[0], n3, id, cls);
     c1.load(q, rg[1], n3, id, cls);
     c2.load(q, rg[2], n3, id, cls);
@@ -561,9 +632,9 @@ __global__ __launch_bounds__(256) void k_h_shell(const ShellPiece *pcs, const Sh
     }
     exc = exp1;
     eyc = eyp1;
-    rg[0].p += rg[0].psz;
-    rg[1].p += rg[1].psz;
-    rg[2].p += rg[2].psz;
+    rg[0].q += rg[0].psz;
+    rg[1].q += rg[1].psz;
+    rg[2].q += rg[2].psz;
   }
 }
 

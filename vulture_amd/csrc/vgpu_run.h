@@ -76,6 +76,9 @@ template <typename T> static void fill_volume_args(Ctx *c, EArgs<T> &ea, HArgs<T
       ea.P[r][d] = P<T>(c->aux2[r][d]);
       ha.B[r][d] = P<T>(c->auxB[r][d]);
     }
+  ea.PPpool = P<T>(c->auxPool[0]);
+  ea.Ppool = P<T>(c->auxPool[1]);
+  ha.Bpool = P<T>(c->auxPool[2]);
 }
 
 // UPML auxiliaries of slab r, component C, carried by a shell run.
