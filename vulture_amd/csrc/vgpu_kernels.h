@@ -1084,7 +1084,8 @@ template <typename T> __global__ void k_observe(ObsArgs<T> a)
   const int nf = a.nfreq[o];
   if (f >= nf) return;
   const double ot = a.omega[a.dftoff[o] / 6 + f] * t;
-  const double co = cos(ot), si = sin(ot);
+  double si, co;
+  sincos(ot, &si, &co);
   double *re = a.dre + a.dftoff[o], *im = a.dim + a.dftoff[o];
   for (int c = 0; c < 6; ++c) {
     re[c * nf + f] += v[c] * co;

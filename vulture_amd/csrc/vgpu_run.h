@@ -624,9 +624,14 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
     fast_e();
     if (timeKernels && ne < (int)c->evE.size() / 2) { hipEventRecord(c->evE[2 * ne + 1], s); ++ne; }
     shell_e();
-    for (int l = c->debyeFused ? 1 : 0; l < c->debyeLayers; ++l) {
+    // Layers 0 and 1 hold each cell's first block, inside and outside the
+    // fast E box, so their sites never share an edge: without fusion they
+    // go in one launch.  Later layers follow one launch each, in block order.
+    for (int l = c->debyeFused ? 1 : 0; l < c->debyeLayers;) {
+      const int l1 = (!c->debyeFused && l == 0 && c->debyeLayers > 1) ? 2 : l + 1;
       da.base = c->debyeLayerStart[l];
-      da.n = c->debyeLayerStart[l + 1] - c->debyeLayerStart[l];
+      da.n = c->debyeLayerStart[l1] - c->debyeLayerStart[l];
+      l = l1;
       if (da.n == 0) continue;
       hipLaunchKernelGGL(k_debye<T>, dim3((unsigned)((da.n + 255) / 256)), dim3(256), 0, s, da);
     }
