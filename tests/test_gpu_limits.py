@@ -89,3 +89,23 @@ def test_full_size_update_once_and_fields_unchanged(tmp_path):
         for a, b in zip(before, after):
             assert np.count_nonzero(a) > 1000
             assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("key,rc", [("freespace_mur/freespace_mur.mesh", 6), ("freespace_pml/freespace_pml.mesh", 0)])
+def test_command_line_check(key, rc, tmp_path):
+    """vgvulture -L: the command-line form of the check warns for exactly the
+    reference's un-updated cells and exits 6 when there are any, as the
+    reference's limit-checking build does."""
+    import re
+    import subprocess
+    from conftest import ROOT
+    mesh = _stage(key, tmp_path)
+    print(f"case vgvulture -L {key}", flush=True)
+    r = subprocess.run([os.path.join(ROOT, "vulture_amd", "bin", "vgvulture"), "-L", mesh], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == rc, r.stdout[-2000:] + r.stderr[-2000:]
+    got = {(m.group(1), int(m.group(2)), int(m.group(3)), int(m.group(4)))
+           for m in re.finditer(r"Warning: (\w+)\[(\d+)\]\[(\d+)\]\[(\d+)\] not updated", r.stdout)}
+    exp = {(f, *c) for f in FIELDS for c in FX[key]["unvisited"][f]}
+    assert got == exp
+    assert "repeated 0 outside 0" in r.stdout

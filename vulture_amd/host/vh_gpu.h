@@ -29,6 +29,11 @@ double vh_dsolve_loop_ms(const vh_dsolve *d);
 vgpu_ctx *vh_dsolve_context(vh_dsolve *d);
 void vh_dsolve_close(vh_dsolve *d);
 
+/* Parse, set up and upload a mesh without running it; vh_gpu_close frees both. */
+int vh_gpu_open(const char *mesh, int prec, int device, long steps, unsigned long chunk, vh_solve **solve,
+                vgpu_ctx **ctx, char *err, int errlen);
+void vh_gpu_close(vh_solve *s, vgpu_ctx *c);
+
 /* out[0] gobox cells, out[1] mesh cells, out[2] time steps; numCells. */
 void vh_solve_sizes(const vh_solve *s, long long out[3]);
 void vh_solve_shape(const vh_solve *s, long long out[3]);
