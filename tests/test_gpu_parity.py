@@ -158,6 +158,23 @@ def test_full_size_fp32_tracks_fp64_and_fp64_is_linear(tmp_path):
         assert np.array_equal(c, 2.0 * a)
 
 
+@pytest.mark.parametrize("name,kw", [("enclosure", {}), ("debye", {})], ids=["enclosure-full", "debye-full"])
+def test_full_config_fp32_tracks_fp64(name, kw, tmp_path):
+    """The full configurations, too slow for the CPU oracle: the graded
+    shielded enclosure (200x160x120 cells, 10-cell UPML, TF/SF plane wave,
+    512-frequency DFT, 4000 steps) and the Debye-sphere volume (256^3, 3000
+    steps), FP32 observers within 1e-4 time-series relative L2 and 0.05 dB of
+    the FP64 device run (itself held to the oracle at 1e-12 on the same
+    generators at reduced size above)."""
+    mesh = meshgen.write(name, str(tmp_path), **kw)
+    for prec in ("fp64", "fp32"):
+        print(f"case {name} full {prec}", flush=True)
+        vulture_amd.run_mesh(mesh, str(tmp_path / prec), precision=prec)
+    r = compare_outputs(str(tmp_path / "fp32"), str(tmp_path / "fp64"))
+    assert r["files"] > 0
+    assert r["td"] <= 1e-4 and r["fd_db"] <= 0.05, r
+
+
 @pytest.mark.parametrize("name", ["parplate1D_debyepml_xyz", "closedbox_go04", "freespace_mur", "dgref_pml6"])
 def test_full_coefficient_arrays_path(name, tmp_path, monkeypatch):
     """The sweep kernels' fallback that reads the per-cell alpha/beta/gamma
