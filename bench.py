@@ -198,6 +198,8 @@ def main():
     ap.add_argument("--serve-steps", type=int, nargs=2, default=[200, 800], help="step range of serving requests")
     ap.add_argument("--sibc-steps", type=int, default=30,
                     help="timed steps of the SIBC + Mur box at 384x384x256 in FP32 (0 skips it)")
+    ap.add_argument("--sibc-only", action="store_true",
+                    help="run only the SIBC + Mur box leg and print its record (for profiling that workload)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -205,6 +207,9 @@ def main():
         return 2
     from vulture_amd import meshgen
     work = tempfile.mkdtemp(prefix="vgbench_")
+    if args.sibc_only:
+        print(json.dumps(sibc_box(work, max(args.sibc_steps, 1))), flush=True)
+        return 0
     mesh = meshgen.write("debye", work, n=args.n, steps=args.warmup + args.steps, spheres=args.spheres,
                          boundary=args.boundary, planewave=not args.no_planewave)
     res = {}

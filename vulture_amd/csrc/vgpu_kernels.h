@@ -738,7 +738,11 @@ template <typename T> __global__ __launch_bounds__(128) void k_sibc_e(SibcEArgs<
   if (f >= w.f1) return;
   const int i = a.ijk[3 * f], j = a.ijk[3 * f + 1], k = a.ijk[3 * f + 2];
   const int s = w.surf;
-  const int nrm = a.normal[s];
+  // the surface's matrices and filter tables are the same for every face of
+  // the block: read through the constant address space they are scalar loads
+  const VG_CONST double *qtab = cp(a.q), *frat = cp(a.fratio), *asym = cp(a.asymp);
+  const VG_CONST int *npol = cp(a.npoles), *fbas = cp(a.fbase);
+  const int nrm = cp(a.normal)[s];
   const long long SJ = a.PX, P = a.plane;
   const long long id = ((long long)k * a.NY + j) * SJ + i;
   const unsigned char *A = a.adjA + 4 * f, *B = a.adjB + 4 * f;
@@ -764,13 +768,13 @@ template <typename T> __global__ __launch_bounds__(128) void k_sibc_e(SibcEArgs<
     Htan[2] = T(0.5 * (1 + A[2] + A[3]) * (double)(Hy[id - P] + Hy[id - P + SJ]));
     Htan[3] = T(0.5 * (1 + B[2] + B[3]) * (double)(Hy[id] + Hy[id + SJ]));
   }
-  const double *Am = a.A + 16 * s, *Bm = a.B + 16 * s;
+  const VG_CONST double *Am = cp(a.A) + 16 * s, *Bm = cp(a.B) + 16 * s;
   T Hin[4], Eout[4];
   for (int p = 0; p < 4; ++p) {
     Hin[p] = T(0);
     for (int q = 0; q < 4; ++q) Hin[p] = Hin[p] + T(Am[4 * p + q]) * Htan[q];
   }
-  const int set = a.filtset[s];
+  const int set = cp(a.filtset)[s];
   const long long NF = a.nfaces;
   // every filter fed by input q saw the same input last step: one stored
   // previous input per (input, face), read before and written after all
@@ -778,18 +782,18 @@ template <typename T> __global__ __launch_bounds__(128) void k_sibc_e(SibcEArgs<
   bool fed[4] = {false, false, false, false};
   for (int q = 0; q < 4; ++q) {
     xo4[q] = a.old[q * NF + f];
-    for (int p = 0; p < 4; ++p) fed[q] = fed[q] || a.npoles[16 * set + 4 * p + q] > 0;
+    for (int p = 0; p < 4; ++p) fed[q] = fed[q] || npol[16 * set + 4 * p + q] > 0;
   }
   if (a.factored) {
     // one state per (input, pole), scaled by each filter's residue ratio
     double y16[16];
-    for (int flt = 0; flt < 16; ++flt) y16[flt] = a.asymp[16 * set + flt] * (double)Hin[flt & 3];
+    for (int flt = 0; flt < 16; ++flt) y16[flt] = asym[16 * set + flt] * (double)Hin[flt & 3];
     for (int q = 0; q < 4; ++q) {
-      const int b = a.fbase[4 * set + q];
+      const int b = fbas[4 * set + q];
       if (b < 0) continue;
       const double x = (double)Hin[q], xo = xo4[q];
-      const int np = a.npoles[16 * set + b];
-      const double *qq = a.q + ((long long)(16 * set + b) * a.maxPoles) * 6;
+      const int np = npol[16 * set + b];
+      const VG_CONST double *qq = qtab + ((long long)(16 * set + b) * a.maxPoles) * 6;
       VG_GLOBAL double *zq = gp(a.zeta) + (long long)q * a.maxPoles * NF + f;
       // states in groups of eight: all loads of a group issued before its math
       for (int k0 = 0; k0 < np; k0 += 8) {
@@ -805,7 +809,7 @@ template <typename T> __global__ __launch_bounds__(128) void k_sibc_e(SibcEArgs<
           zq[(long long)kp * NF] = xi;
           for (int p = 0; p < 4; ++p) {
             const int flt = 4 * p + q;
-            if (a.npoles[16 * set + flt] > 0) y16[flt] += a.fratio[((long long)(16 * set + flt)) * a.maxPoles + kp] * xi;
+            if (npol[16 * set + flt] > 0) y16[flt] += frat[((long long)(16 * set + flt)) * a.maxPoles + kp] * xi;
           }
         }
       }
@@ -814,7 +818,7 @@ template <typename T> __global__ __launch_bounds__(128) void k_sibc_e(SibcEArgs<
       Eout[p] = T(0);
       for (int q = 0; q < 4; ++q) {
         const int flt = 4 * p + q;
-        if (a.npoles[16 * set + flt] == 0 && a.asymp[16 * set + flt] == 0.0) continue;
+        if (npol[16 * set + flt] == 0 && asym[16 * set + flt] == 0.0) continue;
         Eout[p] = T((double)Eout[p] + y16[flt]);
       }
     }
@@ -823,14 +827,14 @@ template <typename T> __global__ __launch_bounds__(128) void k_sibc_e(SibcEArgs<
     Eout[p] = T(0);
     for (int q = 0; q < 4; ++q) {
       const int flt = 4 * p + q;
-      const int np = a.npoles[16 * set + flt];
-      const double asy = a.asymp[16 * set + flt];
+      const int np = npol[16 * set + flt];
+      const double asy = asym[16 * set + flt];
       if (np == 0 && asy == 0.0) continue;
       const double x = (double)Hin[q];
       double y = asy * x;
       if (np > 0) {
         const double xo = xo4[q];
-        const double *qq = a.q + ((long long)(16 * set + flt) * a.maxPoles) * 6;
+        const VG_CONST double *qq = qtab + ((long long)(16 * set + flt) * a.maxPoles) * 6;
         if (a.real) {
           // real coefficients: the complex step below with zero imaginary
           // parts, which leave the same real part bit for bit
