@@ -140,6 +140,8 @@ template <typename T> struct MurArgs {
   int box[12][6];        // limits of face f component slot s at [2f+s]
   int comp[12];          // component of each slot (-1 unused)
   T zeta[6];
+  const uint16_t *ecls;  // coefficient classes (null: read the per-cell alpha/beta arrays)
+  const T *etab;
   unsigned *vis[3];      // update-once check (null in ordinary runs)
 };
 

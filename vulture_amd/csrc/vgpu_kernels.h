@@ -671,7 +671,14 @@ template <typename T, bool CHK = false> __global__ void k_mur_e(MurArgs<T> a)
   else if (c == 1) curl = a.idh[2][k] * (Hx[nb] - Hx[nb - P]) + a.idh[0][i] * (Hz[nb - 1] - Hz[nb]);
   else curl = a.idh[0][i] * (Hy[nb] - Hy[nb - 1]) + a.idh[1][j] * (Hx[nb - SJ] - Hx[nb]);
   T *E = a.E[c];
-  const T tmp = a.alpha[c][nb] * E[nb] + a.beta[c][nb] * curl;
+  T al, be;
+  if (a.ecls) {
+    load_pair(a.etab + (unsigned)a.ecls[nb] * ETAB_STRIDE + 2 * c, al, be);
+  } else {
+    al = a.alpha[c][nb];
+    be = a.beta[c][nb];
+  }
+  const T tmp = al * E[nb] + be * curl;
   if (CHK) visit(a.vis[c], id);
   else E[id] = E[nb] + a.zeta[face] * (tmp - E[id]);
 }

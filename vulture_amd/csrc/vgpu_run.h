@@ -465,6 +465,7 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
       ma.E[d] = P<T>(c->F[d]); ma.H[d] = P<T>(c->F[3 + d]);
       ma.alpha[d] = P<T>(c->alpha[d]); ma.beta[d] = P<T>(c->beta[d]); ma.idh[d] = P<T>(c->idh[d]);
     }
+    if (c->indexed) { ma.ecls = ea.ecls; ma.etab = ea.etab; }
     if (chk)
       for (int d = 0; d < 3; ++d) ma.vis[d] = chk->vis[d];
     static const int order[6][2] = {{1, 2}, {1, 2}, {0, 2}, {0, 2}, {1, 0}, {1, 0}};
