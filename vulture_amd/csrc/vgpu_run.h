@@ -380,8 +380,8 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
   EArgs<T> ea{};
   HArgs<T> ha{};
   fill_volume_args<T>(c, ea, ha);
-  // Fast kernels: 64x4 tiles (64 x ty) over the fast boxes, marching z in chunks sized
-  // for about 16k blocks at 272^3 (at least 4 planes so the carry pays).
+  // Fast kernels: 64x4 tiles (64 x ty) over the fast boxes, marching z in chunks of 4
+  // planes (measured best from 272^3 to 386x386x258; 16k blocks and more at those sizes).
   // rows per block (tuning override VGPU_FAST_TY: 1..8)
   int ty = 4;
   if (const char *ev = getenv("VGPU_FAST_TY")) ty = std::max(1, std::min(atoi(ev), 8));
@@ -389,8 +389,7 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
   auto fast_grid = [&](const int *fb, int &kchunk) {
     if (fb[0] > fb[1] || fb[2] > fb[3] || fb[4] > fb[5]) return dim3(0, 0, 0);
     const int gx = (fb[1] - fb[0] + 64) / 64, gy = (fb[3] - fb[2] + ty) / ty, nz = fb[5] - fb[4] + 1;
-    kchunk = (int)((long long)nz * gx * gy / 16384);
-    kchunk = std::max(4, std::min(kchunk, 32));
+    kchunk = std::min(4, nz);
     if (const char *ev = getenv("VGPU_FAST_KCHUNK"))  // tuning override: planes per block
       kchunk = std::max(1, std::min(atoi(ev), nz));
     return dim3(gx, gy, (nz + kchunk - 1) / kchunk);
