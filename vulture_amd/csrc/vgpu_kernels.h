@@ -964,11 +964,11 @@ VG_DEV double wf_value(const WaveformDev &w, double t, double delay, double pi)
 
 // ---------------------------------------------------------------------------
 // 1-D auxiliary grids of the TF/SF plane waves (planewave.c:1315-1372).
-// One block per plane wave.
+// One block per plane wave; the blocks ride in other launches (k_head for E,
+// the H corrections' k_corr for H), whose own work they do not touch.
 // ---------------------------------------------------------------------------
-template <typename T> __global__ void k_aux_e(AuxArgs<T> a, double pi)
+template <typename T> VG_DEV void aux_e_block(const AuxArgs<T> &a, int p, double pi)
 {
-  const int p = blockIdx.x;
   const PwDev &pw = a.pw[p];
   if (!pw.use_aux) return;
   T *Eyi = a.aux + (long long)p * 2 * a.auxmax, *Hzi = Eyi + a.auxmax;
@@ -991,9 +991,8 @@ template <typename T> __global__ void k_aux_e(AuxArgs<T> a, double pi)
   if (threadIdx.x == 0) Eyi[0] = T(wf_value(a.wf[pw.waveform], t, pw.delay, pi));
 }
 
-template <typename T> __global__ void k_aux_h(AuxArgs<T> a)
+template <typename T> VG_DEV void aux_h_block(const AuxArgs<T> &a, int p)
 {
-  const int p = blockIdx.x;
   const PwDev &pw = a.pw[p];
   if (!pw.use_aux) return;
   T *Eyi = a.aux + (long long)p * 2 * a.auxmax, *Hzi = Eyi + a.auxmax;
@@ -1031,8 +1030,15 @@ __global__ void k_corr_gather(long long nrows, const long long *key, int shift, 
   if (rowalpha) rowalpha[r] = alpha ? alpha[c][id] : T(1);
 }
 
-template <typename T> __global__ void k_corr(CorrArgs<T> a, double pi)
+// With AUXH, blocks from corrBlocks on carry the aux grids' H update: it
+// writes aux H, which the H corrections do not read (they read aux E).
+template <typename T, bool AUXH = false>
+__global__ void k_corr(CorrArgs<T> a, double pi, AuxArgs<T> xa, int corrBlocks)
 {
+  if (AUXH && (int)blockIdx.x >= corrBlocks) {
+    aux_h_block(xa, (int)blockIdx.x - corrBlocks);
+    return;
+  }
   const long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (r >= a.nrows) return;
   const double t = ((double)(*a.step + (unsigned long long)a.soff) + a.half) * a.dt;
@@ -1077,10 +1083,8 @@ template <typename T> __global__ void k_corr(CorrArgs<T> a, double pi)
 // samples for impulse.dat (observer.c:1164-1198).  One thread per
 // (observer, frequency); thread f==0 also stores the time sample.
 // ---------------------------------------------------------------------------
-template <typename T> __global__ void k_observe(ObsArgs<T> a)
+template <typename T> VG_DEV void observe_block(const ObsArgs<T> &a, int o)
 {
-  const int o = blockIdx.y;
-  if (o >= a.nobs) return;
   const unsigned long long n = *a.step + (unsigned long long)a.soff;
   if (n < a.first[o] || n > a.last[o]) return;
   const double t = (double)n * a.dt;
@@ -1102,6 +1106,18 @@ template <typename T> __global__ void k_observe(ObsArgs<T> a)
     re[c * nf + f] += v[c] * co;
     im[c * nf + f] -= v[c] * si;
   }
+}
+
+// Head of a step: the observers sample the fields (blocks y < nobs) and each
+// plane wave's aux grid takes its E update (blocks y >= nobs, x = 0).  The
+// aux E update reads the aux H of the previous step and is read only by the
+// H corrections, so it may run before the E sweep instead of after it
+// (planewave.c:576-590 runs it just before the E corrections).
+template <typename T> __global__ void k_head(ObsArgs<T> oa, AuxArgs<T> xa, double pi)
+{
+  const int y = blockIdx.y;
+  if (y < oa.nobs) observe_block(oa, y);
+  else if (blockIdx.x == 0) aux_e_block(xa, y - oa.nobs, pi);
 }
 
 template <typename T> __global__ void k_observe_volume(VolObsArgs<T> a)

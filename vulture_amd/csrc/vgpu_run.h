@@ -2,9 +2,12 @@
 // reference's order (simulation.c:157-222):
 //   observers -> Mur E -> E sweep (fast box with the first Debye layer, then
 //   the shell: inner-grid edges and UPML) -> remaining Debye layers -> SIBC E
-//   -> aux grids E -> sources/TF-SF E -> ghost E -> H sweep (fast box, shell:
-//   Mur normal H and UPML) -> SIBC H -> aux grids H -> sources/TF-SF H ->
-//   ghost H
+//   -> sources/TF-SF E -> ghost E -> H sweep (fast box, shell: Mur normal H
+//   and UPML) -> SIBC H -> sources/TF-SF H -> ghost H
+// The aux grids of the plane waves touch nothing but their own arrays: their
+// E update rides in the observers' launch and their H update in the H
+// corrections' launch, which keeps the order in which each aux array is
+// written and read.
 #pragma once
 #include "vgpu_kernels.h"
 #include <algorithm>
@@ -609,9 +612,12 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
   int ne = 0, nh = 0;
   for (uint64_t q = 0; q < nsteps; ++q) {
     const int soff = (int)q;
-    if (oa.nobs) {
+    // observers, and the aux grids' E update as extra blocks (k_head)
+    if (oa.nobs || anyAux) {
       oa.soff = soff;
-      hipLaunchKernelGGL(k_observe<T>, dim3((maxnf + 63) / 64, oa.nobs), dim3(64), 0, s, oa);
+      xa.soff = soff;
+      const unsigned gy = (unsigned)(oa.nobs + (anyAux ? xa.npw : 0));
+      hipLaunchKernelGGL(k_head<T>, dim3(oa.nobs ? (maxnf + 63) / 64 : 1, gy), dim3(64), 0, s, oa, xa, pi);
     }
     if (va.nnodes) {
       va.soff = soff;
@@ -637,13 +643,10 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
     }
     if (c->sibcFaces)
       hipLaunchKernelGGL(k_sibc_e<T>, dim3((unsigned)c->sibcBlocks), dim3(128), 0, s, se);
-    if (anyAux) {
-      xa.soff = soff;
-      hipLaunchKernelGGL(k_aux_e<T>, dim3(xa.npw), dim3(256), 0, s, xa, pi);
-    }
     if (ce[0].nrows) {
       ce[0].soff = soff;
-      hipLaunchKernelGGL(k_corr<T>, dim3((unsigned)((ce[0].nrows + 255) / 256)), dim3(256), 0, s, ce[0], pi);
+      const int nb = (int)((ce[0].nrows + 255) / 256);
+      hipLaunchKernelGGL((k_corr<T, false>), dim3((unsigned)nb), dim3(256), 0, s, ce[0], pi, xa, nb);
     }
     ghost(1);
     if (timeKernels && nh < (int)c->evH.size() / 2) hipEventRecord(c->evH[2 * nh], s);
@@ -652,10 +655,13 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
     shell_h();
     if (c->sibcFaces)
       hipLaunchKernelGGL(k_sibc_h<T>, dim3((unsigned)((sh.nrows + 255) / 256)), dim3(256), 0, s, sh);
-    if (anyAux) hipLaunchKernelGGL(k_aux_h<T>, dim3(xa.npw), dim3(256), 0, s, xa);
-    if (ce[1].nrows) {
+    // H corrections, and the aux grids' H update as extra blocks
+    if (ce[1].nrows || anyAux) {
       ce[1].soff = soff;
-      hipLaunchKernelGGL(k_corr<T>, dim3((unsigned)((ce[1].nrows + 255) / 256)), dim3(256), 0, s, ce[1], pi);
+      const int nb = (int)((ce[1].nrows + 255) / 256);
+      const dim3 g((unsigned)(nb + (anyAux ? xa.npw : 0)));
+      if (anyAux) hipLaunchKernelGGL((k_corr<T, true>), g, dim3(256), 0, s, ce[1], pi, xa, nb);
+      else hipLaunchKernelGGL((k_corr<T, false>), g, dim3(256), 0, s, ce[1], pi, xa, nb);
     }
     ghost(0);
   }
