@@ -1030,15 +1030,8 @@ __global__ void k_corr_gather(long long nrows, const long long *key, int shift, 
   if (rowalpha) rowalpha[r] = alpha ? alpha[c][id] : T(1);
 }
 
-// With AUXH, blocks from corrBlocks on carry the aux grids' H update: it
-// writes aux H, which the H corrections do not read (they read aux E).
-template <typename T, bool AUXH = false>
-__global__ void k_corr(CorrArgs<T> a, double pi, AuxArgs<T> xa, int corrBlocks)
+template <typename T> VG_DEV void corr_rows(const CorrArgs<T> &a, double pi)
 {
-  if (AUXH && (int)blockIdx.x >= corrBlocks) {
-    aux_h_block(xa, (int)blockIdx.x - corrBlocks);
-    return;
-  }
   const long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (r >= a.nrows) return;
   const double t = ((double)(*a.step + (unsigned long long)a.soff) + a.half) * a.dt;
@@ -1076,6 +1069,16 @@ __global__ void k_corr(CorrArgs<T> a, double pi, AuxArgs<T> xa, int corrBlocks)
     }
   }
   F[id] = f;
+}
+
+template <typename T> __global__ void k_corr(CorrArgs<T> a, double pi) { corr_rows(a, pi); }
+
+// H corrections with the aux grids' H update in the blocks from corrBlocks
+// on: it writes aux H, which the H corrections do not read (they read aux E).
+template <typename T> __global__ void k_corr_aux(CorrArgs<T> a, double pi, AuxArgs<T> xa, int corrBlocks)
+{
+  if ((int)blockIdx.x >= corrBlocks) aux_h_block(xa, (int)blockIdx.x - corrBlocks);
+  else corr_rows(a, pi);
 }
 
 // ---------------------------------------------------------------------------

@@ -645,8 +645,7 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
       hipLaunchKernelGGL(k_sibc_e<T>, dim3((unsigned)c->sibcBlocks), dim3(128), 0, s, se);
     if (ce[0].nrows) {
       ce[0].soff = soff;
-      const int nb = (int)((ce[0].nrows + 255) / 256);
-      hipLaunchKernelGGL((k_corr<T, false>), dim3((unsigned)nb), dim3(256), 0, s, ce[0], pi, xa, nb);
+      hipLaunchKernelGGL(k_corr<T>, dim3((unsigned)((ce[0].nrows + 255) / 256)), dim3(256), 0, s, ce[0], pi);
     }
     ghost(1);
     if (timeKernels && nh < (int)c->evH.size() / 2) hipEventRecord(c->evH[2 * nh], s);
@@ -660,8 +659,8 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
       ce[1].soff = soff;
       const int nb = (int)((ce[1].nrows + 255) / 256);
       const dim3 g((unsigned)(nb + (anyAux ? xa.npw : 0)));
-      if (anyAux) hipLaunchKernelGGL((k_corr<T, true>), g, dim3(256), 0, s, ce[1], pi, xa, nb);
-      else hipLaunchKernelGGL((k_corr<T, false>), g, dim3(256), 0, s, ce[1], pi, xa, nb);
+      if (anyAux) hipLaunchKernelGGL(k_corr_aux<T>, g, dim3(256), 0, s, ce[1], pi, xa, nb);
+      else hipLaunchKernelGGL(k_corr<T>, g, dim3(256), 0, s, ce[1], pi);
     }
     ghost(0);
   }

@@ -18,6 +18,7 @@ import argparse
 import csv
 import glob
 import os
+import re
 import sys
 
 
@@ -32,11 +33,19 @@ def _rows(d, suffix):
     return out
 
 
-def counter_per_dispatch(d, kernel, counter):
-    """Mean value of `counter` over dispatches whose kernel name contains `kernel`."""
+def _is_kernel(name, kernel):
+    """The dispatch's kernel is `kernel` itself (any template arguments), not
+    another kernel whose name starts with it (k_corr, not k_corr_gather)."""
+    return re.search(r"(^|[\s:])" + re.escape(kernel) + r"[<(]", name) is not None
+
+
+def counter_per_dispatch(d, kernel, counter, exact=True):
+    """Mean value of `counter` over the dispatches of kernel `kernel` (with
+    exact=False, of every kernel whose name contains `kernel`)."""
     per = {}
     for r in _rows(d, "counter_collection.csv"):
-        if kernel not in r.get("Kernel_Name", "") or r.get("Counter_Name") != counter:
+        name = r.get("Kernel_Name", "")
+        if not (_is_kernel(name, kernel) if exact else kernel in name) or r.get("Counter_Name") != counter:
             continue
         key = r.get("Dispatch_Id") or r.get("Correlation_Id")
         per[key] = per.get(key, 0.0) + float(r["Counter_Value"])
@@ -70,8 +79,8 @@ def cmd_stats(args):
 
 def cmd_traffic(args):
     copy_bytes = args.mib * (1 << 20)
-    cf, _ = counter_per_dispatch(args.copy_fetch, args.copy_kernel, "FETCH_SIZE")
-    cw, _ = counter_per_dispatch(args.copy_write, args.copy_kernel, "WRITE_SIZE")
+    cf, _ = counter_per_dispatch(args.copy_fetch, args.copy_kernel, "FETCH_SIZE", exact=False)
+    cw, _ = counter_per_dispatch(args.copy_write, args.copy_kernel, "WRITE_SIZE", exact=False)
     kf, nf = counter_per_dispatch(args.fetch, args.kernel, "FETCH_SIZE")
     kw, nw = counter_per_dispatch(args.write, args.kernel, "WRITE_SIZE")
     ff = copy_bytes / (cf * 1024.0)
