@@ -3,7 +3,9 @@
   python -m vulture_amd.profiling copy [--mib N]
       A plain elementwise copy of N MiB (default 1024), five times: the
       calibration kernel for the FETCH_SIZE / WRITE_SIZE counters (it reads
-      and writes exactly N MiB per dispatch).
+      and writes exactly N MiB per dispatch).  Then twenty more, timed with
+      events, for the bandwidth a plain streaming kernel reaches (read plus
+      write bytes per second).
   python -m vulture_amd.profiling stats DIR
       Print the kernel_stats.csv that `rocprofv3 --kernel-trace --stats -d DIR`
       wrote, as a text table.
@@ -62,6 +64,14 @@ def cmd_copy(args):
     for _ in range(5):
         torch.mul(a, 1.0, out=b)  # one elementwise kernel: reads and writes N MiB
     torch.cuda.synchronize()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(20):
+        torch.mul(a, 1.0, out=b)
+    t1.record()
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / 20
+    print(f"copy_ms {ms:.4f} copy_GBps {2 * args.mib * (1 << 20) / (ms * 1e-3) / 1e9:.1f}", flush=True)
     print(f"copied {args.mib} MiB x5", flush=True)
 
 
