@@ -9,6 +9,11 @@
   python -m vulture_amd.profiling stats DIR
       Print the kernel_stats.csv that `rocprofv3 --kernel-trace --stats -d DIR`
       wrote, as a text table.
+  python -m vulture_amd.profiling requests --kernel NAME --pmc DIR --copy DIR [--mib N]
+      From `rocprofv3 --pmc TCC_EA0_RDREQ_sum TCC_EA0_WRREQ_sum -d DIR` runs of
+      the workload and of the copy kernel, print the L2-to-memory bytes per
+      launch of kernel NAME, each request counter scaled by the bytes the copy
+      moves per request.
   python -m vulture_amd.profiling traffic --kernel NAME --fetch DIR --write DIR
           --copy-fetch DIR --copy-write DIR [--mib N]
       From four `rocprofv3 --pmc FETCH_SIZE|WRITE_SIZE -d DIR` runs (the
@@ -112,6 +117,29 @@ def cmd_traffic(args):
     print(f"traffic_bytes_per_launch {fetch + write:.0f}")
 
 
+def cmd_requests(args):
+    """L2-to-memory requests per launch (one --pmc pass with both counters),
+    each counter turned into bytes by what it reads for the copy of known
+    size, which reads and writes exactly N MiB."""
+    copy_bytes = args.mib * (1 << 20)
+    cr, _ = counter_per_dispatch(args.copy, args.copy_kernel, args.read_counter, exact=False)
+    cw, _ = counter_per_dispatch(args.copy, args.copy_kernel, args.write_counter, exact=False)
+    kr, n = counter_per_dispatch(args.pmc, args.kernel, args.read_counter)
+    kw, _ = counter_per_dispatch(args.pmc, args.kernel, args.write_counter)
+    br, bw = copy_bytes / cr, copy_bytes / cw
+    print(f"kernel {args.kernel}")
+    print(f"dispatches {n}")
+    print(f"copy_{args.read_counter} {cr:.0f}")
+    print(f"copy_{args.write_counter} {cw:.0f}")
+    print(f"bytes_per_read_request {br:.1f}")
+    print(f"bytes_per_write_request {bw:.1f}")
+    print(f"{args.read_counter}_per_launch {kr:.0f}")
+    print(f"{args.write_counter}_per_launch {kw:.0f}")
+    print(f"read_bytes_per_launch {kr * br:.0f}")
+    print(f"write_bytes_per_launch {kw * bw:.0f}")
+    print(f"traffic_bytes_per_launch {kr * br + kw * bw:.0f}")
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description="rocprofv3 helpers")
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -127,8 +155,16 @@ def main(argv=None):
     p.add_argument("--copy-write", required=True)
     p.add_argument("--copy-kernel", default="elementwise")
     p.add_argument("--mib", type=int, default=1024)
+    p = sub.add_parser("requests")
+    p.add_argument("--kernel", required=True)
+    p.add_argument("--pmc", required=True)
+    p.add_argument("--copy", required=True)
+    p.add_argument("--copy-kernel", default="elementwise")
+    p.add_argument("--read-counter", default="TCC_EA0_RDREQ_sum")
+    p.add_argument("--write-counter", default="TCC_EA0_WRREQ_sum")
+    p.add_argument("--mib", type=int, default=1024)
     args = ap.parse_args(argv)
-    {"copy": cmd_copy, "stats": cmd_stats, "traffic": cmd_traffic}[args.cmd](args)
+    {"copy": cmd_copy, "stats": cmd_stats, "traffic": cmd_traffic, "requests": cmd_requests}[args.cmd](args)
 
 
 if __name__ == "__main__":
