@@ -7,6 +7,9 @@
 //   - every other cell lies in at most one shell piece, whose x runs give
 //     the cell's own regions (and UPML slab indices), and a cell in no piece
 //     has no component updated;
+//   - a piece's rows shorter than half a line are padded to whole lines
+//     (lanes per row a multiple of the line's elements, the first cell at
+//     its own lane of a line), longer rows are not;
 //   - the launch table lists every block of every piece exactly once, each
 //     XCD (block b runs on XCD b % 8) taking one contiguous run of blocks of
 //     each piece, and the runs of a piece differ in length by at most one.
@@ -36,12 +39,12 @@ static int fail(const char *mesh, const char *what, int i, int j, int k, int C)
 
 template <int NB, typename A>
 static int check_sweep(const char *mesh, const A &a, const int (*in)[NB][6], const int *fb, const int N[3],
-                       long long &npcs, long long &shell)
+                       long long &npcs, long long &shell, int lw)
 {
   std::vector<ShellPiece> pcs;
   std::vector<ShellWork> work;
   std::string err;
-  if (build_shell<NB>(a, in, fb, N, pcs, work, shell, err)) {
+  if (build_shell<NB>(a, in, fb, N, pcs, work, shell, err, lw)) {
     printf("%s FAIL build_shell: %s\n", mesh, err.c_str());
     return 1;
   }
@@ -105,7 +108,13 @@ static int check_sweep(const char *mesh, const A &a, const int (*in)[NB][6], con
   std::vector<std::vector<int>> seen(pcs.size());
   for (size_t q = 0; q < pcs.size(); ++q) {
     const ShellPiece &pc = pcs[q];
-    if (pc.nk != pc.box[5] - pc.box[4] + 1 || pc.nxy != pc.ni * pc.nj || pc.nxyb != (pc.nxy + 255) / 256 ||
+    // rows shorter than half a line: nip lanes, a whole number of lw-element
+    // lines, the first cell at lane xo = box[0] % lw; other rows unpadded
+    const bool rows = lw && 2 * pc.ni < lw
+                          ? pc.xo == pc.box[0] % lw && pc.nip % lw == 0 && pc.nip >= pc.xo + pc.ni &&
+                                pc.nip < pc.xo + pc.ni + lw
+                          : pc.xo == 0 && pc.nip == pc.ni;
+    if (!rows || pc.nk != pc.box[5] - pc.box[4] + 1 || pc.nxy != pc.nip * pc.nj || pc.nxyb != (pc.nxy + 255) / 256 ||
         pc.kc < 1 || pc.kc > pc.nk || pc.nblk != pc.nxyb * ((pc.nk + pc.kc - 1) / pc.kc))
       return fail(mesh, "piece block layout", pc.box[0], pc.box[2], pc.box[4], -1);
     seen[q].assign(pc.nblk, -1);
@@ -172,8 +181,8 @@ int main(int argc, char **argv)
     // exact fast boxes, and boxes cut to whole lines of FP64 and FP32 elements
     for (int lw : {0, 16, 32}) {
       fast_boxes(c->grid.gfilim, N, c->fastE, c->fastH, lw);
-      if (check_sweep<1>(argv[m], ea, ein, c->fastE, N, pe, se) ||
-          check_sweep<3>(argv[m], ha, ha.in, c->fastH, N, ph, sh))
+      if (check_sweep<1>(argv[m], ea, ein, c->fastE, N, pe, se, lw) ||
+          check_sweep<3>(argv[m], ha, ha.in, c->fastH, N, ph, sh, lw))
         return 1;
     }
     printf("%s OK cells %lld E pieces %lld shell %lld H pieces %lld shell %lld\n", argv[m],

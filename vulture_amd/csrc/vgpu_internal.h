@@ -116,8 +116,9 @@ struct ShellRun {
 struct ShellPiece {
   int box[6];            // inclusive cell limits
   int ni, nj, nk, cells; // extents, cell count
-  int nxy, nxyb, kc, nblk;  // x-y plane cells, its 256-cell blocks, planes per block, blocks
+  int nxy, nxyb, kc, nblk;  // x-y plane lanes (nip per row), its 256-lane blocks, planes per block, blocks
   int nrun[3];           // runs per component (cells outside every run are not updated)
+  int nip, xo;           // lanes per row (a whole number of 128-byte lines) and the row's first cell's lane
   int pad_;
   ShellRun run[3][SHELL_RUNS];
 };

@@ -322,8 +322,9 @@ VG_DEV bool shell_column(const ShellPiece *pcs, const ShellWork *work, const She
   const int zc = w.block / pc->nxyb, xyb = w.block - zc * pc->nxyb;
   const int l = xyb * 256 + (int)threadIdx.x;
   if (l >= pc->nxy) return false;
-  dj = l / pc->ni;
-  di = l - dj * pc->ni;
+  dj = l / pc->nip;
+  di = l - dj * pc->nip - pc->xo;
+  if (di < 0 || di >= pc->ni) return false;
   dk0 = zc * pc->kc;
   dk1 = min(dk0 + pc->kc, pc->nk);
   return true;

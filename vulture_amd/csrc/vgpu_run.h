@@ -107,7 +107,7 @@ template <typename T> static void run_aux(const HArgs<T> &a, int r, int C, void 
 // each XCD gets one contiguous eighth of every piece's blocks.
 template <int NB, typename A>
 static int build_shell(const A &a, const int (*in)[NB][6], const int *fb, const int N[3], std::vector<ShellPiece> &pcs,
-                       std::vector<ShellWork> &work, long long &cells, std::string &err)
+                       std::vector<ShellWork> &work, long long &cells, std::string &err, int lw = 0)
 {
   const bool hasFast = fb[0] <= fb[1] && fb[2] <= fb[3] && fb[4] <= fb[5];
   std::vector<int> cut[3];
@@ -262,7 +262,19 @@ static int build_shell(const A &a, const int (*in)[NB][6], const int *fb, const 
       return -1;
     }
     pc.cells = (int)n;
-    pc.nxy = pc.ni * pc.nj;
+    // Rows shorter than half a line are padded to whole lines of lw
+    // elements, each starting on a line, so a wavefront's loads touch one
+    // line per row instead of one line per lane (a piece of one- or
+    // two-cell rows, as at a Mur face's x end).  Longer rows stay unpadded:
+    // there the idle lanes cost more than the split lines (measured).
+    const bool pad = lw > 0 && 2 * pc.ni < lw;
+    pc.xo = pad ? pc.box[0] % lw : 0;
+    pc.nip = pad ? (pc.xo + pc.ni + lw - 1) / lw * lw : pc.ni;
+    if ((long long)pc.nip * pc.nj > (1LL << 30)) {
+      err = "shell piece too large";
+      return -1;
+    }
+    pc.nxy = pc.nip * pc.nj;
     pc.nxyb = (pc.nxy + 255) / 256;
     pc.kc = std::min(pc.nk, kcmax);
     pc.nblk = pc.nxyb * ((pc.nk + pc.kc - 1) / pc.kc);
@@ -308,8 +320,9 @@ template <typename T> int prepare_sweeps(Ctx *c)
   int ein[3][1][6];
   for (int C = 0; C < 3; ++C)
     for (int q = 0; q < 6; ++q) ein[C][0][q] = ea.in[C][q];
-  if (build_shell<1>(ea, ein, c->fastE, N, ep, ew, c->shellCells[0], err) ||
-      build_shell<3>(ha, ha.in, c->fastH, N, hp, hw, c->shellCells[1], err))
+  const int lw = getenv("VGPU_SHELL_ROWS") && getenv("VGPU_SHELL_ROWS")[0] == '0' ? 0 : (int)(128 / sizeof(T));
+  if (build_shell<1>(ea, ein, c->fastE, N, ep, ew, c->shellCells[0], err, lw) ||
+      build_shell<3>(ha, ha.in, c->fastH, N, hp, hw, c->shellCells[1], err, lw))
     return set_error(c, VGPU_EINVAL, "%s", err.c_str());
   c->nEpcs = (int)ep.size(); c->nHpcs = (int)hp.size();
   c->nEwork = (int)ew.size(); c->nHwork = (int)hw.size();
