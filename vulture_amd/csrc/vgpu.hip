@@ -1273,6 +1273,8 @@ int vgpu_check_limits(vgpu_ctx *c, const int folim[6][6], vgpu_limits_report *ou
     for (int a = 0; a < 3; ++a)
       if (folim[C][2 * a] < 0 || folim[C][2 * a + 1] >= N[a])
         return set_error(c, VGPU_EINVAL, "field limits outside the grid");
+  if (c->NY > 65535 || c->NZ > 65535)  // the tally launches one block row per (j, k)
+    return set_error(c, VGPU_EINVAL, "update-once check: more than 65535 cells along y or z");
   hipSetDevice(c->device);
   (void)hipGetLastError();
   return c->prec == VGPU_FP64 ? check_limits<double>(c, folim, out, counts) : check_limits<float>(c, folim, out, counts);
