@@ -60,3 +60,19 @@ def test_python_front_end_fails_loudly_without_library(monkeypatch, tmp_path):
     monkeypatch.setattr(vulture_amd, "_host", None)
     with pytest.raises(vulture_amd.VgpuError):
         vulture_amd.load_vgpu()
+
+
+def test_command_line_solver_fails_loudly_without_a_device(tmp_path):
+    """No CPU fallback: with no visible GPU the solver exits with the
+    reference's fatal status (6) and says why; no arguments print the usage."""
+    import shutil
+    cli = os.path.join(ROOT, "vulture_amd", "bin", "vgvulture")
+    assert os.path.exists(cli), "build first: python -c 'import __graft_entry__ as g; g.build()'"
+    shutil.copytree(os.path.join(ROOT, "tests", "golden", "ref_inputs", "freespace_pml"), tmp_path / "m")
+    env = dict(os.environ, HIP_VISIBLE_DEVICES="-1", ROCR_VISIBLE_DEVICES="-1")
+    r = subprocess.run([cli, str(tmp_path / "m" / "freespace_pml.mesh")], env=env, capture_output=True, text=True,
+                       timeout=120, cwd=str(tmp_path))
+    assert r.returncode == 6, r.stdout + r.stderr
+    assert "no usable GPU" in r.stderr
+    r = subprocess.run([cli], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr

@@ -38,7 +38,13 @@ static double *cplx(const double complex *z, int n)
 
 static int fail(vh_solve *s, vgpu_ctx *c, const char *what, int rc)
 {
-  snprintf(s->err, sizeof(s->err), "%s failed (%d): %s", what, rc, c ? vgpu_last_error(c) : "");
+  /* without a context (vgpu_create itself failed) the code is all there is */
+  const char *why = c ? vgpu_last_error(c)
+                      : rc == VGPU_EDEVICE ? "no usable GPU (the HIP runtime found no device or could not select it)"
+                      : rc == VGPU_ENOMEM  ? "out of device memory"
+                      : rc == VGPU_EINVAL  ? "invalid grid description"
+                                           : "";
+  snprintf(s->err, sizeof(s->err), "%s failed (%d): %s", what, rc, why);
   return -1;
 }
 
