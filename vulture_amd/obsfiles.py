@@ -56,11 +56,20 @@ def compare_outputs(test_dir, ref_dir, band_db=60.0):
     multiplied back by the waveform DFT, so that frequencies where the
     excitation has no energy do not amplify rounding noise), and the worst
     magnitude error in dB over the band where the waveform spectrum lies
-    within `band_db` of its peak (`fd_db`).  Missing or misshapen files give
+    within `band_db` of its peak (`fd_db`).  `fd_db` leaves out bins where the
+    reference magnitude is below 1e-3 of its in-band maximum (the nulls of a
+    cavity or standing-wave spectrum, where a dB ratio measures rounding
+    noise); `fd_db_all` is the same maximum over every in-band bin with a
+    non-zero reference.  Missing
+    or misshapen files, and any non-finite value in a tested file, give
     infinity.
     """
-    res = {"td": 0.0, "fd_raw": 0.0, "fd_db": 0.0, "files": 0}
+    res = {"td": 0.0, "fd_raw": 0.0, "fd_db": 0.0, "fd_db_all": 0.0, "files": 0}
     inf = float("inf")
+
+    def worst(key, m):
+        # max() drops NaN (max(0.0, nan) == 0.0): a non-finite metric fails
+        res[key] = inf if not np.isfinite(m) else max(res[key], m)
     for f in observer_files(ref_dir, "eh_*_td.asc"):
         g = os.path.join(test_dir, os.path.basename(f))
         if not os.path.exists(g):
@@ -68,7 +77,8 @@ def compare_outputs(test_dir, ref_dir, band_db=60.0):
             continue
         _, _, va = read_td(g)
         _, _, vb = read_td(f)
-        res["td"] = max(res["td"], rel_l2(va, vb) if va.shape == vb.shape else inf)
+        finite = bool(np.all(np.isfinite(va)))
+        worst("td", rel_l2(va, vb) if va.shape == vb.shape and finite else inf)
         res["files"] += 1
     wf = waveform_spectrum(ref_dir)
     for f in observer_files(ref_dir, "eh_*_fd.asc"):
@@ -79,18 +89,20 @@ def compare_outputs(test_dir, ref_dir, band_db=60.0):
         _, va = read_fd(g)
         _, vb = read_fd(f)
         res["files"] += 1
-        if va.shape != vb.shape:
-            res["fd_raw"] = inf
+        if va.shape != vb.shape or not np.all(np.isfinite(va)):
+            res["fd_raw"] = res["fd_db"] = res["fd_db_all"] = inf
             continue
         w = wf[:, None] if wf is not None and len(wf) == len(vb) else np.ones((len(vb), 1))
-        res["fd_raw"] = max(res["fd_raw"],
-                            float(np.linalg.norm(va * w - vb * w) / max(np.linalg.norm(vb * w), 1e-300)))
+        worst("fd_raw", float(np.linalg.norm(va * w - vb * w) / max(np.linalg.norm(vb * w), 1e-300)))
         band = np.abs(w[:, 0]) >= np.abs(w[:, 0]).max() * 10 ** (-band_db / 20) if wf is not None else \
             np.ones(len(vb), bool)
         ma, mb = np.abs(va[band]), np.abs(vb[band])
         floor = mb.max() * 1e-3 if mb.size else 0.0
         sel = mb > floor
-        if sel.any():
-            db = np.abs(20 * np.log10(ma[sel] / mb[sel]))
-            res["fd_db"] = max(res["fd_db"], float(db.max()))
+        with np.errstate(divide="ignore", invalid="ignore"):
+            if sel.any():
+                worst("fd_db", float(np.abs(20 * np.log10(ma[sel] / mb[sel])).max()))
+            nz = mb > 0
+            if nz.any():
+                worst("fd_db_all", float(np.abs(20 * np.log10(ma[nz] / mb[nz])).max()))
     return res
