@@ -60,11 +60,23 @@ def debye_site_words(st):
 
 
 def kernel_bytes(st, esz):
-    """Compulsory bytes per launch of the fast E and H kernels.  Fast E: read
-    E and H (6 words) and the coefficient class (2 B; 6 words of alpha/beta
-    without classes), write E (3), plus the fused Debye sites; fast H: read H
-    and E (6), the class (2 B; 3 words of gamma), write H (3)."""
+    """Compulsory bytes per launch of the two volume kernels of the inner grid.
+
+    Fused sweep (k_leap) and its fix-up (k_h_fix): the fused sweep reads old E
+    and H (6 words) and both coefficient classes (2 + 2 B; 9 words of
+    alpha/beta/gamma without classes) and writes new E and H (6 words) of
+    every fast-box cell; the fix-up reads old H and the cell's new E (6 words)
+    and its H class (2 B; 3 words of gamma) and writes new H (3) of each H
+    cell updated again after the E corrections.
+
+    Separate sweeps (k_e_fast, k_h_fast): fast E reads E and H (6 words) and
+    the class (2 B; 6 words of alpha/beta without classes), writes E (3),
+    plus the fused Debye sites; fast H reads H and E (6), the class (2 B; 3
+    words of gamma), writes H (3)."""
     ce, ch = st["fast_cells"]
+    if st.get("leap"):
+        coef, coef_h = (4, 2) if st["indexed_media"] else (9 * esz, 3 * esz)
+        return ce * (12 * esz + coef), st["fix_cells"] * (9 * esz + coef_h)
     coef_e, coef_h = (2, 2) if st["indexed_media"] else (6 * esz, 3 * esz)
     e = ce * (9 * esz + coef_e) + st["debye_fused_sites"] * debye_site_words(st) * esz
     h = ch * (9 * esz + coef_h)
@@ -95,10 +107,11 @@ def measure(mesh, prec, steps, warmup):
         e_ms, h_ms = ds.kernel_ms()
     esz = 8 if prec == "fp64" else 4
     eb, hb = kernel_bytes(st, esz)
+    names = ("k_leap", "k_h_fix") if st.get("leap") else ("k_e_fast", "k_h_fast")
     if e_ms >= h_ms:
-        name, kb, kms = "k_e_fast", eb, e_ms
+        name, kb, kms = names[0], eb, e_ms
     else:
-        name, kb, kms = "k_h_fast", hb, h_ms
+        name, kb, kms = names[1], hb, h_ms
     ms_step = total_ms / steps
     achieved = kb / (kms * 1e-3) / 1e9
     sb = step_bytes(st, esz)
@@ -215,8 +228,10 @@ def main():
     res = {}
     for prec in (["fp64", "fp32"] if args.precision == "both" else [args.precision]):
         res[prec] = measure(mesh, prec, args.steps, args.warmup)
+        rf = res[prec]["roofline"]
         print(f"# {prec}: {res[prec]['value']:.1f} Mcell-updates/s, {res[prec]['ms_per_step']:.3f} ms/step, "
-              f"setup {res[prec]['setup_s']} s", file=sys.stderr, flush=True)
+              f"setup {res[prec]['setup_s']} s; volume kernels {rf['e_ms']:.4f} / {rf['h_ms']:.4f} ms, "
+              f"fix cells {res[prec]['stats'].get('fix_cells', 0)}", file=sys.stderr, flush=True)
     head = res.get("fp32") or res["fp64"]
     st = head["stats"]
     out = {
@@ -228,7 +243,9 @@ def main():
                                f"spheres, aux-grid TF/SF plane wave, 8 DFT observers (seed 3)",
                    "cells": st["cells"], "mesh_cells": st["mesh_cells"], "debye_sites": st["debye_sites"],
                    "device_bytes": st["bytes"], "coefficients": "16-bit classes" if st["indexed_media"] else
-                   "per-cell arrays", "debye_fused_sites": st["debye_fused_sites"]},
+                   "per-cell arrays", "debye_fused_sites": st["debye_fused_sites"],
+                   "sweep": "fused leapfrog" if st.get("leap") else "separate E and H sweeps",
+                   "fix_cells": st.get("fix_cells", 0)},
         "roofline": head["roofline"], "step_roofline": head["step_roofline"],
     }
     if "fp64" in res and "fp32" in res:

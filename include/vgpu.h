@@ -188,7 +188,9 @@ typedef struct {
   int indexed_media;           /* 1: coefficients read as 16-bit classes */
   int debye_fused;             /* 1: first Debye layer fused into the E sweep */
   int sibc_state;              /* SIBC filter state: 0 complex per filter, 1 real per filter, 2 real per input */
-  int pad_;
+  int leap;                    /* 1: fused leapfrog sweep (E then H in one pass over the fast box) */
+  uint64_t fix_cells;          /* H cells of the fused box updated again after the E corrections */
+  uint64_t fix_chunks;         /* the 64-cell row chunks holding them */
 } vgpu_stats;
 int vgpu_get_stats(const vgpu_ctx *c, vgpu_stats *out);
 

@@ -147,6 +147,45 @@ static int check_sweep(const char *mesh, const A &a, const int (*in)[NB][6], con
   return 0;
 }
 
+// The fused sweep's tiling (k_leap): tile tx owns x-cells box[0] + 63 tx +
+// lane for lanes 0..62, wavefront w of y-block ty owns rows box[2] +
+// (4 ty + w) R + r for r < R, z chunk tz owns planes box[4] + kz tz + p for
+// p < kz, each clipped to the box.  Every cell of the box must be owned
+// exactly once, for each row count R and chunk depth kz the library uses.
+static int check_leap_tiles(const char *mesh, const int *fb, const int N[3])
+{
+  if (fb[0] > fb[1] || fb[2] > fb[3] || fb[4] > fb[5]) return 0;
+  if (fb[0] < 1 || fb[2] < 1 || fb[4] < 1 || fb[1] > N[0] - 2 || fb[3] > N[1] - 2 || fb[5] > N[2] - 2)
+    return fail(mesh, "fused box not inside the grid", fb[0], fb[2], fb[4], -1);
+  const int nx = fb[1] - fb[0] + 1, ny = fb[3] - fb[2] + 1, nz = fb[5] - fb[4] + 1;
+  for (int R : {2, 4})
+    for (int kz0 : {1, 8, 16}) {
+      const int kz = std::min(kz0, nz);
+      const int ntx = (nx + 62) / 63, ngy = (ny + 4 * R - 1) / (4 * R), ntz = (nz + kz - 1) / kz;
+      std::vector<unsigned char> seen((size_t)nx * ny * nz, 0);
+      for (int tz = 0; tz < ntz; ++tz)
+        for (int ty = 0; ty < ngy; ++ty)
+          for (int w = 0; w < 4; ++w)
+            for (int tx = 0; tx < ntx; ++tx) {
+              const int jb = fb[2] + (ty * 4 + w) * R, kb = fb[4] + tz * kz;
+              if (jb > fb[3] || kb > fb[5]) continue;
+              const int ke = std::min(kb + kz, fb[5] + 1);
+              for (int k = kb; k < ke; ++k)
+                for (int r = 0; r < R && jb + r <= fb[3]; ++r)
+                  for (int lane = 0; lane < 63; ++lane) {
+                    const int i = fb[0] + 63 * tx + lane;
+                    if (i > fb[1]) break;
+                    unsigned char &s = seen[((size_t)(k - fb[4]) * ny + (jb + r - fb[2])) * nx + (i - fb[0])];
+                    if (s) return fail(mesh, "fused tile cell owned twice", i, jb + r, k, R);
+                    s = 1;
+                  }
+            }
+      for (size_t q = 0; q < seen.size(); ++q)
+        if (!seen[q]) return fail(mesh, "fused tile cell not owned", (int)(q % nx), (int)(q / nx % ny), (int)(q / nx / ny), R);
+    }
+  return 0;
+}
+
 int main(int argc, char **argv)
 {
   if (argc < 3) {
@@ -173,16 +212,24 @@ int main(int argc, char **argv)
     const int N[3] = {c->NX, c->NY, c->NZ};
     EArgs<double> ea{};
     HArgs<double> ha{};
-    fill_volume_args<double>(c, ea, ha);
+    fill_volume_args<double>(c, ea, ha, 0);
     int ein[3][1][6];
     for (int C = 0; C < 3; ++C)
       for (int q = 0; q < 6; ++q) ein[C][0][q] = ea.in[C][q];
     long long pe = 0, ph = 0, se = 0, sh = 0;
-    // exact fast boxes, and boxes cut to whole lines of FP64 and FP32 elements
-    for (int lw : {0, 16, 32}) {
-      fast_boxes(c->grid.gfilim, N, c->fastE, c->fastH, lw);
-      if (check_sweep<1>(argv[m], ea, ein, c->fastE, N, pe, se, lw) ||
-          check_sweep<3>(argv[m], ha, ha.in, c->fastH, N, ph, sh, lw))
+    // exact fast boxes, boxes cut to whole lines of FP64 and FP32 elements,
+    // and the fused box of the leapfrog sweep (the same box for E and H)
+    for (int lw : {0, 16, 32, -1}) {
+      if (lw >= 0) {
+        fast_boxes(c->grid.gfilim, N, c->fastE, c->fastH, lw);
+      } else {
+        leap_box(c->grid.gfilim, N, c->fastE);
+        memcpy(c->fastH, c->fastE, sizeof(c->fastH));
+        if (check_leap_tiles(argv[m], c->fastE, N)) return 1;
+      }
+      const int rw = lw < 0 ? 32 : lw;
+      if (check_sweep<1>(argv[m], ea, ein, c->fastE, N, pe, se, rw) ||
+          check_sweep<3>(argv[m], ha, ha.in, c->fastH, N, ph, sh, rw))
         return 1;
     }
     printf("%s OK cells %lld E pieces %lld shell %lld H pieces %lld shell %lld\n", argv[m],

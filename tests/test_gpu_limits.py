@@ -5,8 +5,8 @@ build (tests/golden/limits_unvisited.json.gz, one-step runs):
   - no cell is updated twice and no cell outside the limits is updated;
   - the cells left un-updated are exactly the ones the reference leaves;
 in both precisions, with the fast boxes exact or cut to whole lines, with
-per-cell coefficient arrays, and at the full 256^3 Debye size.  The check
-changes no field."""
+per-cell coefficient arrays, with the fused leapfrog sweep, and at the full
+256^3 Debye size.  The check changes no field."""
 import os
 import shutil
 
@@ -62,7 +62,7 @@ def test_update_once_fp32(key, tmp_path):
     _check(_stage(key, tmp_path), FX[key], "fp32")
 
 
-@pytest.mark.parametrize("env", [("VGPU_ALIGN_FAST", "0"), ("VGPU_NO_INDEXED_MEDIA", "1")])
+@pytest.mark.parametrize("env", [("VGPU_ALIGN_FAST", "0"), ("VGPU_NO_INDEXED_MEDIA", "1"), ("VGPU_LEAP", "1")])
 @pytest.mark.parametrize("key", ["gen_debye_pml", "gen_sibc_box", "gen_enclosure",
                                  "freespace_mur/freespace_mur.mesh", "parplate1D_debyepml_zyx/parplate1D_debyepml_zyx.mesh"])
 def test_update_once_variants(key, env, tmp_path, monkeypatch):
@@ -70,14 +70,16 @@ def test_update_once_variants(key, env, tmp_path, monkeypatch):
     _check(_stage(key, tmp_path), FX[key], "fp32")
 
 
-def test_full_size_update_once_and_fields_unchanged(tmp_path):
+@pytest.mark.parametrize("leap", ["0", "1"])
+def test_full_size_update_once_and_fields_unchanged(leap, tmp_path, monkeypatch):
     """256^3 Debye volume with UPML (no Mur faces): every cell of the outer
-    limits updated exactly once, in both precisions; the fields after a
-    check equal the fields before it."""
+    limits updated exactly once, in both precisions, with the separate and
+    the fused sweeps; the fields after a check equal the fields before it."""
+    monkeypatch.setenv("VGPU_LEAP", leap)
     mesh = tmp_path / "d.mesh"
     mesh.write_text(meshgen.debye(steps=20))
     for prec in ("fp32", "fp64"):
-        print(f"case debye256 {prec}", flush=True)
+        print(f"case debye256 {prec} leap {leap}", flush=True)
         with vulture_amd.DeviceSolve(str(mesh), precision=prec) as ds:
             ds.run(0, 10)
             ds.synchronize()

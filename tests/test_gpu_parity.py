@@ -207,6 +207,37 @@ def test_chunked_runs(name, chunk, tmp_path):
         assert np.linalg.norm(fa - fb) <= 1e-6 * max(np.linalg.norm(fb), 1e-30)
 
 
+LEAP_MESHES = ["dgref_pml6", "freespace_mur", "closedbox_go04", "parplate1D_debyepml_xyz", "planewave_empty_xyz",
+               "parplate1D_planewave_xyz", "hardsrc_ez", "parplate1D_sibcspar_xyz", "periodic_planewave_xyz"]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("name", [m for m in LEAP_MESHES if os.path.isdir(os.path.join(INP, m))])
+def test_fused_leapfrog(name, precision, tmp_path, monkeypatch):
+    """The fused leapfrog sweep (k_leap, with the H cells next to corrected E
+    updated again by k_h_fix) gives the oracle's results: FP64 to 1e-12,
+    FP32 to the FP32 bars.  Meshes with UPML, Mur, SIBC, Debye, sources,
+    TF/SF and periodic ghosts."""
+    monkeypatch.setenv("VGPU_LEAP", "1")
+    mesh = _stage(name, tmp_path / "in")
+    r = _run_pair(mesh, tmp_path, precision)
+    if precision == "fp64":
+        _check(r, "fp64")
+    else:
+        oracle.run_mesh(mesh, str(tmp_path / "oracle32"), precision="fp32")
+        ref = compare_outputs(str(tmp_path / "oracle32"), str(tmp_path / "oracle"))
+        assert r["files"] > 0
+        assert r["td"] <= max(1e-4, 1.5 * ref["td"]), (r, ref)
+        assert r["fd_db"] <= max(0.05, 1.5 * ref["fd_db"]), (r, ref)
+
+
+@pytest.mark.parametrize("name,kw", WORKLOADS[1:4], ids=["debye32", "debye64", "enclosure"])
+def test_fused_leapfrog_workload(name, kw, tmp_path, monkeypatch):
+    monkeypatch.setenv("VGPU_LEAP", "1")
+    mesh = meshgen.write(name, str(tmp_path), **kw)
+    _check(_run_pair(mesh, tmp_path, "fp64"), "fp64")
+
+
 @pytest.mark.parametrize("name", ["parplate1D_debye_xyz", "parplate1D_debyepml_xyz", "parplate1D_debyeslab_zyx"])
 def test_fused_debye_layer(name, tmp_path, monkeypatch):
     """The first Debye layer applied inside the fast E kernel (numbered by a

@@ -113,7 +113,7 @@ class Stats(ctypes.Structure):
                 ("correction_rows", ctypes.c_uint64 * 2), ("bytes", ctypes.c_uint64),
                 ("fast_cells", ctypes.c_uint64 * 2), ("debye_fused_sites", ctypes.c_uint64),
                 ("indexed_media", ctypes.c_int), ("debye_fused", ctypes.c_int), ("sibc_state", ctypes.c_int),
-                ("pad_", ctypes.c_int)]
+                ("leap", ctypes.c_int), ("fix_cells", ctypes.c_uint64), ("fix_chunks", ctypes.c_uint64)]
 
 
 class LimitsReport(ctypes.Structure):

@@ -211,9 +211,25 @@ int vgpu_create(const vgpu_grid_desc *g, vgpu_prec prec, int device, vgpu_ctx **
   c->grid.idhx = c->grid.idhy = c->grid.idhz = nullptr;
   {
     const int N[3] = {c->NX, c->NY, c->NZ};
-    // x range of the fast boxes on whole 128-byte lines (VGPU_ALIGN_FAST=0 keeps the exact boxes)
-    const char *al = getenv("VGPU_ALIGN_FAST");
-    fast_boxes(g->gfilim, N, c->fastE, c->fastH, al && al[0] == '0' ? 0 : (int)(128 / c->esz));
+    // The fused leapfrog sweep with VGPU_LEAP=1 (not with the fused Debye
+    // layer, VGPU_DEBYE_FUSION=1, which only the separate sweeps have; its
+    // kernel indexes cells with 32-bit offsets).  The separate E and H
+    // sweeps are the default: on the Debye volume at 256^3 the fused sweep
+    // and the H cells it leaves to k_h_fix measured slower.
+    const char *lp = getenv("VGPU_LEAP"), *df = getenv("VGPU_DEBYE_FUSION");
+    c->leap = lp && lp[0] == '1' && !(df && df[0] == '1') && c->ncell + FIELD_DISCARD < (1ull << 31);
+    if (c->leap) {
+      int fb[6];
+      leap_box(g->gfilim, N, fb);
+      for (int q = 0; q < 6; ++q) c->fastE[q] = c->fastH[q] = fb[q];
+      c->leapR = 4;
+      if (const char *ev = getenv("VGPU_LEAP_ROWS")) c->leapR = atoi(ev) <= 1 ? 1 : atoi(ev) == 2 ? 2 : 4;  // tuning override
+      if (const char *ev = getenv("VGPU_LEAP_KZ")) c->leapKZ = std::max(1, std::min(atoi(ev), 64));
+    } else {
+      // x range of the fast boxes on whole 128-byte lines (VGPU_ALIGN_FAST=0 keeps the exact boxes)
+      const char *al = getenv("VGPU_ALIGN_FAST");
+      fast_boxes(g->gfilim, N, c->fastE, c->fastH, al && al[0] == '0' ? 0 : (int)(128 / c->esz));
+    }
   }
   c->NCH = c->fastE[0] <= c->fastE[1] ? (c->fastE[1] - c->fastE[0] + 64) / 64 : 0;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -223,8 +239,9 @@ int vgpu_create(const vgpu_grid_desc *g, vgpu_prec prec, int device, vgpu_ctx **
   hipEventCreate(&c->ev0);
   hipEventCreate(&c->ev1);
   for (int d = 0; d < 6; ++d) {
-    c->F[d] = dev_alloc(c, c->ncell * c->esz);
-    if (!c->F[d]) {
+    c->F[d] = dev_alloc(c, (c->ncell + FIELD_DISCARD) * c->esz);
+    c->F1[d] = c->leap ? dev_alloc(c, (c->ncell + FIELD_DISCARD) * c->esz) : c->F[d];
+    if (!c->F[d] || !c->F1[d]) {
       vgpu_destroy(c);
       return VGPU_ENOMEM;
     }
@@ -528,6 +545,8 @@ static int build_debye(Ctx *c)
           }
     }
   seen.clear();
+  if (c->leap)
+    for (const Site &st : sites) c->eTouched.push_back(st.cell * 3 + st.comp);
   std::sort(sites.begin(), sites.end(), [](const Site &a, const Site &b) {
     if (a.layer != b.layer) return a.layer < b.layer;
     if (a.comp != b.comp) return a.comp < b.comp;
@@ -715,6 +734,19 @@ static int build_sibc(Ctx *c)
               f0 + ((long long)(k - d.gbbox[4]) * nj + (j - d.gbbox[2])) * ni + (i - d.gbbox[0]);
           ijk[3 * f] = i; ijk[3 * f + 1] = j; ijk[3 * f + 2] = k;
           surf[f] = s;
+          if (c->leap) {
+            // the tangential E that k_sibc_e sets to zero on this face
+            const long long id = c->lin(i, j, k), P = c->plane, SJ = c->PX;
+            const int t1 = (d.normal + 1) % 3, t2 = (d.normal + 2) % 3;
+            // each tangential component at the face cell and at its next
+            // cell along the other tangential axis
+            const long long o1 = d.normal == 0 ? P : d.normal == 1 ? 1 : SJ;
+            const long long o2 = d.normal == 0 ? SJ : d.normal == 1 ? P : 1;
+            c->eTouched.push_back(id * 3 + t1);
+            c->eTouched.push_back((id + o1) * 3 + t1);
+            c->eTouched.push_back(id * 3 + t2);
+            c->eTouched.push_back((id + o2) * 3 + t2);
+          }
           const unsigned char *a = &c->sibc[s].adjA[4 * lf], *b = &c->sibc[s].adjB[4 * lf];
           for (int r = 0; r < 4; ++r) { adjA[4 * f + r] = a[r]; adjB[4 * f + r] = b[r]; }
           // terms in sibc.c:638-757 order: (comp, di, dj, dk, sign, weight dir/index, Etan slot, id index)
@@ -979,6 +1011,8 @@ static int build_corrections(Ctx *c)
             }
       }
     }
+  if (c->leap)
+    for (const RowEntry &r : rows[0]) c->eTouched.push_back(r.key);
   for (int e = 0; e < 2; ++e) {
     std::stable_sort(rows[e].begin(), rows[e].end(), [](const RowEntry &a, const RowEntry &b) { return a.key < b.key; });
     std::vector<int> rstart;
@@ -1097,6 +1131,60 @@ static int build_observers(Ctx *c)
   return VGPU_OK;
 }
 
+// H cells of the fused box that the fused sweep cannot finish: those with an
+// E input that is not an ordinary update (outside its component's inner-grid
+// limits, so a UPML, Mur or ghost value) or that is changed after the E
+// sweep (a Debye site, a source or TF/SF row, a zeroed SIBC face edge).  Hx
+// reads Ey and Ez at its cell and Ez at j+1, Ey at k+1; Hy reads Ez, Ex and
+// Ez at i+1, Ex at k+1; Hz reads Ex, Ey and Ey at i+1, Ex at j+1
+// (grid.c:800-888).  They go to k_h_fix as 64-cell row chunks with masks.
+static int build_leap(Ctx *c)
+{
+  if (!c->leap) return VGPU_OK;
+  const int *fb = c->fastE;
+  std::vector<unsigned char> ord(c->ncell, 0);
+  for (int C = 0; C < 3; ++C) {
+    const int *l = c->grid.gfilim[C];
+    for (int k = std::max(l[4], 0); k <= std::min(l[5], c->NZ - 1); ++k)
+      for (int j = std::max(l[2], 0); j <= std::min(l[3], c->NY - 1); ++j)
+        for (int i = std::max(l[0], 0); i <= std::min(l[1], c->NX - 1); ++i) ord[c->lin(i, j, k)] |= 1u << C;
+  }
+  for (long long key : c->eTouched) {
+    const long long cell = key / 3;
+    if (cell >= 0 && cell < (long long)c->ncell) ord[cell] &= ~(1u << (key % 3));
+  }
+  std::vector<long long>().swap(c->eTouched);
+  std::vector<FixEnt> ents;
+  const long long P = c->plane, SJ = c->PX;
+  long long cells = 0;
+  for (int k = fb[4]; k <= fb[5]; ++k)
+    for (int j = fb[2]; j <= fb[3]; ++j) {
+      FixEnt cur{-1, j, k, 0, 0ull};
+      for (int i = fb[0]; i <= fb[1]; ++i) {
+        const long long id = c->lin(i, j, k);
+        const bool ok = (ord[id] & 7u) == 7u && (ord[id + 1] & 6u) == 6u && (ord[id + SJ] & 5u) == 5u &&
+                        (ord[id + P] & 3u) == 3u;
+        if (ok) continue;
+        const int i0 = i / 64 * 64;
+        if (cur.i0 != i0) {
+          if (cur.mask) ents.push_back(cur);
+          cur.i0 = i0;
+          cur.mask = 0;
+        }
+        cur.mask |= 1ull << (i - i0);
+        ++cells;
+      }
+      if (cur.mask) ents.push_back(cur);
+    }
+  c->nFix = (long long)ents.size();
+  c->fixCells = cells;
+  if (!ents.empty()) {
+    c->d_fix = upload_vec(c, ents);
+    if (!c->d_fix) return set_error(c, VGPU_ENOMEM, "fused sweep fix-up list");
+  }
+  return VGPU_OK;
+}
+
 int vgpu_finalize(vgpu_ctx *c, uint64_t total_steps, uint64_t max_steps_per_run)
 {
   if (!c || c->finalized) return VGPU_ESTATE;
@@ -1107,7 +1195,7 @@ int vgpu_finalize(vgpu_ctx *c, uint64_t total_steps, uint64_t max_steps_per_run)
   if (!c->d_step) return set_error(c, VGPU_ENOMEM, "step counter");
   int r;
   if ((r = build_debye(c)) || (r = build_class_tables(c)) || (r = build_sibc(c)) || (r = build_corrections(c)) ||
-      (r = build_observers(c)))
+      (r = build_observers(c)) || (r = build_leap(c)))
     return r;
   if ((r = c->prec == VGPU_FP64 ? prepare_sweeps<double>(c) : prepare_sweeps<float>(c))) return r;
   if (hipDeviceSynchronize() != hipSuccess) return set_error(c, VGPU_EDEVICE, "device error during set-up");
@@ -1142,6 +1230,7 @@ static void capture_graph(Ctx *c, uint64_t nsteps)
   c->graph = g;
   c->graphExec = x;
   c->graphSteps = nsteps;
+  c->graphPar = c->par;
 }
 
 int vgpu_run(vgpu_ctx *c, uint64_t first, uint64_t nsteps, void *stream)
@@ -1163,11 +1252,13 @@ int vgpu_run(vgpu_ctx *c, uint64_t first, uint64_t nsteps, void *stream)
   // diverged and the device faulted, so it stays off until that interaction
   // is understood.
   const char *ge = getenv("VGPU_GRAPHS");
+  // (a graph of an even number of steps starts and ends on the same field
+  // set, so it replays wherever that set is current)
   const bool graphable = ge && ge[0] == '1' && s == c->stream && c->evE.empty() && nsteps >= 2 &&
-                         nsteps == c->maxRun && c->totalSteps >= 2 * nsteps;
+                         nsteps % 2 == 0 && nsteps == c->maxRun && c->totalSteps >= 2 * nsteps;
   int r = VGPU_OK;
   if (graphable && !c->graphExec && !c->graphTried) capture_graph(c, nsteps);
-  if (graphable && c->graphExec && c->graphSteps == nsteps) {
+  if (graphable && c->graphExec && c->graphSteps == nsteps && c->graphPar == c->par) {
     if (hipGraphLaunch(c->graphExec, s) != hipSuccess)
       r = set_error(c, VGPU_EDEVICE, "graph launch failed: %s", hipGetErrorString(hipGetLastError()));
     c->lastRunTimed = false;
@@ -1176,6 +1267,7 @@ int vgpu_run(vgpu_ctx *c, uint64_t first, uint64_t nsteps, void *stream)
     c->lastRunTimed = true;
   }
   hipEventRecord(c->ev1, s);
+  if (c->leap) c->par ^= (int)(nsteps & 1);  // the current fields are in the set the last step wrote
   c->stepHost = first + nsteps;
   c->lastRunFirst = first;
   c->lastRunSteps = nsteps;
@@ -1252,7 +1344,7 @@ int vgpu_download_field(vgpu_ctx *c, int component, double *host)
   if (!c || component < 0 || component > 5 || !host) return VGPU_EINVAL;
   hipStreamSynchronize(c->stream);
   std::vector<unsigned char> buf(c->ncell * c->esz);
-  if (hipMemcpy(buf.data(), c->F[component], buf.size(), hipMemcpyDeviceToHost) != hipSuccess)
+  if (hipMemcpy(buf.data(), c->fld(c->par, component), buf.size(), hipMemcpyDeviceToHost) != hipSuccess)
     return set_error(c, VGPU_EDEVICE, "download failed");
   for (int i = 0; i < c->NX; ++i)
     for (int j = 0; j < c->NY; ++j)
@@ -1305,6 +1397,9 @@ int vgpu_get_stats(const vgpu_ctx *c, vgpu_stats *out)
   out->indexed_media = c->indexed ? 1 : 0;
   out->debye_fused = c->debyeFused ? 1 : 0;
   out->sibc_state = c->sibcFactored ? 2 : c->sibcReal ? 1 : 0;
+  out->leap = c->leap ? 1 : 0;
+  out->fix_cells = (uint64_t)c->fixCells;
+  out->fix_chunks = (uint64_t)c->nFix;
   return VGPU_OK;
 }
 

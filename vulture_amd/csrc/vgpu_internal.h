@@ -5,6 +5,15 @@
 // boundaries.  Extents NX,NY,NZ are the reference's allocation extents
 // (gobox plus one ghost layer each side), so grid indices i,j,k are the
 // reference's own.
+//
+// With the fused leapfrog sweep the fields are held twice.  A step reads the
+// fields of step n from one set and writes those of step n+1 to the other
+// (the sweeps, Mur E); the corrections after each sweep (Debye, SIBC,
+// sources, TF/SF, ghosts) then modify the new set in place.  Nothing a
+// kernel reads from the old set is written during the step, so a sweep may
+// recompute a neighbour's new value from the old fields at any time, in any
+// block, without racing the block that stores it.  The separate sweeps
+// update one set in place, as the reference does.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -22,8 +31,9 @@ namespace vgpu {
 template <typename T> struct EArgs {
   int NX, NY, NZ, PX;
   long long plane;
-  T *E[3];
-  const T *H[3];
+  T *E[3];               // new E (written)
+  const T *Ein[3];       // old E (read)
+  const T *H[3];         // old H
   const T *alpha[3], *beta[3];
   const T *idh[3];
   int in[3][6];          // inner-grid limits per E component
@@ -50,8 +60,9 @@ template <typename T> struct EArgs {
 template <typename T> struct HArgs {
   int NX, NY, NZ, PX;
   long long plane;
-  T *H[3];
-  const T *E[3];
+  T *H[3];               // new H (written)
+  const T *Hin[3];       // old H (read)
+  const T *E[3];         // new E
   const T *gamma[3];
   const T *ide[3];
   int in[3][3][6];       // ordinary-update boxes per H component (inner grid, two Mur faces)
@@ -72,8 +83,9 @@ template <typename T> struct FastE {
   int box[6];            // fast box (every E component inner-grid)
   int kchunk, nchx;      // planes per block; 64-cell x chunks from box[0]
   int gy, gz;            // y blocks (blockDim.y rows), z chunks (blocks = nchx*gy*gz, launched in XCD order)
-  T *E[3];
-  const T *H[3];
+  T *E[3];               // new E (written)
+  const T *Ein[3];       // old E
+  const T *H[3];         // old H
   const T *idh[3];
   const uint16_t *ecls;  // coefficient classes
   const T *etab;
@@ -89,13 +101,62 @@ template <typename T> struct FastH {
   int box[6];
   int kchunk, nchx;
   int gy, gz;
-  T *H[3];
-  const T *E[3];
+  T *H[3];               // new H (written)
+  const T *Hin[3];       // old H
+  const T *E[3];         // new E
   const T *ide[3];
   const uint16_t *hcls;
   const T *htab;
   const T *gamma[3];
   unsigned *vis[3];      // update-once check (null in ordinary runs)
+};
+
+// Fused leapfrog sweep over the fused box (E and H of every cell inside take
+// the ordinary update).  A wave owns a tile of 63 x-cells by R rows and
+// marches over kz planes; it computes E on 64 x 1+R cells per plane (its
+// tile and the x and y neighbours its H needs) and H of the plane below.
+template <typename T> struct LeapArgs {
+  int NX, NY, NZ, PX;
+  long long plane;
+  int box[6];            // fused box
+  int ntx, ngy, ntz, kz; // x tiles (63 cells), y tiles (4 waves of R rows per block), z chunks of kz planes
+  T *E[3];               // new E, H (written)
+  T *H[3];
+  const T *Ein[3];       // old E, H
+  const T *Hin[3];
+  const T *idh[3], *ide[3];
+  const uint16_t *ecls, *hcls;  // coefficient classes
+  const T *etab, *htab;
+  const T *alpha[3], *beta[3], *gamma[3];  // per-cell arrays when there are no classes
+  int discard;           // first element of the discard area past every field array's cells
+  unsigned *vis[6];      // update-once check (null in ordinary runs)
+};
+
+// Elements past the cells of every field array that the fused sweep writes
+// the values of cells a lane does not own to (never read).
+constexpr int FIELD_DISCARD = 65536;
+
+// H cells of the fused box whose new value the fused sweep cannot give
+// (an E input of theirs is corrected after the sweep -- a Debye site, a
+// source or TF/SF row, an SIBC face -- or is not an ordinary update): one
+// 64-cell x chunk of one row per entry, with a mask of its cells.  They are
+// updated again after the E corrections, from the old H and the final E.
+struct FixEnt {
+  int i0, j, k, pad_;    // first cell of the chunk
+  unsigned long long mask;  // cells i0 + lane to update
+};
+template <typename T> struct FixArgs {
+  long long n;
+  const FixEnt *ent;
+  int NY, PX;
+  long long plane;
+  T *H[3];
+  const T *Hin[3];
+  const T *E[3];
+  const T *ide[3];
+  const uint16_t *hcls;
+  const T *htab;
+  const T *gamma[3];
 };
 
 // A piece of a sweep's shell (the cells outside its fast box): a box whose
@@ -134,8 +195,9 @@ struct ShellWork {
 template <typename T> struct MurArgs {
   int NX, NY, PX;
   long long plane;
-  T *E[3];
-  const T *H[3];
+  T *E[3];               // new E (written on the face)
+  const T *Ein[3];       // old E
+  const T *H[3];         // old H
   const T *alpha[3], *beta[3];
   const T *idh[3];
   int box[12][6];        // limits of face f component slot s at [2f+s]
@@ -366,6 +428,21 @@ inline void fast_boxes(const int gfilim[6][6], const int N[3], int fastE[6], int
   }
 }
 
+// Fused box of the leapfrog sweep: where E and H both take the ordinary
+// update, its x range cut to whole 63-cell tiles when that leaves at most 16
+// columns to the shells.
+inline void leap_box(const int gfilim[6][6], const int N[3], int fb[6])
+{
+  int fe[6], fh[6];
+  fast_boxes(gfilim, N, fe, fh, 0);
+  for (int a = 0; a < 3; ++a) {
+    fb[2 * a] = fe[2 * a] > fh[2 * a] ? fe[2 * a] : fh[2 * a];
+    fb[2 * a + 1] = fe[2 * a + 1] < fh[2 * a + 1] ? fe[2 * a + 1] : fh[2 * a + 1];
+  }
+  const int w = fb[1] - fb[0] + 1;
+  if (w >= 63 && w % 63 <= 16) fb[1] = fb[0] + w / 63 * 63 - 1;
+}
+
 struct Ctx {
   std::string err;
   vgpu_prec prec;
@@ -385,7 +462,12 @@ struct Ctx {
 
   // device allocations
   std::vector<void *> allocs;
-  void *F[6] = {};
+  // the field sets (Ex..Hz each): set `par` holds the current fields; with
+  // the fused sweep there are two and every step flips it, otherwise F1 is F
+  void *F[6] = {}, *F1[6] = {};
+  int par = 0;
+  void *fld(int set, int d) const { return set ? F1[d] : F[d]; }
+  int next(int set) const { return leap ? set ^ 1 : set; }  // the set a step writes
   void *alpha[3] = {}, *beta[3] = {}, *gamma[3] = {};
   void *ide[3] = {}, *idh[3] = {};
   bool media_set = false;
@@ -405,9 +487,16 @@ struct Ctx {
   // layer-0 Debye sites handled inside the E sweep
   bool debyeFused = false;
   long long debyeL0 = 0;
+  // fused leapfrog sweep: the fast boxes are its fused box, H cells to
+  // update again after the E corrections
+  bool leap = false;
+  int leapR = 4, leapKZ = 8;
+  long long nFix = 0, fixCells = 0;
+  void *d_fix = nullptr;
+  std::vector<long long> eTouched;           // E values (cell * 3 + component) changed after the E sweep
   int NCH = 0;                               // 64-cell x chunks of the fast E box
   int fastE[6] = {}, fastH[6] = {};          // boxes of the sweeps' fast kernels
-  void *d_eargs = nullptr, *d_hargs = nullptr;  // argument blocks in device memory
+  void *d_eargs[2] = {}, *d_hargs[2] = {};   // argument blocks in device memory, by the set read
   void *d_epcs = nullptr, *d_ework = nullptr;   // E shell: pieces, launch blocks
   void *d_hpcs = nullptr, *d_hwork = nullptr;   // H shell
   int nEwork = 0, nHwork = 0, nEpcs = 0, nHpcs = 0;
@@ -483,6 +572,7 @@ struct Ctx {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graphExec = nullptr;
   uint64_t graphSteps = 0;
+  int graphPar = 0;                          // field set current when the graph was captured
   bool graphTried = false, lastRunTimed = false;
   void *d_ocell = nullptr, *d_ofirst = nullptr, *d_olast = nullptr, *d_onf = nullptr, *d_ooff = nullptr;
   void *d_oomega = nullptr, *d_otd = nullptr, *d_odre = nullptr, *d_odim = nullptr;

@@ -6,6 +6,7 @@
 // ordinary update, and a shell kernel for the rest (see below).
 #pragma once
 #include "vgpu_internal.h"
+#include <type_traits>
 
 namespace vgpu {
 
@@ -190,7 +191,7 @@ VG_DEV T e_fast(const FastE<T> &f, int bx, int j, int k, bool vx, long long id, 
     al = f.alpha[C][id];
     be = f.beta[C][id];
   }
-  T e = al * f.E[C][id] + be * curl;
+  T e = al * f.Ein[C][id] + be * curl;
   if (FUSE) debye_fused<T, C>(f, bx, j, k, vx, cls, be, e);
   return e;
 }
@@ -270,6 +271,9 @@ template <typename T, bool IDX, bool CHK = false> __global__ __launch_bounds__(5
   T *__restrict__ H0 = f.H[0];
   T *__restrict__ H1 = f.H[1];
   T *__restrict__ H2 = f.H[2];
+  const T *__restrict__ I0 = f.Hin[0];
+  const T *__restrict__ I1 = f.Hin[1];
+  const T *__restrict__ I2 = f.Hin[2];
   const T idex = f.ide[0][ic], idey = f.ide[1][j];
   long long id = ((long long)kb * f.NY + j) * SJ + ic;
   // The fast box ends at index N-2 or less, so i+1, j+1 and k+1 are inside.
@@ -287,9 +291,9 @@ template <typename T, bool IDX, bool CHK = false> __global__ __launch_bounds__(5
       g1 = f.gamma[1][id];
       g2 = f.gamma[2][id];
     }
-    const T h0 = H0[id] + g0 * (idez * (eyp1 - ey) + idey * (ez - ezy));
-    const T h1 = H1[id] + g1 * (idex * (ezx - ez) + idez * (ex - exp1));
-    const T h2 = H2[id] + g2 * (idey * (exy - ex) + idex * (ey - eyx));
+    const T h0 = I0[id] + g0 * (idez * (eyp1 - ey) + idey * (ez - ezy));
+    const T h1 = I1[id] + g1 * (idex * (ezx - ez) + idez * (ex - exp1));
+    const T h2 = I2[id] + g2 * (idey * (exy - ex) + idex * (ey - eyx));
     if (vx) {
       if (CHK) {
         visit(f.vis[0], id);
@@ -304,6 +308,243 @@ template <typename T, bool IDX, bool CHK = false> __global__ __launch_bounds__(5
     ex = exp1;
     ey = eyp1;
   }
+}
+
+// ===========================================================================
+// Fused leapfrog sweep (grid.c:709-797 then 800-888, in one pass).  A wave
+// owns a tile of 63 x-cells by R rows of the fused box and marches over kz
+// planes.  At plane k it computes new E of its tile's 64 x-cells on R + 1
+// rows (its own cells, the first cell of the next tile and the first row of
+// the next tile, which its H needs), stores E of the cells it owns, and then
+// updates H of plane k-1 from the new E of planes k-1 and k.  Every input is
+// an old field, never written during the step, so a neighbour's new E is
+// recomputed here bit for bit as its owner computes it, without waiting for
+// it.  The E plane above the chunk (plane ke) is computed once more as the
+// first plane of the next chunk.  One E and one H update per cell use the
+// operands and operation order of the two separate sweeps, so the results
+// are the same bits.  H cells whose E inputs are changed after the E sweep
+// are updated again by k_h_fix.
+// ===========================================================================
+template <typename T, bool IDX, int C>
+VG_DEV void leap_ecoef(const LeapArgs<T> &a, unsigned cls, int id, T &al, T &be)
+{
+  if (IDX) {
+    load_pair(a.etab + cls * ETAB_STRIDE + 2 * C, al, be);
+  } else {
+    al = a.alpha[C][id];
+    be = a.beta[C][id];
+  }
+}
+
+template <typename T, bool IDX>
+VG_DEV void leap_gammas(const LeapArgs<T> &a, unsigned cls, int id, T &g0, T &g1, T &g2)
+{
+  if (IDX) {
+    load_gammas(a.htab + cls * HTAB_STRIDE, g0, g1, g2);
+  } else {
+    g0 = a.gamma[0][id];
+    g1 = a.gamma[1][id];
+    g2 = a.gamma[2][id];
+  }
+}
+
+template <bool B> using Flag = std::integral_constant<bool, B>;
+
+template <typename T, bool IDX, int R, bool CHK = false>
+__global__ __launch_bounds__(256) void k_leap(LeapArgs<T> a)
+{
+  const unsigned L = xcd_block();
+  if (L >= (unsigned)(a.ntx * a.ngy * a.ntz)) return;
+  const int tx = (int)(L % a.ntx), ty = (int)((L / a.ntx) % a.ngy), tz = (int)(L / (a.ntx * a.ngy));
+  const int lane = (int)threadIdx.x;
+  // the wavefront's rows are uniform: kept in scalar registers
+  const int wy = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+  const int jb = a.box[2] + (ty * (int)blockDim.y + wy) * R;
+  if (jb > a.box[3]) return;
+  const int i = a.box[0] + 63 * tx + lane;
+  // E is needed up to one cell past the box (the last owned cell's x
+  // neighbour); lanes beyond read that cell again
+  const int ic = min(i, a.box[1] + 1);
+  const bool ownx = lane < 63 && i <= a.box[1];
+  const int kb = a.box[4] + tz * a.kz;
+  const int ke = min(kb + a.kz, a.box[5] + 1);
+  const int jmax = a.box[3] + 1;
+  // 32-bit cell offsets (the host keeps the allocation below 2^31 cells)
+  const int P = (int)a.plane, SJ = a.PX;
+  // Cells this lane does not own are computed but stored to the lane's slot
+  // of the discard area past the end of every field array, so the plane's
+  // code has no branches and its loads of all rows can be issued together.
+  const int dslot = a.discard + ((((int)L * 4 + wy) & 1023) << 6 | lane);
+  const VG_GLOBAL T *__restrict__ Ex0 = gp(a.Ein[0]);
+  const VG_GLOBAL T *__restrict__ Ey0 = gp(a.Ein[1]);
+  const VG_GLOBAL T *__restrict__ Ez0 = gp(a.Ein[2]);
+  const VG_GLOBAL T *__restrict__ Hx = gp(a.Hin[0]);
+  const VG_GLOBAL T *__restrict__ Hy = gp(a.Hin[1]);
+  const VG_GLOBAL T *__restrict__ Hz = gp(a.Hin[2]);
+  VG_GLOBAL T *__restrict__ Ex1 = gp(a.E[0]);
+  VG_GLOBAL T *__restrict__ Ey1 = gp(a.E[1]);
+  VG_GLOBAL T *__restrict__ Ez1 = gp(a.E[2]);
+  VG_GLOBAL T *__restrict__ Hx1 = gp(a.H[0]);
+  VG_GLOBAL T *__restrict__ Hy1 = gp(a.H[1]);
+  VG_GLOBAL T *__restrict__ Hz1 = gp(a.H[2]);
+  // Rows jb .. jb+R (row jb+R: the next tile's first row, E only); rows past
+  // the box's next row read that row again.  The box ends at N-2 at most, so
+  // every index below stays inside the allocation.
+  int rowo[R + 1];
+  bool own[R + 1];
+#pragma unroll
+  for (int r = 0; r <= R; ++r) {
+    rowo[r] = min(jb + r, jmax) * SJ + ic;
+    own[r] = ownx && r < R && jb + r <= a.box[3];
+  }
+  const T idhx = a.idh[0][ic], idex = a.ide[0][ic];
+  // carried from plane k-1: old H, new E, the H class
+  T hxm[R + 1], hym[R + 1], hzm[R + 1], ex[R + 1], ey[R + 1], ez[R + 1];
+  unsigned hcm[R + 1];
+#pragma unroll
+  for (int r = 0; r <= R; ++r) {
+    const int id = (kb - 1) * P + rowo[r];
+    hxm[r] = Hx[id];
+    hym[r] = Hy[id];
+    hzm[r] = T(0);
+    ex[r] = ey[r] = ez[r] = T(0);
+    hcm[r] = 0u;
+  }
+  // One plane: new E of plane k on all rows (stored when SE), then H of
+  // plane k-1 (when DH).
+  auto plane = [&](int k, auto se, auto dh) {
+    constexpr bool SE = decltype(se)::value, DH = decltype(dh)::value;
+    const int pk = k * P;
+    const T idhz = cp(a.idh[2])[k];
+    const T idez = cp(a.ide[2])[DH ? k - 1 : k];
+    T hx[R + 1], hy[R + 1], hz[R + 1], hzx[R + 1], hyx[R + 1], e0[R + 1], e1[R + 1], e2[R + 1];
+    unsigned ecl[R + 1], hcl[R + 1];
+    const T hzy0 = Hz[pk + rowo[0] - SJ], hxy0 = Hx[pk + rowo[0] - SJ];
+#pragma unroll
+    for (int r = 0; r <= R; ++r) {
+      const int id = pk + rowo[r];
+      hx[r] = Hx[id];
+      hy[r] = Hy[id];
+      hz[r] = Hz[id];
+      hzx[r] = Hz[id - 1];
+      hyx[r] = Hy[id - 1];
+      e0[r] = Ex0[id];
+      e1[r] = Ey0[id];
+      e2[r] = Ez0[id];
+      ecl[r] = IDX ? (unsigned)a.ecls[id] : 0u;
+      hcl[r] = IDX ? (unsigned)a.hcls[id] : 0u;
+    }
+    // new E of plane k (grid.c:709-797)
+    T n0[R + 1], n1[R + 1], n2[R + 1];
+#pragma unroll
+    for (int r = 0; r <= R; ++r) {
+      const int id = pk + rowo[r];
+      const T idhy = cp(a.idh[1])[min(jb + r, jmax)];
+      const T hzy = r == 0 ? hzy0 : hz[r == 0 ? 0 : r - 1];
+      const T hxy = r == 0 ? hxy0 : hx[r == 0 ? 0 : r - 1];
+      T al, be;
+      leap_ecoef<T, IDX, 0>(a, ecl[r], id, al, be);
+      n0[r] = al * e0[r] + be * (idhy * (hz[r] - hzy) + idhz * (hym[r] - hy[r]));
+      leap_ecoef<T, IDX, 1>(a, ecl[r], id, al, be);
+      n1[r] = al * e1[r] + be * (idhz * (hx[r] - hxm[r]) + idhx * (hzx[r] - hz[r]));
+      leap_ecoef<T, IDX, 2>(a, ecl[r], id, al, be);
+      n2[r] = al * e2[r] + be * (idhx * (hy[r] - hyx[r]) + idhy * (hxy - hx[r]));
+    }
+    if (DH) {
+      // H of plane k-1 (grid.c:800-888): row r+1 of the carried state is
+      // the new E of plane k-1 one row up, the next lane one cell along x
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const T ezx = __shfl_down(ez[r], 1), eyx = __shfl_down(ey[r], 1);
+        const int idm = pk - P + rowo[r];
+        const T idey = cp(a.ide[1])[min(jb + r, jmax)];
+        T g0, g1, g2;
+        leap_gammas<T, IDX>(a, hcm[r], idm, g0, g1, g2);
+        const T h0 = hxm[r] + g0 * (idez * (n1[r] - ey[r]) + idey * (ez[r] - ez[r + 1]));
+        const T h1 = hym[r] + g1 * (idex * (ezx - ez[r]) + idez * (ex[r] - n0[r]));
+        const T h2 = hzm[r] + g2 * (idey * (ex[r + 1] - ex[r]) + idex * (ey[r] - eyx));
+        if (CHK) {
+          if (own[r]) {
+            visit(a.vis[3], idm);
+            visit(a.vis[4], idm);
+            visit(a.vis[5], idm);
+          }
+        } else {
+          const int d = own[r] ? idm : dslot;
+          Hx1[d] = h0;
+          Hy1[d] = h1;
+          Hz1[d] = h2;
+        }
+      }
+    }
+    if (SE) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int id = pk + rowo[r];
+        if (CHK) {
+          if (own[r]) {
+            visit(a.vis[0], id);
+            visit(a.vis[1], id);
+            visit(a.vis[2], id);
+          }
+        } else {
+          const int d = own[r] ? id : dslot;
+          Ex1[d] = n0[r];
+          Ey1[d] = n1[r];
+          Ez1[d] = n2[r];
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r <= R; ++r) {
+      ex[r] = n0[r];
+      ey[r] = n1[r];
+      ez[r] = n2[r];
+      hxm[r] = hx[r];
+      hym[r] = hy[r];
+      hzm[r] = hz[r];
+      hcm[r] = hcl[r];
+    }
+  };
+  plane(kb, Flag<true>(), Flag<false>());
+  for (int k = kb + 1; k < ke; ++k) plane(k, Flag<true>(), Flag<true>());
+  plane(ke, Flag<false>(), Flag<true>());
+}
+
+// H cells of the fused box updated again once E is final (after Debye,
+// SIBC, sources, TF/SF and ghosts): the ordinary H update from the old H.
+// One wavefront per entry; four entries per block.
+template <typename T, bool IDX> __global__ __launch_bounds__(256) void k_h_fix(FixArgs<T> a)
+{
+  const long long e = (long long)blockIdx.x * blockDim.y + threadIdx.y;
+  if (e >= a.n) return;
+  const FixEnt en = a.ent[e];
+  const int lane = (int)threadIdx.x;
+  if (!((en.mask >> lane) & 1ull)) return;
+  const int i = en.i0 + lane, j = en.j, k = en.k;
+  const long long SJ = a.PX, P = a.plane;
+  const long long id = ((long long)k * a.NY + j) * SJ + i;
+  const T *__restrict__ Ex = a.E[0];
+  const T *__restrict__ Ey = a.E[1];
+  const T *__restrict__ Ez = a.E[2];
+  const T ex = Ex[id], exy = Ex[id + SJ], exz = Ex[id + P];
+  const T ey = Ey[id], eyx = Ey[id + 1], eyz = Ey[id + P];
+  const T ez = Ez[id], ezx = Ez[id + 1], ezy = Ez[id + SJ];
+  const T idex = a.ide[0][i], idey = a.ide[1][j], idez = a.ide[2][k];
+  T g0, g1, g2;
+  if (IDX) {
+    load_gammas(a.htab + (unsigned)a.hcls[id] * HTAB_STRIDE, g0, g1, g2);
+  } else {
+    g0 = a.gamma[0][id];
+    g1 = a.gamma[1][id];
+    g2 = a.gamma[2][id];
+  }
+  const T h0 = a.Hin[0][id] + g0 * (idez * (eyz - ey) + idey * (ez - ezy));
+  const T h1 = a.Hin[1][id] + g1 * (idex * (ezx - ez) + idez * (ex - exz));
+  const T h2 = a.Hin[2][id] + g2 * (idey * (exy - ex) + idex * (ey - eyx));
+  a.H[0][id] = h0;
+  a.H[1][id] = h1;
+  a.H[2][id] = h2;
 }
 
 // Shell sweeps.  A block is 256 cells of one shell piece's x-y plane
@@ -374,7 +615,7 @@ template <int C> VG_HD ColRegion shell_region(const ShellPiece *pc, int i, int d
 template <typename T> struct EShellPtrs {
   VG_GLOBAL T *E[3];
   VG_GLOBAL T *PP, *Pa;  // UPML auxiliary pools
-  VG_GLOBAL const T *H[3], *alpha[3], *beta[3], *ad[3], *bd[3], *ah[3], *ibh[3], *idh[3], *etab;
+  VG_GLOBAL const T *Ein[3], *H[3], *alpha[3], *beta[3], *ad[3], *bd[3], *ah[3], *ibh[3], *idh[3], *etab;
   VG_GLOBAL const uint16_t *ecls;
   VG_DEV explicit EShellPtrs(const EArgs<T> &a)
   {
@@ -382,6 +623,7 @@ template <typename T> struct EShellPtrs {
     Pa = gp(a.Ppool);
     for (int d = 0; d < 3; ++d) {
       E[d] = gp(a.E[d]);
+      Ein[d] = gp(a.Ein[d]);
       H[d] = gp(a.H[d]);
       alpha[d] = gp(a.alpha[d]);
       beta[d] = gp(a.beta[d]);
@@ -426,7 +668,7 @@ template <typename T, bool IDX, int C> struct EShellCell {
   VG_DEV void load(const EShellPtrs<T> &g, const LaneRegion &rg, const int n3[3], long long id, unsigned cls)
   {
     if (rg.reg < 0) return;
-    e = g.E[C][id];
+    e = g.Ein[C][id];
     if (IDX) {
       load_pair((const T *)(g.etab + cls * ETAB_STRIDE + 2 * C), al, be);
     } else {
@@ -515,13 +757,14 @@ __global__ __launch_bounds__(256) void k_e_shell(const ShellPiece *pcs, const Sh
 template <typename T> struct HShellPtrs {
   VG_GLOBAL T *H[3];
   VG_GLOBAL T *B;  // UPML auxiliary pool
-  VG_GLOBAL const T *E[3], *gamma[3], *ad[3], *ah[3], *bh[3], *ibd[3], *ide[3], *htab;
+  VG_GLOBAL const T *Hin[3], *E[3], *gamma[3], *ad[3], *ah[3], *bh[3], *ibd[3], *ide[3], *htab;
   VG_GLOBAL const uint16_t *hcls;
   VG_DEV explicit HShellPtrs(const HArgs<T> &a)
   {
     B = gp(a.Bpool);
     for (int d = 0; d < 3; ++d) {
       H[d] = gp(a.H[d]);
+      Hin[d] = gp(a.Hin[d]);
       E[d] = gp(a.E[d]);
       gamma[d] = gp(a.gamma[d]);
       ad[d] = gp(a.ad[d]);
@@ -563,7 +806,7 @@ template <typename T, bool IDX, int C> struct HShellCell {
   VG_DEV void load(const HShellPtrs<T> &q, const LaneRegion &rg, const int n3[3], long long id, unsigned cls)
   {
     if (rg.reg < 0) return;
-    h = q.H[C][id];
+    h = q.Hin[C][id];
     g = IDX ? q.htab[cls * HTAB_STRIDE + C] : q.gamma[C][id];
     if (rg.reg < 6) {
       ob = q.B[rg.q];
@@ -671,7 +914,7 @@ template <typename T, bool CHK = false> __global__ void k_mur_e(MurArgs<T> a)
   if (c == 0) curl = a.idh[1][j] * (Hz[nb] - Hz[nb - SJ]) + a.idh[2][k] * (Hy[nb - P] - Hy[nb]);
   else if (c == 1) curl = a.idh[2][k] * (Hx[nb] - Hx[nb - P]) + a.idh[0][i] * (Hz[nb - 1] - Hz[nb]);
   else curl = a.idh[0][i] * (Hy[nb] - Hy[nb - 1]) + a.idh[1][j] * (Hx[nb - SJ] - Hx[nb]);
-  T *E = a.E[c];
+  const T *Eo = a.Ein[c];
   T al, be;
   if (a.ecls) {
     load_pair(a.etab + (unsigned)a.ecls[nb] * ETAB_STRIDE + 2 * c, al, be);
@@ -679,9 +922,9 @@ template <typename T, bool CHK = false> __global__ void k_mur_e(MurArgs<T> a)
     al = a.alpha[c][nb];
     be = a.beta[c][nb];
   }
-  const T tmp = al * E[nb] + be * curl;
+  const T tmp = al * Eo[nb] + be * curl;
   if (CHK) visit(a.vis[c], id);
-  else E[id] = E[nb] + a.zeta[face] * (tmp - E[id]);
+  else a.E[c][id] = Eo[nb] + a.zeta[face] * (tmp - Eo[id]);
 }
 
 // Tally of one component's visit counts against its outer field limits lim

@@ -19,18 +19,23 @@ namespace vgpu {
 
 template <typename T> static T *P(void *p) { return static_cast<T *>(p); }
 
-template <typename T> static void fill_volume_args(Ctx *c, EArgs<T> &ea, HArgs<T> &ha)
+// Sweep arguments of a step that reads field set `in` and writes set
+// c->next(in) (the other set with the fused sweep, the same set otherwise).
+template <typename T> static void fill_volume_args(Ctx *c, EArgs<T> &ea, HArgs<T> &ha, int in)
 {
   ea.NX = ha.NX = c->NX;
   ea.NY = ha.NY = c->NY;
   ea.NZ = ha.NZ = c->NZ;
   ea.PX = ha.PX = c->PX;
   ea.plane = ha.plane = c->plane;
+  const int out = c->next(in);
   for (int d = 0; d < 3; ++d) {
-    ea.E[d] = P<T>(c->F[d]);
-    ea.H[d] = P<T>(c->F[3 + d]);
-    ha.H[d] = P<T>(c->F[3 + d]);
-    ha.E[d] = P<T>(c->F[d]);
+    ea.E[d] = P<T>(c->fld(out, d));
+    ea.Ein[d] = P<T>(c->fld(in, d));
+    ea.H[d] = P<T>(c->fld(in, 3 + d));
+    ha.H[d] = P<T>(c->fld(out, 3 + d));
+    ha.Hin[d] = P<T>(c->fld(in, 3 + d));
+    ha.E[d] = P<T>(c->fld(out, d));
     ea.alpha[d] = P<T>(c->alpha[d]);
     ea.beta[d] = P<T>(c->beta[d]);
     ha.gamma[d] = P<T>(c->gamma[d]);
@@ -310,9 +315,10 @@ template <typename T> int prepare_sweeps(Ctx *c)
         (fb[0] < 1 || fb[2] < 1 || fb[4] < 1 || fb[1] > c->NX - 2 || fb[3] > c->NY - 2 || fb[5] > c->NZ - 2))
       return set_error(c, VGPU_EINVAL, "fast box not inside the grid");
   }
-  EArgs<T> ea{};
-  HArgs<T> ha{};
-  fill_volume_args<T>(c, ea, ha);
+  EArgs<T> ea{}, ea1{};
+  HArgs<T> ha{}, ha1{};
+  fill_volume_args<T>(c, ea, ha, 0);
+  fill_volume_args<T>(c, ea1, ha1, 1);
   const int N[3] = {c->NX, c->NY, c->NZ};
   std::vector<ShellPiece> ep, hp;
   std::vector<ShellWork> ew, hw;
@@ -326,16 +332,20 @@ template <typename T> int prepare_sweeps(Ctx *c)
     return set_error(c, VGPU_EINVAL, "%s", err.c_str());
   c->nEpcs = (int)ep.size(); c->nHpcs = (int)hp.size();
   c->nEwork = (int)ew.size(); c->nHwork = (int)hw.size();
-  c->d_eargs = dev_alloc(c, sizeof(ea));
-  c->d_hargs = dev_alloc(c, sizeof(ha));
+  for (int q = 0; q < 2; ++q) {
+    c->d_eargs[q] = dev_alloc(c, sizeof(ea));
+    c->d_hargs[q] = dev_alloc(c, sizeof(ha));
+  }
   c->d_epcs = dev_alloc(c, ep.size() * sizeof(ShellPiece));
   c->d_hpcs = dev_alloc(c, hp.size() * sizeof(ShellPiece));
   c->d_ework = dev_alloc(c, ew.size() * sizeof(ShellWork));
   c->d_hwork = dev_alloc(c, hw.size() * sizeof(ShellWork));
-  if (!c->d_eargs || !c->d_hargs || !c->d_epcs || !c->d_hpcs || !c->d_ework || !c->d_hwork)
+  if (!c->d_eargs[0] || !c->d_hargs[0] || !c->d_eargs[1] || !c->d_hargs[1] || !c->d_epcs || !c->d_hpcs ||
+      !c->d_ework || !c->d_hwork)
     return set_error(c, VGPU_ENOMEM, "sweep arguments");
   auto up = [](void *d, const void *h, size_t n) { return n == 0 || hipMemcpy(d, h, n, hipMemcpyHostToDevice) == hipSuccess; };
-  if (!up(c->d_eargs, &ea, sizeof(ea)) || !up(c->d_hargs, &ha, sizeof(ha)) ||
+  if (!up(c->d_eargs[0], &ea, sizeof(ea)) || !up(c->d_hargs[0], &ha, sizeof(ha)) ||
+      !up(c->d_eargs[1], &ea1, sizeof(ea1)) || !up(c->d_hargs[1], &ha1, sizeof(ha1)) ||
       !up(c->d_epcs, ep.data(), ep.size() * sizeof(ShellPiece)) ||
       !up(c->d_hpcs, hp.data(), hp.size() * sizeof(ShellPiece)) ||
       !up(c->d_ework, ew.data(), ew.size() * sizeof(ShellWork)) ||
@@ -388,14 +398,18 @@ template <typename T> int prepare_sweeps(Ctx *c)
 
 // The loop (simulation.c:136-230).  With chk, one step of the field sweeps
 // only (Mur E, the E sweep, the H sweep), in their update-counting form.
+// With the fused sweep, step q of the run reads field set c->par ^ (q & 1)
+// and writes the other, and the caller flips c->par by the parity of nsteps
+// once the run is enqueued; the separate sweeps read and write set 0.
 template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool timeKernels, const VisitRun *chk)
 {
   const double pi = 4.0 * atan(1.0);
   const double dt = c->grid.dt;
   timeKernels = timeKernels && !c->evE.empty();
-  EArgs<T> ea{};
-  HArgs<T> ha{};
-  fill_volume_args<T>(c, ea, ha);
+  // argument sets by the field set a step reads (index 0/1)
+  EArgs<T> ea[2] = {};
+  HArgs<T> ha[2] = {};
+  for (int q = 0; q < 2; ++q) fill_volume_args<T>(c, ea[q], ha[q], q);
   // Fast kernels: 64x4 tiles (64 x ty) over the fast boxes, marching z in chunks of 4
   // planes (measured best from 272^3 to 386x386x258; 16k blocks and more at those sizes).
   // rows per block (tuning override VGPU_FAST_TY: 1..8)
@@ -410,154 +424,260 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
       kchunk = std::max(1, std::min(atoi(ev), nz));
     return dim3(gx, gy, (nz + kchunk - 1) / kchunk);
   };
-  FastE<T> fe{};
-  FastH<T> fh{};
-  const dim3 egrid = fast_grid(c->fastE, fe.kchunk), hgrid = fast_grid(c->fastH, fh.kchunk);
-  fe.NX = fh.NX = c->NX; fe.NY = fh.NY = c->NY; fe.NZ = fh.NZ = c->NZ; fe.PX = fh.PX = c->PX;
-  fe.plane = fh.plane = c->plane;
-  for (int q = 0; q < 6; ++q) { fe.box[q] = c->fastE[q]; fh.box[q] = c->fastH[q]; }
-  fe.nchx = egrid.x; fh.nchx = hgrid.x;
-  fe.gy = egrid.y; fe.gz = egrid.z; fh.gy = hgrid.y; fh.gz = hgrid.z;
+  FastE<T> fe[2] = {};
+  FastH<T> fh[2] = {};
+  int ekc = 1, hkc = 1;
+  const dim3 egrid = fast_grid(c->fastE, ekc), hgrid = fast_grid(c->fastH, hkc);
   // 1-D launches padded to a multiple of eight blocks (XCD order, xcd_block)
   auto pad8 = [](unsigned long long n) { return dim3((unsigned)((n + 7) / 8 * 8)); };
   const dim3 eg1 = pad8((unsigned long long)egrid.x * egrid.y * egrid.z);
   const dim3 hg1 = pad8((unsigned long long)hgrid.x * hgrid.y * hgrid.z);
-  for (int d = 0; d < 3; ++d) {
-    fe.E[d] = ea.E[d]; fe.H[d] = ea.H[d]; fe.idh[d] = ea.idh[d]; fe.alpha[d] = ea.alpha[d]; fe.beta[d] = ea.beta[d];
-    fh.H[d] = ha.H[d]; fh.E[d] = ha.E[d]; fh.ide[d] = ha.ide[d]; fh.gamma[d] = ha.gamma[d];
+  for (int q = 0; q < 2; ++q) {
+    FastE<T> &e = fe[q];
+    FastH<T> &h = fh[q];
+    e.NX = h.NX = c->NX; e.NY = h.NY = c->NY; e.NZ = h.NZ = c->NZ; e.PX = h.PX = c->PX;
+    e.plane = h.plane = c->plane;
+    e.kchunk = ekc; h.kchunk = hkc;
+    for (int f = 0; f < 6; ++f) { e.box[f] = c->fastE[f]; h.box[f] = c->fastH[f]; }
+    e.nchx = egrid.x; h.nchx = hgrid.x;
+    e.gy = egrid.y; e.gz = egrid.z; h.gy = hgrid.y; h.gz = hgrid.z;
+    for (int d = 0; d < 3; ++d) {
+      e.E[d] = ea[q].E[d]; e.Ein[d] = ea[q].Ein[d]; e.H[d] = ea[q].H[d]; e.idh[d] = ea[q].idh[d];
+      e.alpha[d] = ea[q].alpha[d]; e.beta[d] = ea[q].beta[d];
+      h.H[d] = ha[q].H[d]; h.Hin[d] = ha[q].Hin[d]; h.E[d] = ha[q].E[d]; h.ide[d] = ha[q].ide[d];
+      h.gamma[d] = ha[q].gamma[d];
+    }
+    e.ecls = ea[q].ecls; e.etab = ea[q].etab; e.dfuse = ea[q].dfuse; e.ap = P<EArgs<T>>(c->d_eargs[q]);
+    h.hcls = ha[q].hcls; h.htab = ha[q].htab;
+    if (chk)
+      for (int d = 0; d < 3; ++d) { e.vis[d] = chk->vis[d]; h.vis[d] = chk->vis[3 + d]; }
   }
-  fe.ecls = ea.ecls; fe.etab = ea.etab; fe.dfuse = ea.dfuse; fe.ap = P<EArgs<T>>(c->d_eargs);
-  fh.hcls = ha.hcls; fh.htab = ha.htab;
-  const EArgs<T> *eap = P<EArgs<T>>(chk ? chk->eargs : c->d_eargs);
-  const HArgs<T> *hap = P<HArgs<T>>(chk ? chk->hargs : c->d_hargs);
-  if (chk)
-    for (int d = 0; d < 3; ++d) { fe.vis[d] = chk->vis[d]; fh.vis[d] = chk->vis[3 + d]; }
+  const EArgs<T> *eap[2];
+  const HArgs<T> *hap[2];
+  for (int q = 0; q < 2; ++q) {
+    eap[q] = P<EArgs<T>>(chk ? chk->eargs : c->d_eargs[q]);
+    hap[q] = P<HArgs<T>>(chk ? chk->hargs : c->d_hargs[q]);
+  }
   const ShellPiece *epc = P<ShellPiece>(c->d_epcs), *hpc = P<ShellPiece>(c->d_hpcs);
   const ShellWork *ewk = P<ShellWork>(c->d_ework), *hwk = P<ShellWork>(c->d_hwork);
-  auto fast_e = [&]() {
+  auto fast_e = [&](int q) {
+    const FastE<T> &f = fe[q];
     if (egrid.x && chk) {
-      if (c->indexed) hipLaunchKernelGGL((k_e_fast<T, true, false, true>), eg1, vblk, 0, s, fe);
-      else hipLaunchKernelGGL((k_e_fast<T, false, false, true>), eg1, vblk, 0, s, fe);
+      if (c->indexed) hipLaunchKernelGGL((k_e_fast<T, true, false, true>), eg1, vblk, 0, s, f);
+      else hipLaunchKernelGGL((k_e_fast<T, false, false, true>), eg1, vblk, 0, s, f);
     } else if (egrid.x) {
-      if (c->indexed && c->debyeFused) hipLaunchKernelGGL((k_e_fast<T, true, true>), eg1, vblk, 0, s, fe);
-      else if (c->indexed) hipLaunchKernelGGL((k_e_fast<T, true, false>), eg1, vblk, 0, s, fe);
-      else hipLaunchKernelGGL((k_e_fast<T, false, false>), eg1, vblk, 0, s, fe);
+      if (c->indexed && c->debyeFused) hipLaunchKernelGGL((k_e_fast<T, true, true>), eg1, vblk, 0, s, f);
+      else if (c->indexed) hipLaunchKernelGGL((k_e_fast<T, true, false>), eg1, vblk, 0, s, f);
+      else hipLaunchKernelGGL((k_e_fast<T, false, false>), eg1, vblk, 0, s, f);
     }
   };
-  auto shell_e = [&]() {
+  auto shell_e = [&](int q) {
     if (c->nEwork) {
       const dim3 g((unsigned)c->nEwork);
-      if (chk && c->indexed) hipLaunchKernelGGL((k_e_shell<T, true, true>), g, dim3(256), 0, s, epc, ewk, eap);
-      else if (chk) hipLaunchKernelGGL((k_e_shell<T, false, true>), g, dim3(256), 0, s, epc, ewk, eap);
-      else if (c->indexed) hipLaunchKernelGGL((k_e_shell<T, true>), g, dim3(256), 0, s, epc, ewk, eap);
-      else hipLaunchKernelGGL((k_e_shell<T, false>), g, dim3(256), 0, s, epc, ewk, eap);
+      const EArgs<T> *a = eap[q];
+      if (chk && c->indexed) hipLaunchKernelGGL((k_e_shell<T, true, true>), g, dim3(256), 0, s, epc, ewk, a);
+      else if (chk) hipLaunchKernelGGL((k_e_shell<T, false, true>), g, dim3(256), 0, s, epc, ewk, a);
+      else if (c->indexed) hipLaunchKernelGGL((k_e_shell<T, true>), g, dim3(256), 0, s, epc, ewk, a);
+      else hipLaunchKernelGGL((k_e_shell<T, false>), g, dim3(256), 0, s, epc, ewk, a);
     }
   };
-  auto fast_h = [&]() {
+  auto fast_h = [&](int q) {
+    const FastH<T> &f = fh[q];
     if (hgrid.x && chk) {
-      if (c->indexed) hipLaunchKernelGGL((k_h_fast<T, true, true>), hg1, vblk, 0, s, fh);
-      else hipLaunchKernelGGL((k_h_fast<T, false, true>), hg1, vblk, 0, s, fh);
+      if (c->indexed) hipLaunchKernelGGL((k_h_fast<T, true, true>), hg1, vblk, 0, s, f);
+      else hipLaunchKernelGGL((k_h_fast<T, false, true>), hg1, vblk, 0, s, f);
     } else if (hgrid.x) {
-      if (c->indexed) hipLaunchKernelGGL((k_h_fast<T, true>), hg1, vblk, 0, s, fh);
-      else hipLaunchKernelGGL((k_h_fast<T, false>), hg1, vblk, 0, s, fh);
+      if (c->indexed) hipLaunchKernelGGL((k_h_fast<T, true>), hg1, vblk, 0, s, f);
+      else hipLaunchKernelGGL((k_h_fast<T, false>), hg1, vblk, 0, s, f);
     }
   };
-  auto shell_h = [&]() {
+  auto shell_h = [&](int q) {
     if (c->nHwork) {
       const dim3 g((unsigned)c->nHwork);
-      if (chk && c->indexed) hipLaunchKernelGGL((k_h_shell<T, true, true>), g, dim3(256), 0, s, hpc, hwk, hap);
-      else if (chk) hipLaunchKernelGGL((k_h_shell<T, false, true>), g, dim3(256), 0, s, hpc, hwk, hap);
-      else if (c->indexed) hipLaunchKernelGGL((k_h_shell<T, true>), g, dim3(256), 0, s, hpc, hwk, hap);
-      else hipLaunchKernelGGL((k_h_shell<T, false>), g, dim3(256), 0, s, hpc, hwk, hap);
+      const HArgs<T> *a = hap[q];
+      if (chk && c->indexed) hipLaunchKernelGGL((k_h_shell<T, true, true>), g, dim3(256), 0, s, hpc, hwk, a);
+      else if (chk) hipLaunchKernelGGL((k_h_shell<T, false, true>), g, dim3(256), 0, s, hpc, hwk, a);
+      else if (c->indexed) hipLaunchKernelGGL((k_h_shell<T, true>), g, dim3(256), 0, s, hpc, hwk, a);
+      else hipLaunchKernelGGL((k_h_shell<T, false>), g, dim3(256), 0, s, hpc, hwk, a);
     }
   };
 
-  MurArgs<T> ma{};
+  // fused leapfrog sweep: 4 waves of R rows per block, 63-cell x tiles, kz-plane z chunks
+  LeapArgs<T> la[2] = {};
+  FixArgs<T> fx[2] = {};
+  dim3 lgrid(0);
+  const dim3 lblk(64, 4, 1);
+  if (c->leap) {
+    const int *fb = c->fastE;
+    if (fb[0] <= fb[1] && fb[2] <= fb[3] && fb[4] <= fb[5]) {
+      const int R = c->leapR, kz = std::min(c->leapKZ, fb[5] - fb[4] + 1);
+      const int ntx = (fb[1] - fb[0] + 63) / 63, ngy = (fb[3] - fb[2] + 4 * R) / (4 * R), ntz = (fb[5] - fb[4] + kz) / kz;
+      lgrid = pad8((unsigned long long)ntx * ngy * ntz);
+      for (int q = 0; q < 2; ++q) {
+        LeapArgs<T> &l = la[q];
+        l.NX = c->NX; l.NY = c->NY; l.NZ = c->NZ; l.PX = c->PX; l.plane = c->plane;
+        for (int f = 0; f < 6; ++f) l.box[f] = fb[f];
+        l.ntx = ntx; l.ngy = ngy; l.ntz = ntz; l.kz = kz;
+        for (int d = 0; d < 3; ++d) {
+          l.E[d] = ea[q].E[d]; l.Ein[d] = ea[q].Ein[d]; l.H[d] = ha[q].H[d]; l.Hin[d] = ha[q].Hin[d];
+          l.idh[d] = ea[q].idh[d]; l.ide[d] = ha[q].ide[d];
+          l.alpha[d] = ea[q].alpha[d]; l.beta[d] = ea[q].beta[d]; l.gamma[d] = ha[q].gamma[d];
+        }
+        l.ecls = ea[q].ecls; l.etab = ea[q].etab; l.hcls = ha[q].hcls; l.htab = ha[q].htab;
+        l.discard = (int)c->ncell;
+        if (chk)
+          for (int d = 0; d < 6; ++d) l.vis[d] = chk->vis[d];
+        FixArgs<T> &x = fx[q];
+        x.n = c->nFix; x.ent = P<FixEnt>(c->d_fix);
+        x.NY = c->NY; x.PX = c->PX; x.plane = c->plane;
+        for (int d = 0; d < 3; ++d) {
+          x.H[d] = ha[q].H[d]; x.Hin[d] = ha[q].Hin[d]; x.E[d] = ha[q].E[d]; x.ide[d] = ha[q].ide[d];
+          x.gamma[d] = ha[q].gamma[d];
+        }
+        x.hcls = ha[q].hcls; x.htab = ha[q].htab;
+      }
+    }
+  }
+  auto leap = [&](int q) {
+    if (!lgrid.x) return;
+    const LeapArgs<T> &l = la[q];
+    const int R = c->leapR;
+    if (chk) {
+      if (c->indexed) {
+        if (R == 1) hipLaunchKernelGGL((k_leap<T, true, 1, true>), lgrid, lblk, 0, s, l);
+        else if (R == 2) hipLaunchKernelGGL((k_leap<T, true, 2, true>), lgrid, lblk, 0, s, l);
+        else hipLaunchKernelGGL((k_leap<T, true, 4, true>), lgrid, lblk, 0, s, l);
+      } else {
+        if (R == 1) hipLaunchKernelGGL((k_leap<T, false, 1, true>), lgrid, lblk, 0, s, l);
+        else if (R == 2) hipLaunchKernelGGL((k_leap<T, false, 2, true>), lgrid, lblk, 0, s, l);
+        else hipLaunchKernelGGL((k_leap<T, false, 4, true>), lgrid, lblk, 0, s, l);
+      }
+    } else if (c->indexed) {
+      if (R == 1) hipLaunchKernelGGL((k_leap<T, true, 1>), lgrid, lblk, 0, s, l);
+      else if (R == 2) hipLaunchKernelGGL((k_leap<T, true, 2>), lgrid, lblk, 0, s, l);
+      else hipLaunchKernelGGL((k_leap<T, true, 4>), lgrid, lblk, 0, s, l);
+    } else {
+      if (R == 1) hipLaunchKernelGGL((k_leap<T, false, 1>), lgrid, lblk, 0, s, l);
+      else if (R == 2) hipLaunchKernelGGL((k_leap<T, false, 2>), lgrid, lblk, 0, s, l);
+      else hipLaunchKernelGGL((k_leap<T, false, 4>), lgrid, lblk, 0, s, l);
+    }
+  };
+  auto fix = [&](int q) {
+    if (!lgrid.x || !c->nFix) return;
+    const dim3 g((unsigned)((c->nFix + 3) / 4));
+    if (c->indexed) hipLaunchKernelGGL((k_h_fix<T, true>), g, lblk, 0, s, fx[q]);
+    else hipLaunchKernelGGL((k_h_fix<T, false>), g, lblk, 0, s, fx[q]);
+  };
+
+  MurArgs<T> ma[2] = {};
   int murmax1 = 1, murmax2 = 1;
   if (c->murmask) {
-    ma.NX = c->NX; ma.NY = c->NY; ma.PX = c->PX; ma.plane = c->plane;
-    for (int d = 0; d < 3; ++d) {
-      ma.E[d] = P<T>(c->F[d]); ma.H[d] = P<T>(c->F[3 + d]);
-      ma.alpha[d] = P<T>(c->alpha[d]); ma.beta[d] = P<T>(c->beta[d]); ma.idh[d] = P<T>(c->idh[d]);
+    for (int q = 0; q < 2; ++q) {
+      MurArgs<T> &m = ma[q];
+      m.NX = c->NX; m.NY = c->NY; m.PX = c->PX; m.plane = c->plane;
+      for (int d = 0; d < 3; ++d) {
+        m.E[d] = ea[q].E[d]; m.Ein[d] = ea[q].Ein[d]; m.H[d] = ea[q].H[d];
+        m.alpha[d] = P<T>(c->alpha[d]); m.beta[d] = P<T>(c->beta[d]); m.idh[d] = P<T>(c->idh[d]);
+      }
+      if (c->indexed) { m.ecls = ea[q].ecls; m.etab = ea[q].etab; }
+      if (chk)
+        for (int d = 0; d < 3; ++d) m.vis[d] = chk->vis[d];
     }
-    if (c->indexed) { ma.ecls = ea.ecls; ma.etab = ea.etab; }
-    if (chk)
-      for (int d = 0; d < 3; ++d) ma.vis[d] = chk->vis[d];
     static const int order[6][2] = {{1, 2}, {1, 2}, {0, 2}, {0, 2}, {1, 0}, {1, 0}};
     for (int f = 0; f < 6; ++f) {
-      ma.zeta[f] = T(c->zeta[f]);
-      for (int q = 0; q < 2; ++q) {
-        const int slot = 2 * f + q;
+      for (int q = 0; q < 2; ++q) ma[q].zeta[f] = T(c->zeta[f]);
+      for (int u = 0; u < 2; ++u) {
+        const int slot = 2 * f + u;
         if ((c->murmask >> f) & 1u) {
-          const int comp = order[f][q];
-          ma.comp[slot] = comp;
-          for (int e = 0; e < 6; ++e) ma.box[slot][e] = c->fmlim[f][comp][e];
+          const int comp = order[f][u];
+          for (int q = 0; q < 2; ++q) {
+            ma[q].comp[slot] = comp;
+            for (int e = 0; e < 6; ++e) ma[q].box[slot][e] = c->fmlim[f][comp][e];
+          }
           const int ax = f >> 1, a1 = ax == 0 ? 1 : 0, a2 = ax == 2 ? 1 : 2;
-          murmax1 = std::max(murmax1, ma.box[slot][2 * a1 + 1] - ma.box[slot][2 * a1] + 1);
-          murmax2 = std::max(murmax2, ma.box[slot][2 * a2 + 1] - ma.box[slot][2 * a2] + 1);
+          const int *b = ma[0].box[slot];
+          murmax1 = std::max(murmax1, b[2 * a1 + 1] - b[2 * a1] + 1);
+          murmax2 = std::max(murmax2, b[2 * a2 + 1] - b[2 * a2] + 1);
         } else {
-          ma.comp[slot] = -1;
+          for (int q = 0; q < 2; ++q) ma[q].comp[slot] = -1;
         }
       }
     }
   }
-
-  DebyeArgs<T> da{};
-  if (c->debyeSites) {
-    da.key = P<long long>(c->d_dkey); da.sbeta = P<T>(c->d_dsbeta);
-    for (int d = 0; d < 3; ++d) da.E[d] = P<T>(c->F[d]);
-    da.lastE = P<T>(c->d_dlast); da.J = P<T>(c->d_dJ); da.stride = c->debyeSites;
-    da.dalpha = P<double>(c->d_dalpha); da.dbeta = P<double>(c->d_dbeta); da.npoles = P<int>(c->d_dnp);
-    da.maxPoles = c->debyeMaxPoles; da.cplx = c->debyeCplx;
-  }
+  const dim3 murGrid((murmax1 + 15) / 16, (murmax2 + 15) / 16, 12);
 
   if (chk) {
-    if (c->murmask)
-      hipLaunchKernelGGL((k_mur_e<T, true>), dim3((murmax1 + 15) / 16, (murmax2 + 15) / 16, 12), dim3(16, 16), 0, s,
-                         ma);
-    fast_e();
-    shell_e();
-    fast_h();
-    shell_h();
+    const int q = c->par;
+    if (c->murmask) hipLaunchKernelGGL((k_mur_e<T, true>), murGrid, dim3(16, 16), 0, s, ma[q]);
+    if (c->leap) {
+      leap(q);
+      shell_e(q);
+      shell_h(q);
+    } else {
+      fast_e(q);
+      shell_e(q);
+      fast_h(q);
+      shell_h(q);
+    }
     const hipError_t err = hipGetLastError();
     if (err != hipSuccess) return set_error(c, VGPU_EDEVICE, "kernel launch failed: %s", hipGetErrorString(err));
     return VGPU_OK;
   }
 
-  SibcEArgs<T> se{};
-  SibcHArgs<T> sh{};
-  if (c->sibcFaces) {
-    se.nfaces = c->sibcFaces; se.ijk = P<int>(c->d_sijk); se.surf = P<int>(c->d_ssurf);
-    se.blocks = P<SibcBlock>(c->d_sblocks);
-    se.adjA = P<unsigned char>(c->d_sadjA); se.adjB = P<unsigned char>(c->d_sadjB);
-    se.normal = P<int>(c->d_snormal); se.A = P<double>(c->d_sA); se.B = P<double>(c->d_sB);
-    se.filtset = P<int>(c->d_sset); se.npoles = P<int>(c->d_snp); se.asymp = P<double>(c->d_sasymp);
-    se.q = P<double>(c->d_sq); se.maxPoles = c->sibcMaxPoles; se.zeta = P<double>(c->d_szeta);
-    se.real = c->sibcReal ? 1 : 0;
-    se.factored = c->sibcFactored ? 1 : 0;
-    se.fbase = P<int>(c->d_sfbase); se.fratio = P<double>(c->d_sfratio);
-    se.old = P<double>(c->d_sold); se.Etan = P<T>(c->d_setan);
-    for (int d = 0; d < 3; ++d) { se.E[d] = P<T>(c->F[d]); se.H[d] = P<T>(c->F[3 + d]); }
-    se.NY = c->NY; se.PX = c->PX; se.plane = c->plane;
-    sh.nrows = c->sibcHRows; sh.rowkey = P<long long>(c->d_shkey);
-    sh.rowstart = P<int>(c->d_shstart); sh.rowg = P<T>(c->d_shg);
-    sh.ent = P<SibcHEnt>(c->d_shent); sh.Etan = P<T>(c->d_setan);
-    for (int d = 0; d < 3; ++d) { sh.H[d] = P<T>(c->F[3 + d]); sh.ide[d] = P<T>(c->ide[d]); }
+  DebyeArgs<T> da[2] = {};
+  if (c->debyeSites) {
+    for (int q = 0; q < 2; ++q) {
+      DebyeArgs<T> &a = da[q];
+      a.key = P<long long>(c->d_dkey); a.sbeta = P<T>(c->d_dsbeta);
+      for (int d = 0; d < 3; ++d) a.E[d] = ea[q].E[d];
+      a.lastE = P<T>(c->d_dlast); a.J = P<T>(c->d_dJ); a.stride = c->debyeSites;
+      a.dalpha = P<double>(c->d_dalpha); a.dbeta = P<double>(c->d_dbeta); a.npoles = P<int>(c->d_dnp);
+      a.maxPoles = c->debyeMaxPoles; a.cplx = c->debyeCplx;
+    }
   }
 
-  CorrArgs<T> ce[2];
-  for (int e = 0; e < 2; ++e) {
-    CorrArgs<T> &a = ce[e];
-    a.nrows = c->corrRows[e];
-    a.rowkey = P<long long>(c->d_ckey[e]);
-    a.rowstart = P<int>(c->d_cstart[e]); a.ent = P<CorrEnt>(c->d_cent[e]);
-    a.rowcoef = P<T>(c->d_ccoef[e]); a.rowalpha = P<T>(c->d_calpha[e]);
-    a.electric = e == 0;
-    a.wf = P<WaveformDev>(c->d_wf); a.src = P<SourceDev>(c->d_src); a.pw = P<PwDev>(c->d_pw);
-    a.aux = P<T>(c->d_aux); a.auxmax = c->auxmax; a.eta0 = c->grid.eta0;
+  SibcEArgs<T> se[2] = {};
+  SibcHArgs<T> sh[2] = {};
+  if (c->sibcFaces) {
+    for (int q = 0; q < 2; ++q) {
+      SibcEArgs<T> &e = se[q];
+      SibcHArgs<T> &h = sh[q];
+      e.nfaces = c->sibcFaces; e.ijk = P<int>(c->d_sijk); e.surf = P<int>(c->d_ssurf);
+      e.blocks = P<SibcBlock>(c->d_sblocks);
+      e.adjA = P<unsigned char>(c->d_sadjA); e.adjB = P<unsigned char>(c->d_sadjB);
+      e.normal = P<int>(c->d_snormal); e.A = P<double>(c->d_sA); e.B = P<double>(c->d_sB);
+      e.filtset = P<int>(c->d_sset); e.npoles = P<int>(c->d_snp); e.asymp = P<double>(c->d_sasymp);
+      e.q = P<double>(c->d_sq); e.maxPoles = c->sibcMaxPoles; e.zeta = P<double>(c->d_szeta);
+      e.real = c->sibcReal ? 1 : 0;
+      e.factored = c->sibcFactored ? 1 : 0;
+      e.fbase = P<int>(c->d_sfbase); e.fratio = P<double>(c->d_sfratio);
+      e.old = P<double>(c->d_sold); e.Etan = P<T>(c->d_setan);
+      // SIBC E runs after the E sweep: it zeroes the new tangential E and
+      // reads the old H (sibc.c:444-586)
+      for (int d = 0; d < 3; ++d) { e.E[d] = ea[q].E[d]; e.H[d] = ea[q].H[d]; }
+      e.NY = c->NY; e.PX = c->PX; e.plane = c->plane;
+      h.nrows = c->sibcHRows; h.rowkey = P<long long>(c->d_shkey);
+      h.rowstart = P<int>(c->d_shstart); h.rowg = P<T>(c->d_shg);
+      h.ent = P<SibcHEnt>(c->d_shent); h.Etan = P<T>(c->d_setan);
+      for (int d = 0; d < 3; ++d) { h.H[d] = ha[q].H[d]; h.ide[d] = P<T>(c->ide[d]); }
+    }
   }
-  // the E list is built from E components, the H list from H components
-  for (int d = 0; d < 3; ++d) { ce[0].F[d] = P<T>(c->F[d]); ce[1].F[d] = P<T>(c->F[3 + d]); }
+
+  CorrArgs<T> ce[2][2];
+  for (int q = 0; q < 2; ++q)
+    for (int e = 0; e < 2; ++e) {
+      CorrArgs<T> &a = ce[q][e];
+      a = CorrArgs<T>{};
+      a.nrows = c->corrRows[e];
+      a.rowkey = P<long long>(c->d_ckey[e]);
+      a.rowstart = P<int>(c->d_cstart[e]); a.ent = P<CorrEnt>(c->d_cent[e]);
+      a.rowcoef = P<T>(c->d_ccoef[e]); a.rowalpha = P<T>(c->d_calpha[e]);
+      a.electric = e == 0;
+      a.wf = P<WaveformDev>(c->d_wf); a.src = P<SourceDev>(c->d_src); a.pw = P<PwDev>(c->d_pw);
+      a.aux = P<T>(c->d_aux); a.auxmax = c->auxmax; a.eta0 = c->grid.eta0;
+      // the E list corrects the new E, the H list the new H
+      for (int d = 0; d < 3; ++d) a.F[d] = e == 0 ? ea[q].E[d] : ha[q].H[d];
+    }
 
   AuxArgs<T> xa{};
   bool anyAux = false;
@@ -568,26 +688,34 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
     xa.wf = P<WaveformDev>(c->d_wf);
   }
 
-  ObsArgs<T> oa{};
+  // observers sample the fields a step starts from: the set it reads
+  ObsArgs<T> oa[2] = {};
   int maxnf = 1;
   if (!c->pobs.empty()) {
-    oa.nobs = (int)c->pobs.size(); oa.cell = P<long long>(c->d_ocell);
-    oa.first = P<unsigned long long>(c->d_ofirst); oa.last = P<unsigned long long>(c->d_olast);
-    oa.nfreq = P<int>(c->d_onf); oa.dftoff = P<long long>(c->d_ooff); oa.omega = P<double>(c->d_oomega);
-    oa.td = P<double>(c->d_otd); oa.tdsteps = c->totalSteps; oa.dre = P<double>(c->d_odre);
-    oa.dim = P<double>(c->d_odim);
-    for (int d = 0; d < 6; ++d) oa.F[d] = P<T>(c->F[d]);
+    for (int q = 0; q < 2; ++q) {
+      ObsArgs<T> &o = oa[q];
+      o.nobs = (int)c->pobs.size(); o.cell = P<long long>(c->d_ocell);
+      o.first = P<unsigned long long>(c->d_ofirst); o.last = P<unsigned long long>(c->d_olast);
+      o.nfreq = P<int>(c->d_onf); o.dftoff = P<long long>(c->d_ooff); o.omega = P<double>(c->d_oomega);
+      o.td = P<double>(c->d_otd); o.tdsteps = c->totalSteps; o.dre = P<double>(c->d_odre);
+      o.dim = P<double>(c->d_odim);
+      for (int d = 0; d < 6; ++d) o.F[d] = P<T>(c->fld(q, d));
+    }
     for (auto &o : c->pobs) maxnf = std::max(maxnf, (int)o.omega.size());
   }
-  VolObsArgs<T> va{};
+  VolObsArgs<T> va[2] = {};
   if (c->vnodes) {
-    va.nodecell = P<long long>(c->d_vcell); va.nodeobs = P<int>(c->d_vobs); va.nnodes = c->vnodes;
-    va.first = P<unsigned long long>(c->d_vfirst); va.last = P<unsigned long long>(c->d_vlast);
-    va.stage = P<float>(c->d_vstage);
-    for (int d = 0; d < 6; ++d) va.F[d] = P<T>(c->F[d]);
+    for (int q = 0; q < 2; ++q) {
+      VolObsArgs<T> &v = va[q];
+      v.nodecell = P<long long>(c->d_vcell); v.nodeobs = P<int>(c->d_vobs); v.nnodes = c->vnodes;
+      v.first = P<unsigned long long>(c->d_vfirst); v.last = P<unsigned long long>(c->d_vlast);
+      v.stage = P<float>(c->d_vstage);
+      for (int d = 0; d < 6; ++d) v.F[d] = P<T>(c->fld(q, d));
+    }
   }
 
-  auto ghost = [&](int electric) {
+  // ghosts copy within the new set
+  auto ghost = [&](int electric, int q) {
     for (int f = 0; f < 6; ++f) {
       const int t = c->ghost[f];
       if (!t) continue;
@@ -607,8 +735,8 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
       // E: PEC copy, PMC negate, periodic copy from the opposite side; H: PMC negate, periodic, PEC copy.
       g.src = t == 3 ? opp : inner;
       g.sign = (t == 2) ? T(-1) : T(1);
-      for (int q = 0; q < nc; ++q) {
-        g.F = P<T>(c->F[comps[q]]);
+      for (int u = 0; u < nc; ++u) {
+        g.F = P<T>(c->fld(c->next(q), comps[u]));
         hipLaunchKernelGGL(k_ghost<T>, gg, gb, 0, s, g);
       }
     }
@@ -618,64 +746,69 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
   // (step n = counter + offset within this run), so the same launches can be
   // captured once in a graph and replayed for later runs.
   unsigned long long *stepp = P<unsigned long long>(c->d_step);
-  oa.step = stepp; oa.dt = dt;
-  va.step = stepp;
+  for (int q = 0; q < 2; ++q) {
+    oa[q].step = stepp; oa[q].dt = dt;
+    va[q].step = stepp;
+    for (int e = 0; e < 2; ++e) { ce[q][e].step = stepp; ce[q][e].dt = dt; ce[q][e].half = e == 0 ? 0.0 : 0.5; }
+  }
   xa.step = stepp; xa.dt = dt;
-  for (int e = 0; e < 2; ++e) { ce[e].step = stepp; ce[e].dt = dt; ce[e].half = e == 0 ? 0.0 : 0.5; }
   int ne = 0, nh = 0;
-  for (uint64_t q = 0; q < nsteps; ++q) {
-    const int soff = (int)q;
+  for (uint64_t n = 0; n < nsteps; ++n) {
+    const int soff = (int)n;
+    const int q = c->leap ? c->par ^ (int)(n & 1) : c->par;  // field set this step reads
     // observers, and the aux grids' E update as extra blocks (k_head)
-    if (oa.nobs || anyAux) {
-      oa.soff = soff;
+    if (oa[q].nobs || anyAux) {
+      oa[q].soff = soff;
       xa.soff = soff;
-      const unsigned gy = (unsigned)(oa.nobs + (anyAux ? xa.npw : 0));
-      hipLaunchKernelGGL(k_head<T>, dim3(oa.nobs ? (maxnf + 63) / 64 : 1, gy), dim3(64), 0, s, oa, xa, pi);
+      const unsigned gy = (unsigned)(oa[q].nobs + (anyAux ? xa.npw : 0));
+      hipLaunchKernelGGL(k_head<T>, dim3(oa[q].nobs ? (maxnf + 63) / 64 : 1, gy), dim3(64), 0, s, oa[q], xa, pi);
     }
-    if (va.nnodes) {
-      va.soff = soff;
-      hipLaunchKernelGGL(k_observe_volume<T>, dim3((unsigned)((va.nnodes + 255) / 256)), dim3(256), 0, s, va);
+    if (va[q].nnodes) {
+      va[q].soff = soff;
+      hipLaunchKernelGGL(k_observe_volume<T>, dim3((unsigned)((va[q].nnodes + 255) / 256)), dim3(256), 0, s, va[q]);
     }
-    if (c->murmask)
-      hipLaunchKernelGGL(k_mur_e<T>, dim3((murmax1 + 15) / 16, (murmax2 + 15) / 16, 12), dim3(16, 16), 0, s, ma);
-    // events bracket the fast kernels (the dominant launches)
+    if (c->murmask) hipLaunchKernelGGL(k_mur_e<T>, murGrid, dim3(16, 16), 0, s, ma[q]);
+    // events bracket the fast kernels (the dominant launches): the fused
+    // sweep and its fix-up, or the fast E and H kernels
     if (timeKernels && ne < (int)c->evE.size() / 2) hipEventRecord(c->evE[2 * ne], s);
-    fast_e();
+    if (c->leap) leap(q);
+    else fast_e(q);
     if (timeKernels && ne < (int)c->evE.size() / 2) { hipEventRecord(c->evE[2 * ne + 1], s); ++ne; }
-    shell_e();
+    shell_e(q);
     // Layers 0 and 1 hold each cell's first block, inside and outside the
     // fast E box, so their sites never share an edge: without fusion they
     // go in one launch.  Later layers follow one launch each, in block order.
     for (int l = c->debyeFused ? 1 : 0; l < c->debyeLayers;) {
       const int l1 = (!c->debyeFused && l == 0 && c->debyeLayers > 1) ? 2 : l + 1;
-      da.base = c->debyeLayerStart[l];
-      da.n = c->debyeLayerStart[l1] - c->debyeLayerStart[l];
+      da[q].base = c->debyeLayerStart[l];
+      da[q].n = c->debyeLayerStart[l1] - c->debyeLayerStart[l];
       l = l1;
-      if (da.n == 0) continue;
-      hipLaunchKernelGGL(k_debye<T>, dim3((unsigned)((da.n + 255) / 256)), dim3(256), 0, s, da);
+      if (da[q].n == 0) continue;
+      hipLaunchKernelGGL(k_debye<T>, dim3((unsigned)((da[q].n + 255) / 256)), dim3(256), 0, s, da[q]);
     }
-    if (c->sibcFaces)
-      hipLaunchKernelGGL(k_sibc_e<T>, dim3((unsigned)c->sibcBlocks), dim3(128), 0, s, se);
-    if (ce[0].nrows) {
-      ce[0].soff = soff;
-      hipLaunchKernelGGL(k_corr<T>, dim3((unsigned)((ce[0].nrows + 255) / 256)), dim3(256), 0, s, ce[0], pi);
+    if (c->sibcFaces) hipLaunchKernelGGL(k_sibc_e<T>, dim3((unsigned)c->sibcBlocks), dim3(128), 0, s, se[q]);
+    if (ce[q][0].nrows) {
+      ce[q][0].soff = soff;
+      hipLaunchKernelGGL(k_corr<T>, dim3((unsigned)((ce[q][0].nrows + 255) / 256)), dim3(256), 0, s, ce[q][0], pi);
     }
-    ghost(1);
+    ghost(1, q);
+    if (c->leap) shell_h(q);
     if (timeKernels && nh < (int)c->evH.size() / 2) hipEventRecord(c->evH[2 * nh], s);
-    fast_h();
+    if (c->leap) fix(q);
+    else fast_h(q);
     if (timeKernels && nh < (int)c->evH.size() / 2) { hipEventRecord(c->evH[2 * nh + 1], s); ++nh; }
-    shell_h();
+    if (!c->leap) shell_h(q);
     if (c->sibcFaces)
-      hipLaunchKernelGGL(k_sibc_h<T>, dim3((unsigned)((sh.nrows + 255) / 256)), dim3(256), 0, s, sh);
+      hipLaunchKernelGGL(k_sibc_h<T>, dim3((unsigned)((sh[q].nrows + 255) / 256)), dim3(256), 0, s, sh[q]);
     // H corrections, and the aux grids' H update as extra blocks
-    if (ce[1].nrows || anyAux) {
-      ce[1].soff = soff;
-      const int nb = (int)((ce[1].nrows + 255) / 256);
+    if (ce[q][1].nrows || anyAux) {
+      ce[q][1].soff = soff;
+      const int nb = (int)((ce[q][1].nrows + 255) / 256);
       const dim3 g((unsigned)(nb + (anyAux ? xa.npw : 0)));
-      if (anyAux) hipLaunchKernelGGL(k_corr_aux<T>, g, dim3(256), 0, s, ce[1], pi, xa, nb);
-      else hipLaunchKernelGGL(k_corr<T>, g, dim3(256), 0, s, ce[1], pi);
+      if (anyAux) hipLaunchKernelGGL(k_corr_aux<T>, g, dim3(256), 0, s, ce[q][1], pi, xa, nb);
+      else hipLaunchKernelGGL(k_corr<T>, g, dim3(256), 0, s, ce[q][1], pi);
     }
-    ghost(0);
+    ghost(0, q);
   }
   hipLaunchKernelGGL(k_advance_step, dim3(1), dim3(64), 0, s, stepp, (unsigned long long)nsteps);
   hipError_t err = hipGetLastError();
@@ -711,8 +844,8 @@ int check_limits(Ctx *c, const int folim[6][6], vgpu_limits_report *rep, unsigne
     HArgs<T> ha;
     unsigned long long init[24];
     for (int q = 0; q < 24; ++q) init[q] = (q & 3) == 3 ? ~0ull : 0ull;
-    bool ok = hipMemcpy(&ea, c->d_eargs, sizeof ea, hipMemcpyDeviceToHost) == hipSuccess &&
-              hipMemcpy(&ha, c->d_hargs, sizeof ha, hipMemcpyDeviceToHost) == hipSuccess;
+    bool ok = hipMemcpy(&ea, c->d_eargs[c->par], sizeof ea, hipMemcpyDeviceToHost) == hipSuccess &&
+              hipMemcpy(&ha, c->d_hargs[c->par], sizeof ha, hipMemcpyDeviceToHost) == hipSuccess;
     VisitRun vr{};
     for (int q = 0; q < 6; ++q) vr.vis[q] = vis + q * cells;
     for (int d = 0; d < 3; ++d) { ea.vis[d] = vr.vis[d]; ha.vis[d] = vr.vis[3 + d]; }
