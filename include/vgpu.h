@@ -219,6 +219,20 @@ typedef struct {
 } vgpu_limits_report;
 int vgpu_check_limits(vgpu_ctx *c, const int folim[6][6], vgpu_limits_report *out, unsigned char *counts);
 
+/* The update-once check without allocating device memory.  No counterpart in
+ * the reference: it serves callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * results and synchronization as vgpu_check_limits; its buffers (the visit
+ * counts, 4 bytes per component and device cell, and the staging of
+ * `counts`) come from a fixed scratch area of vgpu_noalloc_scratch_bytes()
+ * bytes that the library holds on each device it runs on until the process
+ * ends.  A grid whose check needs more returns VGPU_EINVAL and says how much
+ * in vgpu_last_error.  All calls on one device share its scratch: calls that
+ * overlap on one device race, so serialize them. */
+#define VGPU_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int vgpu_check_limits_noalloc(vgpu_ctx *c, const int folim[6][6], vgpu_limits_report *out, unsigned char *counts);
+size_t vgpu_noalloc_scratch_bytes(void);
+
 const char *vgpu_last_error(const vgpu_ctx *c);
 void vgpu_destroy(vgpu_ctx *c);
 

@@ -82,6 +82,10 @@ def load_host():
         v.vgpu_check_limits.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(LimitsReport),
                                         ctypes.c_void_p]
         v.vgpu_check_limits.restype = ctypes.c_int
+        v.vgpu_check_limits_noalloc.argtypes = v.vgpu_check_limits.argtypes
+        v.vgpu_check_limits_noalloc.restype = ctypes.c_int
+        v.vgpu_noalloc_scratch_bytes.argtypes = []
+        v.vgpu_noalloc_scratch_bytes.restype = ctypes.c_size_t
     return _host
 
 
@@ -198,20 +202,22 @@ class DeviceSolve:
         load_host().vh_solve_field_limits(self._s, lim)
         return [list(lim[6 * c:6 * c + 6]) for c in range(6)]
 
-    def check_limits(self, counts=True):
+    def check_limits(self, counts=True, noalloc=False):
         """Update-once check of the field sweeps (vgpu_check_limits): one step
         of Mur E, the E sweep and the H sweep counts its updates per cell,
         tallied against the outer field limits; no field changes.  Returns a
         dict of per-component lists (missed, repeated, outside, first) and,
-        with counts, the visits per cell as uint8 [6][nx][ny][nz]."""
+        with counts, the visits per cell as uint8 [6][nx][ny][nz].  With
+        noalloc the entry point that allocates nothing runs the check
+        (vgpu_check_limits_noalloc)."""
         import numpy as np
         lim = (ctypes.c_int * 36)()
         load_host().vh_solve_field_limits(self._s, lim)
         rep = LimitsReport()
         nx, ny, nz = self.grid_shape()
         buf = np.zeros(6 * nx * ny * nz, dtype=np.uint8) if counts else None
-        self._check(self._v.vgpu_check_limits(self._c, lim, ctypes.byref(rep),
-                                              buf.ctypes.data if counts else None))
+        fn = self._v.vgpu_check_limits_noalloc if noalloc else self._v.vgpu_check_limits
+        self._check(fn(self._c, lim, ctypes.byref(rep), buf.ctypes.data if counts else None))
         out = {k: [int(x) for x in getattr(rep, k)] for k, _ in LimitsReport._fields_}
         if counts:
             return out, buf.reshape(6, nz, ny, nx).transpose(0, 3, 2, 1)

@@ -14,6 +14,7 @@
 #include <cstring>
 #include <array>
 #include <map>
+#include <mutex>
 #include <unordered_map>
 #include "vgpu_kernels.h"
 
@@ -182,6 +183,39 @@ using namespace vgpu;
 
 struct vgpu_ctx : public Ctx {};
 
+// Scratch of the entry point that allocates nothing (vgpu_check_limits_noalloc):
+// one file-scope device array, present on every device the library runs on
+// until the process ends.  Host code never uses its name as an address: the
+// device address of each device's copy comes from hipGetSymbolAddress, taken
+// once per device with that device current (scratch_setup, called by
+// vgpu_create) and kept in g_scratch.  Every call on one device shares that
+// device's copy; calls that overlap on one device race.
+static __device__ __align__(256) unsigned char scratch[VGPU_NOALLOC_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static void *g_scratch[64];
+
+// The device address of `device`'s scratch, looked up once (device current).
+static int scratch_setup(int device)
+{
+  if (device < 0 || device >= 64) return VGPU_EINVAL;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[device]) return VGPU_OK;
+  void *p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+    (void)hipGetLastError();
+    return VGPU_EDEVICE;
+  }
+  g_scratch[device] = p;
+  return VGPU_OK;
+}
+
+static void *scratch_of_device(int device)
+{
+  if (device < 0 || device >= 64) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch[device];
+}
+
 extern "C" {
 
 int vgpu_create(const vgpu_grid_desc *g, vgpu_prec prec, int device, vgpu_ctx **out)
@@ -190,6 +224,8 @@ int vgpu_create(const vgpu_grid_desc *g, vgpu_prec prec, int device, vgpu_ctx **
   *out = nullptr;
   if (g->nx < 3 || g->ny < 3 || g->nz < 3) return VGPU_EINVAL;
   if (hipSetDevice(device) != hipSuccess) return VGPU_EDEVICE;
+  // this device's no-allocation scratch address, with the device current
+  if (scratch_setup(device) != VGPU_OK) return VGPU_EDEVICE;
   vgpu_ctx *c = new vgpu_ctx();
   c->prec = prec;
   c->device = device;
@@ -1356,7 +1392,8 @@ int vgpu_download_field(vgpu_ctx *c, int component, double *host)
   return VGPU_OK;
 }
 
-int vgpu_check_limits(vgpu_ctx *c, const int folim[6][6], vgpu_limits_report *out, unsigned char *counts)
+static int check_limits_entry(vgpu_ctx *c, const int folim[6][6], vgpu_limits_report *out, unsigned char *counts,
+                              void *scratch, size_t scratchBytes)
 {
   if (!c || !folim || !out) return VGPU_EINVAL;
   if (!c->finalized) return VGPU_ESTATE;
@@ -1369,8 +1406,25 @@ int vgpu_check_limits(vgpu_ctx *c, const int folim[6][6], vgpu_limits_report *ou
     return set_error(c, VGPU_EINVAL, "update-once check: more than 65535 cells along y or z");
   hipSetDevice(c->device);
   (void)hipGetLastError();
-  return c->prec == VGPU_FP64 ? check_limits<double>(c, folim, out, counts) : check_limits<float>(c, folim, out, counts);
+  return c->prec == VGPU_FP64 ? check_limits<double>(c, folim, out, counts, scratch, scratchBytes)
+                              : check_limits<float>(c, folim, out, counts, scratch, scratchBytes);
 }
+
+int vgpu_check_limits(vgpu_ctx *c, const int folim[6][6], vgpu_limits_report *out, unsigned char *counts)
+{
+  return check_limits_entry(c, folim, out, counts, nullptr, 0);
+}
+
+int vgpu_check_limits_noalloc(vgpu_ctx *c, const int folim[6][6], vgpu_limits_report *out, unsigned char *counts)
+{
+  if (!c) return VGPU_EINVAL;
+  if (!c->finalized) return VGPU_ESTATE;
+  void *p = scratch_of_device(c->device);
+  if (!p) return set_error(c, VGPU_EDEVICE, "no-allocation scratch of device %d not set up", c->device);
+  return check_limits_entry(c, folim, out, counts, p, VGPU_NOALLOC_SCRATCH_BYTES);
+}
+
+size_t vgpu_noalloc_scratch_bytes(void) { return VGPU_NOALLOC_SCRATCH_BYTES; }
 
 size_t vgpu_footprint_bytes(const vgpu_ctx *c) { return c ? c->bytes : 0; }
 

@@ -601,8 +601,12 @@ template <typename T>
 int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool timeKernels, const VisitRun *chk = nullptr);
 // Uploads the sweeps' argument blocks (once, at finalize).
 template <typename T> int prepare_sweeps(Ctx *c);
-// Update-once check of the field sweeps against the outer field limits.
+// Update-once check of the field sweeps against the outer field limits.  Its
+// buffers come from `scratch` (scratchBytes long) when given, else they are
+// allocated for the call.
 template <typename T>
-int check_limits(Ctx *c, const int folim[6][6], vgpu_limits_report *rep, unsigned char *counts);
+int check_limits(Ctx *c, const int folim[6][6], vgpu_limits_report *rep, unsigned char *counts, void *scratch,
+                 size_t scratchBytes);
+template <typename T> size_t check_limits_bytes(const Ctx *c, bool counts, size_t off[4]);
 
 }  // namespace vgpu

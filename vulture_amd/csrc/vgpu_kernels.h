@@ -927,6 +927,12 @@ template <typename T, bool CHK = false> __global__ void k_mur_e(MurArgs<T> a)
   else a.E[c][id] = Eo[nb] + a.zeta[face] * (tmp - Eo[id]);
 }
 
+// Fill n unsigned values with v (grid-stride).
+static __global__ void k_fill_u32(unsigned *p, size_t n, unsigned v)
+{
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) p[q] = v;
+}
+
 // Tally of one component's visit counts against its outer field limits lim
 // (grid.c:951-1025 checks the same limits): cells inside that no sweep
 // updated, cells updated more than once, cells outside that were updated;

@@ -822,8 +822,23 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
 // which adds one to a per-cell count for each update instead of storing it;
 // each component's counts are then tallied against its outer field limits
 // folim.  Fields, UPML auxiliaries and Debye state are left as they were.
+// Bytes of device memory one update-once check uses: visit counts per
+// component (device layout), the two sweep argument blocks, the tally and,
+// with counts, their staging in the counts layout; each part 256-byte aligned.
+template <typename T> size_t check_limits_bytes(const Ctx *c, bool counts, size_t off[4])
+{
+  auto al = [](size_t n) { return (n + 255) / 256 * 256; };
+  const size_t cells = (size_t)c->plane * c->NZ, ncell = (size_t)c->NX * c->NY * c->NZ;
+  off[0] = 0;
+  off[1] = al(6 * cells * sizeof(unsigned));
+  off[2] = off[1] + al(al(sizeof(EArgs<T>)) + sizeof(HArgs<T>));
+  off[3] = off[2] + al(24 * sizeof(unsigned long long));
+  return off[3] + (counts ? al(6 * ncell) : 0);
+}
+
 template <typename T>
-int check_limits(Ctx *c, const int folim[6][6], vgpu_limits_report *rep, unsigned char *counts)
+int check_limits(Ctx *c, const int folim[6][6], vgpu_limits_report *rep, unsigned char *counts, void *scratch,
+                 size_t scratchBytes)
 {
   const size_t cells = (size_t)c->plane * c->NZ;      // device layout, padded rows
   const size_t ncell = (size_t)c->NX * c->NY * c->NZ; // counts layout, x fastest
@@ -833,11 +848,30 @@ int check_limits(Ctx *c, const int folim[6][6], vgpu_limits_report *rep, unsigne
   unsigned char *dcounts = nullptr;
   int r = VGPU_OK;
   const size_t hoff = (sizeof(EArgs<T>) + 255) / 256 * 256;
+  size_t off[4];
+  const size_t need = check_limits_bytes<T>(c, counts != nullptr, off);
+  if (scratch && need > scratchBytes)
+    return set_error(c, VGPU_EINVAL,
+                     "update-once check of this grid needs %zu bytes of scratch, more than the %zu the "
+                     "no-allocation entry point holds",
+                     need, scratchBytes);
   if (hipDeviceSynchronize() != hipSuccess) return set_error(c, VGPU_EDEVICE, "device error before the check");
-  if (hipMalloc((void **)&vis, 6 * cells * sizeof(unsigned)) != hipSuccess ||
-      hipMalloc((void **)&args, hoff + sizeof(HArgs<T>)) != hipSuccess ||
-      hipMalloc((void **)&tally, 24 * sizeof(unsigned long long)) != hipSuccess ||
-      (counts && hipMalloc((void **)&dcounts, 6 * ncell) != hipSuccess)) {
+  bool have;
+  if (scratch) {
+    // the caller's buffer: no allocation
+    char *b = static_cast<char *>(scratch);
+    vis = reinterpret_cast<unsigned *>(b + off[0]);
+    args = b + off[1];
+    tally = reinterpret_cast<unsigned long long *>(b + off[2]);
+    dcounts = counts ? reinterpret_cast<unsigned char *>(b + off[3]) : nullptr;
+    have = true;
+  } else {
+    have = hipMalloc((void **)&vis, 6 * cells * sizeof(unsigned)) == hipSuccess &&
+           hipMalloc((void **)&args, hoff + sizeof(HArgs<T>)) == hipSuccess &&
+           hipMalloc((void **)&tally, 24 * sizeof(unsigned long long)) == hipSuccess &&
+           (!counts || hipMalloc((void **)&dcounts, 6 * ncell) == hipSuccess);
+  }
+  if (!have) {
     r = set_error(c, VGPU_ENOMEM, "update-once check buffers");
   } else {
     EArgs<T> ea;
@@ -851,7 +885,11 @@ int check_limits(Ctx *c, const int folim[6][6], vgpu_limits_report *rep, unsigne
     for (int d = 0; d < 3; ++d) { ea.vis[d] = vr.vis[d]; ha.vis[d] = vr.vis[3 + d]; }
     vr.eargs = args;
     vr.hargs = args + hoff;
-    ok = ok && hipMemset(vis, 0, 6 * cells * sizeof(unsigned)) == hipSuccess &&
+    // the counts start at zero (a fill kernel: no memset node if captured)
+    const size_t nv = 6 * cells;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)std::min<size_t>((nv + 255) / 256, 65535)), dim3(256), 0, c->stream,
+                       vis, nv, 0u);
+    ok = ok && hipStreamSynchronize(c->stream) == hipSuccess &&
          hipMemcpy(args, &ea, sizeof ea, hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(args + hoff, &ha, sizeof ha, hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(tally, init, sizeof init, hipMemcpyHostToDevice) == hipSuccess;
@@ -880,10 +918,12 @@ int check_limits(Ctx *c, const int folim[6][6], vgpu_limits_report *rep, unsigne
       }
     }
   }
-  (void)hipFree(vis);
-  (void)hipFree(args);
-  (void)hipFree(tally);
-  (void)hipFree(dcounts);
+  if (!scratch) {
+    (void)hipFree(vis);
+    (void)hipFree(args);
+    (void)hipFree(tally);
+    (void)hipFree(dcounts);
+  }
   return r;
 }
 
