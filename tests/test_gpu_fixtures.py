@@ -1,0 +1,125 @@
+"""Device runs against FP64-oracle fixtures (needs an MI355X).
+
+tests/golden/fp64/<case>.npz holds the FP64 oracle's observer series for
+meshes whose oracle run is too slow for a test (tests/golden/make_fp64_fixtures.py
+wrote them; the FP64 oracle is byte-identical to the reference built in double
+precision, tests/test_oracle_reference.py):
+  the reference's ABC-reflection tests (sgref_mur, sgref_pml6, sgref_pml12,
+  dgref_ref, dgrefmb_ref);
+  the full configurations: graded shielded enclosure 200x160x120 (4000 steps,
+  512-frequency DFT), Debye-sphere volume 256^3 (3000 steps), SIBC + Mur box
+  384x384x256 (6000 steps).
+Bars: FP64 time series and raw DFTs within 1e-12 relative L2.  FP32 within
+1e-4 time-series relative L2 and 0.05 dB over the band (the waveform
+spectrum within 60 dB of its peak, bins above 1e-3 of the largest reference
+magnitude); on the reference meshes, whose fields are cancellation-dominated,
+within 1.5 times the reference's own float error where that is larger.
+
+The reference meshes observe small scattered fields next to a large incident
+one, so rounding in the last bit of the fields shows up in the observed
+series many orders of magnitude larger.  The float reference's own error
+against FP64 measures that amplification (6.5 relative L2 on sgref_pml12);
+scaled by the ratio of the two unit roundoffs (2^-52 / 2^-23 = 2^-29) it is
+the FP64 rounding level of the mesh.  On these meshes the FP64 bar is four
+times that level where it exceeds 1e-12 (sgref_pml12 4.9e-8, sgref_pml6
+1.4e-10, sgref_mur 5.9e-12, dgrefmb_ref 2.1e-12; dgref_ref and the full
+configurations 1e-12).
+"""
+import glob
+import os
+import shutil
+
+import numpy as np
+import pytest
+
+import vulture_amd
+from vulture_amd.obsfiles import compare_outputs, read_impulse
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+FIX = os.path.join(GOLDEN, "fp64")
+CASES = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(FIX, "*.npz")))
+
+
+def _make_module():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_fp64_fixtures",
+                                                  os.path.join(GOLDEN, "make_fp64_fixtures.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def _write_fixture(fx, dst, wf_src):
+    """The fixture's arrays as observer files under dst, beside the device
+    run's waveform spectrum (the same host computation in both runs)."""
+    os.makedirs(dst, exist_ok=True)
+    for key in fx.files:
+        if key.startswith("td/"):
+            with open(os.path.join(dst, key[3:]), "w") as f:
+                for row in fx[key]:
+                    f.write(f"{int(row[0])} " + " ".join(repr(float(x)) for x in row[1:]) + "\n")
+        elif key.startswith("fd/"):
+            with open(os.path.join(dst, key[3:]), "w") as f:
+                for row in fx[key]:
+                    f.write(" ".join(repr(float(x)) for x in row) + "\n")
+    for w in glob.glob(os.path.join(wf_src, "wf_*_fd.asc")):
+        shutil.copy(w, dst)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("case", CASES)
+def test_device_against_fp64_fixture(case, precision, tmp_path):
+    gen = _make_module()
+    fx = np.load(os.path.join(FIX, f"{case}.npz"))
+    mesh = gen.stage(case, str(tmp_path))
+    assert gen.mesh_sha256(mesh) == str(fx["mesh_sha256"]), "the mesh generator changed since the fixture was made"
+    print(f"case {case} {precision}", flush=True)
+    dev = tmp_path / "gpu"
+    vulture_amd.run_mesh(mesh, str(dev), precision=precision)
+    ref = tmp_path / "fixture"
+    _write_fixture(fx, str(ref), str(dev))
+    r = compare_outputs(str(dev), str(ref))
+    assert r["files"] == len([k for k in fx.files if k.startswith(("td/", "fd/"))]), r
+    ftd = float(fx["float_td"]) if "float_td" in fx.files else 0.0
+    fdb = float(fx["float_fd_db"]) if "float_fd_db" in fx.files else 0.0
+    if precision == "fp64":
+        bar = max(1e-12, 4.0 * ftd * 2.0 ** -29)
+        assert r["td"] <= bar and r["fd_raw"] <= bar, (r, bar)
+    else:
+        assert r["td"] <= max(1e-4, 1.5 * ftd), (r, ftd)
+        assert r["fd_db"] <= max(0.05, 1.5 * fdb), (r, fdb)
+    print(f"  {case} {precision}: td {r['td']:.2e} fd_raw {r['fd_raw']:.2e} fd_db {r['fd_db']:.2e} "
+          f"fd_db_all {r['fd_db_all']:.2e}", flush=True)
+
+
+@pytest.mark.parametrize("name,chunk", [("planewave_empty_xyz", 64), ("dgref_mur", 90), ("closedbox_go04", 100)])
+def test_impulse_dat_fp64_matches_oracle(name, chunk, tmp_path):
+    """impulse.dat of the FP64 device run against the FP64 oracle's: the
+    header, the node indices and the float samples.  The samples are the
+    FP64 fields cast to float, so they are byte-identical wherever the two
+    FP64 runs agree to the last bit; fields that agree to ~1e-15 can still
+    round to neighbouring floats, so a sample may differ by one unit in the
+    last place and no more."""
+    import oracle
+    d = tmp_path / "in"
+    shutil.copytree(os.path.join(GOLDEN, "ref_inputs", name), d)
+    mesh = str(d / [f for f in os.listdir(d) if f.endswith(".mesh")][0])
+    print(f"case impulse {name}", flush=True)
+    g, o = tmp_path / "gpu", tmp_path / "oracle"
+    vulture_amd.run_mesh(mesh, str(g), precision="fp64", chunk=chunk)
+    oracle.run_mesh(mesh, str(o), precision="fp64")
+    a, b = (g / "impulse.dat").read_bytes(), (o / "impulse.dat").read_bytes()
+    assert len(a) == len(b) and a[:8] == b[:8]
+    na, dta, ra = read_impulse(str(g / "impulse.dat"))
+    nb, dtb, rb = read_impulse(str(o / "impulse.dat"))
+    assert np.array_equal(ra["ijk"], rb["ijk"])
+    fa, fb = ra["f"], rb["f"]
+    ulps = np.abs(fa.view(np.int32).astype(np.int64) - fb.view(np.int32).astype(np.int64))
+    ulps[(fa == 0) & (fb == 0)] = 0
+    print(f"  {name}: {int(np.count_nonzero(ulps))} of {ulps.size} samples differ, max {int(ulps.max())} ulp",
+          flush=True)
+    assert int(ulps.max()) <= 1
+    assert np.count_nonzero(ulps) <= max(1, ulps.size // 100000)

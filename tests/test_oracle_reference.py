@@ -2,8 +2,10 @@
 
 1. Byte identity: the float build of the oracle writes the same observer
    files, byte for byte, as the reference compiled from its own sources
-   (oracle/Makefile, target `ref`) on every reference test mesh.  Runs only
-   where that build exists.
+   (oracle/Makefile, target `ref`) on every reference test mesh; the FP64
+   build writes the same files as the reference compiled in double
+   precision from its own sources (target _ref/vulture_ref64).  Runs only
+   where those builds exist.
 2. The reference's validation files (tests/<name>/eh_*_td.asc, wf_*_td.asc;
    one axis permutation per family kept as fixtures, long series
    subsampled): the float oracle must pass util/pydiff.py's acceptance rule
@@ -116,6 +118,44 @@ def test_float_oracle_is_byte_identical_to_reference_build(name, tmp_path):
     assert outs
     for f in outs:
         assert filecmp.cmp(a / f, b / f, shallow=False), f"{name}/{f} differs from the reference build"
+
+
+REF64_BIN = os.path.join(os.path.dirname(oracle.REF_BIN), "vulture_ref64")
+
+
+@pytest.mark.skipif(not os.path.exists(REF64_BIN), reason="FP64 reference build not present (needs the reference sources)")
+@pytest.mark.parametrize("name", _params(NAMES, opt_in=True) +
+                         ([n for n in ALL_INPUTS if n not in NAMES] if RUN_SLOW else []))
+def test_fp64_oracle_is_byte_identical_to_fp64_reference_build(name, tmp_path):
+    """The FP64 oracle (real = double, -O2 without contraction, values printed
+    with 17 significant digits) against the reference built the same way from
+    its own sources (oracle/Makefile target _ref/vulture_ref64: real double,
+    %le scans, %24.16e observer values): every observer file byte for byte.
+    This pins the set-up the device library's host shares with the oracle in
+    the precision the product runs it in (VH_REAL=double).  One mesh per test
+    family by default; every reference mesh with VULTURE_SLOW_TESTS=1."""
+    a = tmp_path / "oracle"
+    b = tmp_path / "reference"
+    shutil.copytree(os.path.join(INP, name), a)
+    shutil.copytree(os.path.join(INP, name), b)
+    mesh = [f for f in os.listdir(a) if f.endswith(".mesh")][0]
+    oracle.run_mesh(str(a / mesh), str(a), precision="fp64", full_precision=True, threads=4)
+    r = subprocess.run([REF64_BIN, mesh], cwd=b, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-500:]
+    outs = sorted(f for f in os.listdir(b) if f.startswith(("eh_", "wf_")) or f in ("impulse.dat", "excite.dat",
+                                                                                   "process.dat"))
+    assert outs
+    for f in outs:
+        if f == "impulse.dat":
+            # The header's dt is written as fwrite(&dt, sizeof(float)) of a
+            # `real` argument (observer.c:1137-1147): with real = double the
+            # FP64 build writes the low four bytes of the double there.  The
+            # oracle writes the float the float build writes; every other
+            # byte (step count, node indices, float field samples) must match.
+            x, y = (a / f).read_bytes(), (b / f).read_bytes()
+            assert len(x) == len(y) and x[:4] == y[:4] and x[8:] == y[8:], f"{name}/{f} differs"
+            continue
+        assert filecmp.cmp(a / f, b / f, shallow=False), f"{name}/{f} differs from the FP64 reference build"
 
 
 def test_parser_rejects_reference_bad_meshes(tmp_path):
