@@ -16,7 +16,7 @@ from vulture_amd import meshgen
 
 HOST = os.path.join(ROOT, "vulture_amd", "host")
 NATIVE = os.path.join(ROOT, "tests", "native")
-SRC = ["vh_parse.c", "vh_setup.c", "vh_observe.c", "vh_gpu.c", "vh_serve.c"]
+SRC = ["vh_parse.c", "vh_setup.c", "vh_voxel.c", "vh_observe.c", "vh_gpu.c", "vh_serve.c"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 

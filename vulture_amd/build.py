@@ -71,7 +71,7 @@ def build(verbose=False, force=True):
     libvgpu = os.path.join(LIB, "libvgpu.so")
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", libvgpu] + objs, verbose)
 
-    hsrc = ["vh_parse.c", "vh_setup.c", "vh_observe.c", "vh_gpu.c", "vh_serve.c"]
+    hsrc = ["vh_parse.c", "vh_setup.c", "vh_voxel.c", "vh_observe.c", "vh_gpu.c", "vh_serve.c"]
     hobjs = []
     for src in hsrc:
         s = os.path.join(HOST, src)

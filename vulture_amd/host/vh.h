@@ -24,6 +24,7 @@
 #include <stdio.h>
 #include <stdbool.h>
 #include <stddef.h>
+#include <stdint.h>
 #include <complex.h>
 
 #ifndef VH_REAL
@@ -110,6 +111,20 @@ typedef struct {
 
 typedef struct { int mbbox[6]; int medium; FaceMask mask; } vh_block;
 
+/* A voxel object (MT <name> VOXEL "<file>"): a key array and its key table. */
+typedef struct {
+  char name[VH_TAG];
+  int bytes, n[3];            /* bytes per key; voxels per axis */
+  double d[3], ref[3];        /* voxel size and reference point, m */
+  unsigned char *data;        /* n[0]*n[1]*n[2] little-endian keys, k fastest */
+  uint32_t *keys;             /* sorted keys ... */
+  int *keyMedium;             /* ... and their media */
+  int numKeys;
+} vh_voxel;
+
+/* One placement of a voxel object (MB <node> <object> [theta [phi]]). */
+typedef struct { int object, node[3]; double theta, phi; } vh_voxel_place;
+
 typedef struct { int mbbox[6]; int boundary; int orient; real angle; int gbbox[6]; } vh_surface;
 
 typedef struct { int mbbox[6]; int wire; } vh_line;
@@ -191,6 +206,9 @@ typedef struct vh_solve {
   char outputDir[VH_PATH];
   int fullPrecision;          /* write observer data with %24.16e instead of %16.8e */
   int openOutputs;            /* write observer files as the run goes */
+  int binaryStreamed;         /* excite.dat and impulse.dat rows are written by
+                                 vh_observe_binary_step as runs complete, not by
+                                 vh_observe_step */
 
   /* physical constants (physical.c:37-48), in `real` */
   real pi, c0, eps0, mu0, eta0;
@@ -224,6 +242,8 @@ typedef struct vh_solve {
   vh_boundary *bounds; int numBounds, capBounds;
   int extBoundary[6];
   vh_block *blocks; int numBlocks, capBlocks;
+  vh_voxel *voxels; int numVoxels, capVoxels;
+  vh_voxel_place *voxelPlaces; int numVoxelPlaces, capVoxelPlaces;
   vh_surface *surfaces; int numSurfaces, capSurfaces;
   char (*wireNames)[VH_TAG]; int *wireTypes; int numWires, capWires;
   vh_line *lines; int numLinesTW, capLinesTW;
@@ -282,6 +302,7 @@ int vh_open_observers(vh_solve *s);
  * device holds them; observers, waveforms and grid lines stay. */
 void vh_release_grid_arrays(vh_solve *s);
 void vh_observe_step(vh_solve *s, unsigned long n, real t, const double *pointVals, const float *binVals);
+void vh_observe_binary_step(vh_solve *s, unsigned long n, real t, const float *binVals);
 void vh_observer_point_values(const vh_solve *s, double *vals);
 void vh_close_observers(vh_solve *s);
 int vh_num_point_observers(const vh_solve *s, int domain);

@@ -55,6 +55,14 @@ int vh_bbox_direction(const int bbox[6]);
 bool vh_field_is_in_boundary(int field, int face);
 real vh_numerical_phase_velocity(vh_solve *s, real theta, real phi);
 
+/* Voxel objects (vh_voxel.c).  A placement's block carries the placeholder
+ * medium vh_place_voxel returns until vh_expand_voxels replaces it. */
+int vh_find_voxel(const vh_solve *s, const char *name);
+void vh_load_voxel(vh_solve *s, const char *name, const char *file);
+int vh_place_voxel(vh_solve *s, int object, const int node[3], double theta, double phi);
+void vh_expand_voxels(vh_solve *s);
+void vh_free_voxels(vh_solve *s);
+
 /* SIBC filter helpers (filter.c:309-368). */
 void vh_rc_from_pole_residue(vh_rcfilter *rc, int numPoles, double asymp, const double complex *poles,
                              const double complex *residues, double T);

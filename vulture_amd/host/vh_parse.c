@@ -379,7 +379,15 @@ static bool parse_MT(vh_solve *s, char *line)
   int numPoles = 0, type = MT_FREE_SPACE, n;
   bool found = false;
   if (sscanf(line, "%31s %31s", name, typeStr) < 2) return false;
-  if (vh_find_medium(s, name) >= 0) return false;
+  if (vh_find_medium(s, name) >= 0 || vh_find_voxel(s, name) >= 0) return false;
+  if (strncmp(typeStr, "VOXEL", VH_TAG) == 0) {
+    /* voxel object loader (doc/ToDo.md "Voxel object loader"; not in the reference's parser) */
+    if (sscanf(line, "%31s %31s \"%1023[^\"]\"", name, typeStr, fileName) != 3 &&
+        sscanf(line, "%31s %31s %1023s", name, typeStr, fileName) != 3)
+      return false;
+    vh_load_voxel(s, name, fileName);
+    return true;
+  }
   for (int t = 0; t < 4; t++)
     if (strncmp(typeStr, TYPES[t], VH_TAG) == 0) { type = t; found = true; }
   if (!found) return false;
@@ -429,13 +437,28 @@ static bool parse_MB(vh_solve *s, char *line)
   int n = sscanf(line, "%d %d %d %d %d %d %31s %31s", &b[0], &b[1], &b[2], &b[3], &b[4], &b[5], medium,
                  maskStr);
   if (n < 7) return false;
-  if (!bbox_is_normal(b) || !bbox_is_within(b, s->mbox) || vh_bbox_type(b) != BB_VOLUME) return false;
-  int m = vh_find_medium(s, medium);
-  if (m < 0) return false;
+  int m;
   FaceMask mask = FACE_MASK_ALL;
-  if (n <= 8) {
-    mask = face_mask_from_string(maskStr);
-    if (mask == FACE_MASK_ERROR) return false;
+  int v = vh_find_voxel(s, medium);
+  if (v >= 0) {
+    /* a voxel object placed with its reference point on a node, two optional angles */
+    double ang[2] = {0.0, 0.0};
+    char extra[VH_TAG];
+    int na = sscanf(line, "%*d %*d %*d %*d %*d %*d %*s %lf %lf %31s", &ang[0], &ang[1], extra);
+    if (na == 3 || !isfinite(ang[0]) || !isfinite(ang[1])) return false;
+    if (na <= 0 && n >= 8) return false;
+    if (!bbox_is_within(b, s->mbox) || vh_bbox_type(b) != BB_POINT) return false;
+    int node[3] = {b[XLO], b[YLO], b[ZLO]};
+    m = vh_place_voxel(s, v, node, ang[0], ang[1]);
+    mask = FACE_MASK_ALL;
+  } else {
+    if (!bbox_is_normal(b) || !bbox_is_within(b, s->mbox) || vh_bbox_type(b) != BB_VOLUME) return false;
+    m = vh_find_medium(s, medium);
+    if (m < 0) return false;
+    if (n <= 8) {
+      mask = face_mask_from_string(maskStr);
+      if (mask == FACE_MASK_ERROR) return false;
+    }
   }
   vh_grow(s, (void **)&s->blocks, &s->capBlocks, s->numBlocks + 1, sizeof(vh_block));
   vh_block *blk = &s->blocks[s->numBlocks++];

@@ -1242,6 +1242,7 @@ int vh_init(vh_solve *s)
   /* initSimulation (simulation.c:113-125) */
   if (s->courantNumber < 0.0) s->courantNumber = sqrt(3.0) / 2.0;
   if (!s->xlines || !s->ylines || !s->zlines) vh_fail(s, "mesh has no mesh lines");
+  vh_expand_voxels(s);
   /* initGrid (grid.c:327-355) */
   init_external_surface_parameters(s);
   set_grid_extents(s);
@@ -1325,6 +1326,7 @@ void vh_destroy(vh_solve *s)
       for (int f = 0; f < 16; f++) free_rc(&s->bounds[b].rc[f]);
   free(s->bounds);
   free(s->blocks);
+  vh_free_voxels(s);
   free(s->surfaces);
   free(s->wireNames);
   free(s->wireTypes);

@@ -110,8 +110,22 @@ def _spheres(rng, n, count, rmin, rmax, gap=2):
     return out
 
 
+def _sphere_runs(c, r):
+    """(i, j, k0, k1) runs of the cells whose centres lie inside the sphere."""
+    for i in range(c[0] - r, c[0] + r + 1):
+        for j in range(c[1] - r, c[1] + r + 1):
+            h2 = r * r - (i + 0.5 - c[0]) ** 2 - (j + 0.5 - c[1]) ** 2
+            if h2 <= 0:
+                continue
+            h = math.sqrt(h2)
+            k0 = math.ceil(c[2] - 0.5 - h)
+            k1 = math.floor(c[2] - 0.5 + h)
+            if k1 >= k0:
+                yield i, j, k0, k1
+
+
 def debye(steps=3000, n=256, spheres=200, pml=8, seed=3, rmin=4, rmax=20, observers=8, nfreq=64, boundary="PML",
-          planewave=True):
+          planewave=True, voxel=False):
     """Debye-sphere volume.  Each sphere is its own single-pole DEBYE medium
     (eps_inf ~ U[2,10], delta_eps ~ U[1,40], tau ~ logU[1e-11,1e-9] s,
     sigma ~ logU[1e-3,1] S/m; residue 0.5*delta_eps/tau, pole -1/tau) and is
@@ -120,7 +134,11 @@ def debye(steps=3000, n=256, spheres=200, pml=8, seed=3, rmin=4, rmax=20, observ
     YHI left to the neighbouring run), so no electric edge belongs to two
     Debye blocks.  `boundary` "PML" (the workload), or "PEC"/"MUR" on all
     faces, and `planewave` False (a hard point source instead) give variants
-    that isolate parts of the time loop when timing it."""
+    that isolate parts of the time loop when timing it.  `voxel` True places
+    the spheres as one voxel object instead (MT spheres VOXEL "spheres.vox",
+    key s+1 for sphere s in a 1- or 2-byte key array of the mesh's own pitch,
+    one MB card at node 0): the same cells, media and face masks, written as
+    spheres.vox and spheres.keys beside the mesh."""
     rng = np.random.default_rng(seed)
     sph = _spheres(rng, n, spheres, min(rmin, n // 4), min(rmax, n // 4))
     lines = ["VM 1.0.0", f"CE Debye spheres: {n}^3 cubic 2 mm, {len(sph)} inclusions, {boundary} boundaries",
@@ -133,17 +151,20 @@ def debye(steps=3000, n=256, spheres=200, pml=8, seed=3, rmin=4, rmax=20, observ
         tau = 10 ** rng.uniform(-11, -9)
         sig = 10 ** rng.uniform(-3, 0)
         lines.append(f"MT d{s} DEBYE {einf:.6e} {sig:.6e} 1.0 {0.5 * de / tau:.6e} {-1.0 / tau:.6e}")
-    for s, (c, r) in enumerate(sph):
-        for i in range(c[0] - r, c[0] + r + 1):
-            for j in range(c[1] - r, c[1] + r + 1):
-                h2 = r * r - (i + 0.5 - c[0]) ** 2 - (j + 0.5 - c[1]) ** 2
-                if h2 <= 0:
-                    continue
-                h = math.sqrt(h2)
-                k0 = math.ceil(c[2] - 0.5 - h)
-                k1 = math.floor(c[2] - 0.5 + h)
-                if k1 < k0:
-                    continue
+    files = {}
+    if voxel:
+        width = 1 if len(sph) < 256 else 2
+        keys = np.zeros((n, n, n), dtype=np.uint8 if width == 1 else "<u2")
+        for s, (c, r) in enumerate(sph):
+            for i, j, k0, k1 in _sphere_runs(c, r):
+                keys[i, j, k0:k1 + 1] = s + 1
+        files["spheres.keys"] = keys.tobytes()
+        files["spheres.vox"] = "\n".join(["spheres.keys", str(width), f"{n} {n} {n}", "2e-3 2e-3 2e-3", "0 0 0"] +
+                                         [f"{s + 1} d{s}" for s in range(len(sph))]) + "\n"
+        lines += ['MT spheres VOXEL "spheres.vox"', "MB 0 0 0 0 0 0 spheres"]
+    else:
+        for s, (c, r) in enumerate(sph):
+            for i, j, k0, k1 in _sphere_runs(c, r):
                 lines.append(f"MB {i} {i + 1} {j} {j + 1} {k0} {k1 + 1} d{s} 101011")
     theta = rng.uniform(0, 180)
     phi = rng.uniform(0, 360)
@@ -160,7 +181,8 @@ def debye(steps=3000, n=256, spheres=200, pml=8, seed=3, rmin=4, rmax=20, observ
         lines.append(f"OP {p[0]} {p[0]} {p[1]} {p[1]} {p[2]} {p[2]} t{o} TDOM_ASCII")
         lines.append(f"OP {p[0]} {p[0]} {p[1]} {p[1]} {p[2]} {p[2]} f{o} FDOM_ASCII")
     lines += ["GE", f"NT {steps}", f"OF 1e8 15e9 {nfreq}", "MS 2e-3", "EN"]
-    return "\n".join(lines) + "\n"
+    text = "\n".join(lines) + "\n"
+    return (text, files) if files else text
 
 
 def sibc_box(steps=6000, nx=384, ny=384, nz=256, dx=1.5e-3, blocks=64, seed=4, observers=4, nfreq=64):
@@ -225,7 +247,7 @@ def write(name, outdir, **kw):
     with open(path, "w") as f:
         f.write(text)
     for fn, body in files.items():
-        with open(os.path.join(outdir, fn), "w") as f:
+        with open(os.path.join(outdir, fn), "wb" if isinstance(body, bytes) else "w") as f:
             f.write(body)
     return path
 
