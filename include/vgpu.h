@@ -165,6 +165,20 @@ int vgpu_read_point_observer(vgpu_ctx *c, int id, double *td, uint64_t nsteps, d
  * the k, j, i node order of impulse.dat.  Returns the node count via *nodes. */
 int vgpu_read_volume_observer(vgpu_ctx *c, int id, float *out, uint64_t nsteps, uint64_t *nodes);
 
+/* Completion of one of the last two runs, without waiting for later ones.
+ * The reference writes volume samples to impulse.dat inside its time loop
+ * (src/observer.c:1164-1198); here each vgpu_run copies its samples into
+ * pinned host memory asynchronously, after its kernels on its stream, so a
+ * host can write them out while the next run proceeds.  For the run that
+ * began at step `first`: wait 1 blocks until it has ended, wait 0 does not.
+ * Returns 1 when it has ended (then *ms is its device time and *samples its
+ * volume observer samples, [step][node][6] floats over every volume observer
+ * in id order and the k, j, i node order of impulse.dat, or NULL without
+ * volume observers; valid until the second vgpu_run after that run is
+ * called), 0 when it has not (wait 0), <0 on error or when `first` names
+ * neither of the last two runs.  Any of ms, samples, nsteps may be NULL. */
+int vgpu_run_result(vgpu_ctx *c, uint64_t first, int wait, double *ms, const float **samples, uint64_t *nsteps);
+
 /* Copy one field component (0..5) back in the host k-fastest layout. */
 int vgpu_download_field(vgpu_ctx *c, int component, double *host);
 
@@ -191,6 +205,7 @@ typedef struct {
   int leap;                    /* 1: fused leapfrog sweep (E then H in one pass over the fast box) */
   uint64_t fix_cells;          /* H cells of the fused box updated again after the E corrections */
   uint64_t fix_chunks;         /* the 64-cell row chunks holding them */
+  uint64_t graph_runs;         /* runs replayed from a captured graph (VGPU_GRAPHS=1) */
 } vgpu_stats;
 int vgpu_get_stats(const vgpu_ctx *c, vgpu_stats *out);
 

@@ -2,8 +2,8 @@
  * Host-only stand-in for libvgpu, for testing the host's device driver and
  * solve server on a machine without a GPU (test infrastructure).  Every
  * entry point of include/vgpu.h is accepted; runs "complete" after a few
- * vgpu_query polls so the server's asynchronous paths are exercised;
- * observer reads return zeros.
+ * vgpu_query or vgpu_run_result polls so the server's asynchronous paths are
+ * exercised; observer reads return zeros.
  */
 #include <stdlib.h>
 #include <string.h>
@@ -23,6 +23,9 @@ struct vgpu_ctx {
   uint64_t total;
   int nf[256];
   uint64_t vnodes[256];
+  uint64_t maxrun, runs, rfirst[2], rn[2];
+  int rpolls[2], rvalid[2];
+  float *zeros;             /* samples of a run: maxrun x all volume nodes x 6 */
 };
 
 int vgpu_create(const vgpu_grid_desc *g, vgpu_prec prec, int device, vgpu_ctx **out)
@@ -82,8 +85,42 @@ int vgpu_add_volume_observer(vgpu_ctx *c, const int b[6], const int s[3], uint64
   if (c->nvol < 256) c->vnodes[c->nvol] = n;
   return c->nvol++;
 }
-int vgpu_finalize(vgpu_ctx *c, uint64_t total, uint64_t maxrun) { (void)maxrun; c->total = total; c->finalized = 1; return 0; }
-int vgpu_run(vgpu_ctx *c, uint64_t first, uint64_t n, void *s) { (void)first; (void)n; (void)s; c->polls = rand() % 4; return 0; }
+int vgpu_finalize(vgpu_ctx *c, uint64_t total, uint64_t maxrun)
+{
+  uint64_t nodes = 0;
+  for (int v = 0; v < c->nvol && v < 256; v++) nodes += c->vnodes[v];
+  c->total = total;
+  c->maxrun = maxrun ? maxrun : 1;
+  c->zeros = nodes ? calloc(c->maxrun * nodes * 6, sizeof(float)) : NULL;
+  c->finalized = 1;
+  return 0;
+}
+int vgpu_run(vgpu_ctx *c, uint64_t first, uint64_t n, void *s)
+{
+  (void)s;
+  const int q = (int)(c->runs++ & 1);
+  c->rfirst[q] = first;
+  c->rn[q] = n;
+  c->rvalid[q] = 1;
+  c->rpolls[q] = rand() % 4;
+  c->polls = rand() % 4;
+  return 0;
+}
+int vgpu_run_result(vgpu_ctx *c, uint64_t first, int wait, double *ms, const float **samples, uint64_t *nsteps)
+{
+  int q = -1;
+  for (int back = 1; back >= 0; back--) {
+    if ((uint64_t)back >= c->runs) continue;
+    const int slot = (int)((c->runs - 1 - (uint64_t)back) & 1);
+    if (c->rvalid[slot] && c->rfirst[slot] == first) q = slot;
+  }
+  if (q < 0) return VGPU_EINVAL;
+  if (!wait && c->rpolls[q]-- > 0) return 0;
+  if (ms) *ms = 0;
+  if (samples) *samples = c->zeros;
+  if (nsteps) *nsteps = c->rn[q];
+  return 1;
+}
 int vgpu_synchronize(vgpu_ctx *c) { c->polls = 0; return 0; }
 int vgpu_query(vgpu_ctx *c) { return c->polls-- <= 0 ? 1 : 0; }
 int vgpu_read_point_observer(vgpu_ctx *c, int id, double *td, uint64_t n, double *re, double *im)
@@ -107,4 +144,8 @@ int vgpu_get_stats(const vgpu_ctx *c, vgpu_stats *o) { (void)c; memset(o, 0, siz
 int vgpu_last_run_timing(const vgpu_ctx *c, double *t, double *e, double *h)
 { (void)c; if (t) *t = 0; if (e) *e = 0; if (h) *h = 0; return 0; }
 const char *vgpu_last_error(const vgpu_ctx *c) { (void)c; return ""; }
-void vgpu_destroy(vgpu_ctx *c) { free(c); }
+void vgpu_destroy(vgpu_ctx *c)
+{
+  if (c) free(c->zeros);
+  free(c);
+}

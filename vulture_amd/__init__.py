@@ -117,7 +117,8 @@ class Stats(ctypes.Structure):
                 ("correction_rows", ctypes.c_uint64 * 2), ("bytes", ctypes.c_uint64),
                 ("fast_cells", ctypes.c_uint64 * 2), ("debye_fused_sites", ctypes.c_uint64),
                 ("indexed_media", ctypes.c_int), ("debye_fused", ctypes.c_int), ("sibc_state", ctypes.c_int),
-                ("leap", ctypes.c_int), ("fix_cells", ctypes.c_uint64), ("fix_chunks", ctypes.c_uint64)]
+                ("leap", ctypes.c_int), ("fix_cells", ctypes.c_uint64), ("fix_chunks", ctypes.c_uint64),
+                ("graph_runs", ctypes.c_uint64)]
 
 
 class LimitsReport(ctypes.Structure):
@@ -136,7 +137,7 @@ class DeviceSolve:
     time the E and H volume kernels (``kernel_ms()``).
     """
 
-    def __init__(self, mesh, precision="fp32", device=0, steps=-1, kernel_events=0):
+    def __init__(self, mesh, precision="fp32", device=0, steps=-1, kernel_events=0, chunk=0):
         if precision not in ("fp64", "fp32"):
             raise ValueError("precision must be 'fp64' or 'fp32'")
         host = load_host()
@@ -147,7 +148,7 @@ class DeviceSolve:
         self._c = ctypes.c_void_p()
         err = ctypes.create_string_buffer(1024)
         try:
-            rc = host.vh_gpu_open(os.fsencode(mesh), 1 if precision == "fp32" else 0, device, steps, 0,
+            rc = host.vh_gpu_open(os.fsencode(mesh), 1 if precision == "fp32" else 0, device, steps, chunk,
                                   ctypes.byref(self._s), ctypes.byref(self._c), err, len(err))
         finally:
             if kernel_events:

@@ -15,6 +15,7 @@
 #include <array>
 #include <map>
 #include <mutex>
+#include <shared_mutex>
 #include <unordered_map>
 #include "vgpu_kernels.h"
 
@@ -216,11 +217,24 @@ static void *scratch_of_device(int device)
   return g_scratch[device];
 }
 
+// A stream capture must not overlap a synchronization of the whole device
+// (HIP follows CUDA here: synchronizing a device while any of its streams is
+// being captured is invalid, and hipFree and hipDeviceSynchronize synchronize
+// the device).  One solve's thread may capture while other threads set up,
+// read back or destroy other solves, so the entry points that synchronize
+// the device or the null stream -- allocation, synchronous copies, frees --
+// hold this lock shared, and a capture holds it alone.
+static std::shared_mutex g_capture_mu;
+using DeviceCalls = std::shared_lock<std::shared_mutex>;
+
 extern "C" {
+
+static void destroy_ctx(vgpu_ctx *c);
 
 int vgpu_create(const vgpu_grid_desc *g, vgpu_prec prec, int device, vgpu_ctx **out)
 {
   if (!g || !out) return VGPU_EINVAL;
+  const DeviceCalls guard(g_capture_mu);
   *out = nullptr;
   if (g->nx < 3 || g->ny < 3 || g->nz < 3) return VGPU_EINVAL;
   if (hipSetDevice(device) != hipSuccess) return VGPU_EDEVICE;
@@ -274,11 +288,15 @@ int vgpu_create(const vgpu_grid_desc *g, vgpu_prec prec, int device, vgpu_ctx **
   }
   hipEventCreate(&c->ev0);
   hipEventCreate(&c->ev1);
+  for (auto &rr : c->runs) {
+    hipEventCreate(&rr.start);
+    hipEventCreate(&rr.end);
+  }
   for (int d = 0; d < 6; ++d) {
     c->F[d] = dev_alloc(c, (c->ncell + FIELD_DISCARD) * c->esz);
     c->F1[d] = c->leap ? dev_alloc(c, (c->ncell + FIELD_DISCARD) * c->esz) : c->F[d];
     if (!c->F[d] || !c->F1[d]) {
-      vgpu_destroy(c);
+      destroy_ctx(c);
       return VGPU_ENOMEM;
     }
   }
@@ -287,7 +305,7 @@ int vgpu_create(const vgpu_grid_desc *g, vgpu_prec prec, int device, vgpu_ctx **
     c->beta[d] = dev_alloc(c, c->ncell * c->esz);
     c->gamma[d] = dev_alloc(c, c->ncell * c->esz);
     if (!c->alpha[d] || !c->beta[d] || !c->gamma[d]) {
-      vgpu_destroy(c);
+      destroy_ctx(c);
       return VGPU_ENOMEM;
     }
   }
@@ -312,6 +330,7 @@ int vgpu_create(const vgpu_grid_desc *g, vgpu_prec prec, int device, vgpu_ctx **
 int vgpu_set_media(vgpu_ctx *c, const double *alpha[3], const double *beta[3], const double *gamma[3])
 {
   if (!c || c->finalized) return VGPU_ESTATE;
+  const DeviceCalls guard(g_capture_mu);
   for (int d = 0; d < 3; ++d) {
     int r;
     if ((r = upload_field(c, c->alpha[d], alpha[d])) || (r = upload_field(c, c->beta[d], beta[d])) ||
@@ -328,6 +347,7 @@ int vgpu_set_media(vgpu_ctx *c, const double *alpha[3], const double *beta[3], c
 
 int vgpu_set_pml(vgpu_ctx *c, const vgpu_pml_desc *p)
 {
+  const DeviceCalls guard(g_capture_mu);
   if (!c || c->finalized || !p) return VGPU_ESTATE;
   c->pmlmask = p->region_mask & 0x3Fu;
   memcpy(c->fplim, p->fplim, sizeof(c->fplim));
@@ -1161,8 +1181,15 @@ static int build_observers(Ctx *c)
     c->vnodes = (long long)cell.size();
     c->d_vcell = upload_vec(c, cell); c->d_vobs = upload_vec(c, obs);
     c->d_vfirst = upload_vec(c, first); c->d_vlast = upload_vec(c, last);
-    c->d_vstage = dev_alloc(c, (size_t)c->vnodes * 6 * std::max<uint64_t>(c->maxRun, 1) * sizeof(float));
+    c->vring = 2 * std::max<uint64_t>(c->maxRun, 1);
+    const size_t ringBytes = (size_t)c->vnodes * 6 * c->vring * sizeof(float);
+    c->d_vstage = dev_alloc(c, ringBytes);
     if (!c->d_vstage) return set_error(c, VGPU_ENOMEM, "volume observer staging");
+    if (hipHostMalloc((void **)&c->h_vring, ringBytes, hipHostMallocDefault) != hipSuccess || !c->h_vring) {
+      c->h_vring = nullptr;
+      (void)hipGetLastError();
+      return set_error(c, VGPU_ENOMEM, "pinned volume observer ring (%zu bytes)", ringBytes);
+    }
   }
   return VGPU_OK;
 }
@@ -1225,6 +1252,7 @@ int vgpu_finalize(vgpu_ctx *c, uint64_t total_steps, uint64_t max_steps_per_run)
 {
   if (!c || c->finalized) return VGPU_ESTATE;
   if (!c->media_set) return set_error(c, VGPU_ESTATE, "vgpu_set_media was not called");
+  const DeviceCalls guard(g_capture_mu);
   c->totalSteps = total_steps;
   c->maxRun = max_steps_per_run ? max_steps_per_run : total_steps;
   c->d_step = dev_alloc(c, 64);
@@ -1234,7 +1262,9 @@ int vgpu_finalize(vgpu_ctx *c, uint64_t total_steps, uint64_t max_steps_per_run)
       (r = build_observers(c)) || (r = build_leap(c)))
     return r;
   if ((r = c->prec == VGPU_FP64 ? prepare_sweeps<double>(c) : prepare_sweeps<float>(c))) return r;
-  if (hipDeviceSynchronize() != hipSuccess) return set_error(c, VGPU_EDEVICE, "device error during set-up");
+  // set-up copies are synchronous and its kernels ran on the solve's stream
+  if (hipStreamSynchronize(c->stream) != hipSuccess || hipGetLastError() != hipSuccess)
+    return set_error(c, VGPU_EDEVICE, "device error during set-up");
   c->finalized = true;
   return VGPU_OK;
 }
@@ -1243,6 +1273,7 @@ int vgpu_finalize(vgpu_ctx *c, uint64_t total_steps, uint64_t max_steps_per_run)
 // context's stream.  On any failure the context keeps launching directly.
 static void capture_graph(Ctx *c, uint64_t nsteps)
 {
+  std::unique_lock<std::shared_mutex> alone(g_capture_mu);
   c->graphTried = true;
   hipStream_t s = c->stream;
   if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
@@ -1277,32 +1308,48 @@ int vgpu_run(vgpu_ctx *c, uint64_t first, uint64_t nsteps, void *stream)
   hipSetDevice(c->device);
   (void)hipGetLastError();  // launch errors reported below belong to this run only
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  Ctx::RunRec &rr = c->runs[c->runCount & 1];
   hipEventRecord(c->ev0, s);
+  hipEventRecord(rr.start, s);
   if (c->stepHost != first)
     hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, s, (unsigned long long *)c->d_step,
                        (unsigned long long)first);
-  // Runs of the staging length may be replayed from a captured graph once
-  // the solve is long enough to replay it at least twice.  Capture is off
-  // unless VGPU_GRAPHS=1: with captures running while other host threads
-  // allocated, copied and freed device memory (the solve server), results
-  // diverged and the device faulted, so it stays off until that interaction
-  // is understood.
+  // Runs of the staging length are replayed from a graph captured once,
+  // when the solve is long enough to replay it at least twice (a graph of an
+  // even number of steps starts and ends on the same field set, so it
+  // replays wherever that set is current); VGPU_GRAPHS=0 launches every run
+  // directly.  The capture holds g_capture_mu alone, so no other thread
+  // synchronizes the device while it runs.
   const char *ge = getenv("VGPU_GRAPHS");
-  // (a graph of an even number of steps starts and ends on the same field
-  // set, so it replays wherever that set is current)
-  const bool graphable = ge && ge[0] == '1' && s == c->stream && c->evE.empty() && nsteps >= 2 &&
+  const bool graphable = !(ge && ge[0] == '0') && s == c->stream && c->evE.empty() && nsteps >= 2 &&
                          nsteps % 2 == 0 && nsteps == c->maxRun && c->totalSteps >= 2 * nsteps;
   int r = VGPU_OK;
   if (graphable && !c->graphExec && !c->graphTried) capture_graph(c, nsteps);
   if (graphable && c->graphExec && c->graphSteps == nsteps && c->graphPar == c->par) {
     if (hipGraphLaunch(c->graphExec, s) != hipSuccess)
       r = set_error(c, VGPU_EDEVICE, "graph launch failed: %s", hipGetErrorString(hipGetLastError()));
+    ++c->graphRuns;
     c->lastRunTimed = false;
   } else {
     r = c->prec == VGPU_FP64 ? run_steps<double>(c, nsteps, s, true) : run_steps<float>(c, nsteps, s, true);
     c->lastRunTimed = true;
   }
+  // this run's volume samples to its slot of the pinned ring, after its
+  // kernels on the same stream (rows wrap at the end of the device ring)
+  if (c->vnodes && nsteps) {
+    const size_t row = (size_t)c->vnodes * 6;
+    float *dst = c->h_vring + (size_t)(c->runCount & 1) * c->maxRun * row;
+    const uint64_t r0 = first % c->vring, n1 = std::min<uint64_t>(nsteps, c->vring - r0);
+    hipMemcpyAsync(dst, (float *)c->d_vstage + r0 * row, n1 * row * sizeof(float), hipMemcpyDeviceToHost, s);
+    if (n1 < nsteps)
+      hipMemcpyAsync(dst + n1 * row, c->d_vstage, (nsteps - n1) * row * sizeof(float), hipMemcpyDeviceToHost, s);
+  }
+  hipEventRecord(rr.end, s);
   hipEventRecord(c->ev1, s);
+  rr.first = first;
+  rr.n = nsteps;
+  rr.valid = true;
+  ++c->runCount;
   if (c->leap) c->par ^= (int)(nsteps & 1);  // the current fields are in the set the last step wrote
   c->stepHost = first + nsteps;
   c->lastRunFirst = first;
@@ -1347,6 +1394,7 @@ int vgpu_query(vgpu_ctx *c)
 int vgpu_read_point_observer(vgpu_ctx *c, int id, double *td, uint64_t nsteps, double *dft_re, double *dft_im)
 {
   if (!c || !c->finalized || id < 0 || id >= (int)c->pobs.size()) return VGPU_EINVAL;
+  const DeviceCalls guard(g_capture_mu);
   hipStreamSynchronize(c->stream);
   if (td) {
     const uint64_t n = std::min<uint64_t>(nsteps, c->totalSteps);
@@ -1364,20 +1412,51 @@ int vgpu_read_point_observer(vgpu_ctx *c, int id, double *td, uint64_t nsteps, d
 int vgpu_read_volume_observer(vgpu_ctx *c, int id, float *out, uint64_t nsteps, uint64_t *nodes)
 {
   if (!c || !c->finalized || id < 0 || id >= (int)c->vobs.size()) return VGPU_EINVAL;
+  const DeviceCalls guard(g_capture_mu);
   hipStreamSynchronize(c->stream);
   const long long n0 = c->vobsNodeStart[id], n1 = c->vobsNodeStart[id + 1];
   if (nodes) *nodes = (uint64_t)(n1 - n0);
   if (!out) return VGPU_OK;
   const uint64_t ns = std::min<uint64_t>(nsteps, c->lastRunSteps);
   for (uint64_t s = 0; s < ns; ++s)
-    hipMemcpy(out + s * (n1 - n0) * 6, (float *)c->d_vstage + ((size_t)s * c->vnodes + n0) * 6,
+    hipMemcpy(out + s * (n1 - n0) * 6,
+              (float *)c->d_vstage + (((c->lastRunFirst + s) % c->vring) * c->vnodes + n0) * 6,
               (size_t)(n1 - n0) * 6 * sizeof(float), hipMemcpyDeviceToHost);
   return VGPU_OK;
+}
+
+int vgpu_run_result(vgpu_ctx *c, uint64_t first, int wait, double *ms, const float **samples, uint64_t *nsteps)
+{
+  if (!c || !c->finalized) return VGPU_EINVAL;
+  // records are kept for the last two runs only; the later one wins a tie
+  int slot = -1;
+  for (int back = 1; back >= 0 && c->runCount; --back) {
+    const int q = (int)((c->runCount - 1 - (uint64_t)back) & 1);
+    if ((uint64_t)back < c->runCount && c->runs[q].valid && c->runs[q].first == first) slot = q;
+  }
+  if (slot < 0)
+    return set_error(c, VGPU_EINVAL, "no record of a run that began at step %llu", (unsigned long long)first);
+  const Ctx::RunRec &rr = c->runs[slot];
+  hipError_t e = hipEventQuery(rr.end);
+  if (e == hipErrorNotReady) {
+    if (!wait) return 0;
+    const DeviceCalls guard(g_capture_mu);
+    e = hipEventSynchronize(rr.end);
+  }
+  if (e != hipSuccess) return set_error(c, VGPU_EDEVICE, "device error: %s", hipGetErrorString(e));
+  if (ms) {
+    float t = 0;
+    *ms = hipEventElapsedTime(&t, rr.start, rr.end) == hipSuccess ? t : 0.0;
+  }
+  if (samples) *samples = c->vnodes ? c->h_vring + (size_t)slot * c->maxRun * c->vnodes * 6 : nullptr;
+  if (nsteps) *nsteps = rr.n;
+  return 1;
 }
 
 int vgpu_download_field(vgpu_ctx *c, int component, double *host)
 {
   if (!c || component < 0 || component > 5 || !host) return VGPU_EINVAL;
+  const DeviceCalls guard(g_capture_mu);
   hipStreamSynchronize(c->stream);
   std::vector<unsigned char> buf(c->ncell * c->esz);
   if (hipMemcpy(buf.data(), c->fld(c->par, component), buf.size(), hipMemcpyDeviceToHost) != hipSuccess)
@@ -1404,6 +1483,7 @@ static int check_limits_entry(vgpu_ctx *c, const int folim[6][6], vgpu_limits_re
         return set_error(c, VGPU_EINVAL, "field limits outside the grid");
   if (c->NY > 65535 || c->NZ > 65535)  // the tally launches one block row per (j, k)
     return set_error(c, VGPU_EINVAL, "update-once check: more than 65535 cells along y or z");
+  const DeviceCalls guard(g_capture_mu);
   hipSetDevice(c->device);
   (void)hipGetLastError();
   return c->prec == VGPU_FP64 ? check_limits<double>(c, folim, out, counts, scratch, scratchBytes)
@@ -1454,6 +1534,7 @@ int vgpu_get_stats(const vgpu_ctx *c, vgpu_stats *out)
   out->leap = c->leap ? 1 : 0;
   out->fix_cells = (uint64_t)c->fixCells;
   out->fix_chunks = (uint64_t)c->nFix;
+  out->graph_runs = c->graphRuns;
   return VGPU_OK;
 }
 
@@ -1471,6 +1552,14 @@ const char *vgpu_last_error(const vgpu_ctx *c) { return c ? c->err.c_str() : "nu
 void vgpu_destroy(vgpu_ctx *c)
 {
   if (!c) return;
+  const DeviceCalls guard(g_capture_mu);
+  destroy_ctx(c);
+}
+
+}  // extern "C"
+
+static void destroy_ctx(vgpu_ctx *c)
+{
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
   for (void *p : c->allocs) hipFree(p);
@@ -1480,8 +1569,11 @@ void vgpu_destroy(vgpu_ctx *c)
   for (auto e : c->evH) hipEventDestroy(e);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
+  for (auto &rr : c->runs) {
+    if (rr.start) hipEventDestroy(rr.start);
+    if (rr.end) hipEventDestroy(rr.end);
+  }
+  if (c->h_vring) hipHostFree(c->h_vring);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
-
-}  // extern "C"

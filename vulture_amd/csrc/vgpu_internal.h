@@ -381,10 +381,11 @@ template <typename T> struct VolObsArgs {
   const int *nodeobs;
   long long nnodes;
   const unsigned long long *first, *last;
-  float *stage;              // [run step][node][6]
+  float *stage;              // [row][node][6], row = step modulo ring
   const T *F[6];
-  const unsigned long long *step;  // n = *step + soff; staging slot soff
+  const unsigned long long *step;  // n = *step + soff
   int soff;
+  unsigned long long ring;   // staging rows: two runs' worth
 };
 
 template <typename T> struct GhostArgs {
@@ -573,12 +574,23 @@ struct Ctx {
   hipGraphExec_t graphExec = nullptr;
   uint64_t graphSteps = 0;
   int graphPar = 0;                          // field set current when the graph was captured
+  uint64_t graphRuns = 0;                    // runs replayed from it
   bool graphTried = false, lastRunTimed = false;
   void *d_ocell = nullptr, *d_ofirst = nullptr, *d_olast = nullptr, *d_onf = nullptr, *d_ooff = nullptr;
   void *d_oomega = nullptr, *d_otd = nullptr, *d_odre = nullptr, *d_odim = nullptr;
   long long vnodes = 0;
   void *d_vcell = nullptr, *d_vobs = nullptr, *d_vfirst = nullptr, *d_vlast = nullptr, *d_vstage = nullptr;
   std::vector<long long> vobsNodeStart;
+  // Volume samples stream out without a sync per run: the device staging
+  // holds 2*maxRun rows (row = step modulo that), each run's rows are copied
+  // asynchronously into its slot of a pinned host ring after its kernels,
+  // and a record of each of the last two runs (first step, length, start and
+  // end events) lets the host finish one run while the next one runs.
+  uint64_t vring = 0;
+  float *h_vring = nullptr;  // [2][maxRun][vnodes][6], pinned
+  struct RunRec { uint64_t first = 0, n = 0; bool valid = false; hipEvent_t start = nullptr, end = nullptr; };
+  RunRec runs[2];
+  uint64_t runCount = 0;
 
   long long lin(int i, int j, int k) const { return ((long long)k * NY + j) * PX + i; }
 };

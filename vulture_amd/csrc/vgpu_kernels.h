@@ -1381,7 +1381,7 @@ template <typename T> __global__ void k_observe_volume(VolObsArgs<T> a)
   const unsigned long long n = *a.step + (unsigned long long)a.soff;
   if (n < a.first[o] || n > a.last[o]) return;
   const long long id = a.nodecell[nd];
-  float *out = a.stage + ((long long)a.soff * a.nnodes + nd) * 6;
+  float *out = a.stage + ((long long)(n % a.ring) * a.nnodes + nd) * 6;
   for (int c = 0; c < 6; ++c) out[c] = (float)a.F[c][id];
 }
 

@@ -710,6 +710,7 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
       v.nodecell = P<long long>(c->d_vcell); v.nodeobs = P<int>(c->d_vobs); v.nnodes = c->vnodes;
       v.first = P<unsigned long long>(c->d_vfirst); v.last = P<unsigned long long>(c->d_vlast);
       v.stage = P<float>(c->d_vstage);
+      v.ring = c->vring;
       for (int d = 0; d < 6; ++d) v.F[d] = P<T>(c->fld(q, d));
     }
   }

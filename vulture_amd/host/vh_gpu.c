@@ -199,11 +199,15 @@ bad:
 struct vh_dsolve {
   vh_solve *s;
   vgpu_ctx *c;
-  unsigned long NT, chunk, next, inflight;
+  unsigned long NT, chunk;
+  unsigned long next;          /* steps completed */
+  unsigned long enq;           /* steps enqueued */
+  unsigned long qfirst[2], qn[2];  /* runs in flight, oldest first */
+  int qlen;
+  int nbin;                    /* binary observers: rows streamed as runs complete */
   int *pid, *vid;
   uint64_t *vnodes;
   size_t volNodes;
-  float *vol;
   double loop_ms;
 };
 
@@ -214,7 +218,6 @@ void vh_dsolve_close(vh_dsolve *d)
   free(d->pid);
   free(d->vid);
   free(d->vnodes);
-  free(d->vol);
   free(d);
 }
 
@@ -247,67 +250,66 @@ vh_dsolve *vh_dsolve_open(vh_solve *s, int prec, int device, unsigned long chunk
       vgpu_read_volume_observer(d->c, d->vid[o], NULL, 0, &d->vnodes[d->vid[o]]);
       d->volNodes += d->vnodes[d->vid[o]];
     }
-  if (d->volNodes) {
-    d->vol = malloc(d->volNodes * 6 * sizeof(float) * (NT ? NT : 1));
-    if (!d->vol) {
-      snprintf(s->err, sizeof(s->err), "volume observer buffer too large");
-      vh_dsolve_close(d);
-      return NULL;
-    }
-  }
+  for (int o = 0; o < s->numObs; o++) d->nbin += s->obs[o].format == OF_BINARY;
+  s->binaryStreamed = d->nbin > 0;
   return d;
 }
 
 vgpu_ctx *vh_dsolve_context(vh_dsolve *d) { return d->c; }
-unsigned long vh_dsolve_remaining(const vh_dsolve *d) { return d->NT - d->next; }
+unsigned long vh_dsolve_remaining(const vh_dsolve *d) { return d->NT - d->enq; }
+int vh_dsolve_done(const vh_dsolve *d) { return d->next >= d->NT; }
 
-/* Enqueue the next chunk of steps on the context's stream (asynchronous).
- * Returns the steps enqueued (0 when the loop is complete), <0 on error. */
+/* Enqueue the next chunk of steps on the context's stream (asynchronous),
+ * keeping at most two chunks in flight, so the device runs one while the
+ * host completes the other.  Returns the steps enqueued (0 when two are in
+ * flight or every step is enqueued), <0 on error. */
 long vh_dsolve_enqueue(vh_dsolve *d)
 {
-  if (d->inflight || d->next >= d->NT) return 0;
-  unsigned long n = d->NT - d->next < d->chunk ? d->NT - d->next : d->chunk;
-  int rc = vgpu_run(d->c, d->next, n, NULL);
+  if (d->qlen >= 2 || d->enq >= d->NT) return 0;
+  unsigned long n = d->NT - d->enq < d->chunk ? d->NT - d->enq : d->chunk;
+  int rc = vgpu_run(d->c, d->enq, n, NULL);
   if (rc) return fail(d->s, d->c, "vgpu_run", rc);
-  d->inflight = n;
+  d->qfirst[d->qlen] = d->enq;
+  d->qn[d->qlen] = n;
+  d->qlen++;
+  d->enq += n;
   return (long)n;
 }
 
-/* 1 when the chunk in flight has finished (or none is), 0 while it runs. */
+/* 1 when the oldest chunk in flight has finished (or none is in flight), 0
+ * while it runs, <0 on a device error. */
 int vh_dsolve_ready(vh_dsolve *d)
 {
-  if (!d->inflight) return 1;
-  return vgpu_query(d->c);
+  if (!d->qlen) return 1;
+  int rc = vgpu_run_result(d->c, d->qfirst[0], 0, NULL, NULL, NULL);
+  return rc < 0 ? fail(d->s, d->c, "vgpu_run", rc) : rc;
 }
 
-/* Complete the chunk in flight: wait for it, add its device time and drain
- * the volume-observer staging.  Returns 0 or <0 on error. */
+/* Complete the oldest chunk in flight without waiting for the one behind it:
+ * add its device time and write its excite.dat lines and impulse.dat rows
+ * straight from the pinned copy of its samples (observer.c:1164-1198 writes
+ * them inside the loop; the files come out the same).  Returns 0 or <0. */
 int vh_dsolve_complete(vh_dsolve *d)
 {
-  if (!d->inflight) return 0;
+  if (!d->qlen) return 0;
   vh_solve *s = d->s;
-  int rc = vgpu_synchronize(d->c);
-  if (rc) return fail(s, d->c, "vgpu_run", rc);
-  double ms;
-  vgpu_last_run_timing(d->c, &ms, NULL, NULL);
+  double ms = 0;
+  const float *smp = NULL;
+  uint64_t n = 0;
+  int rc = vgpu_run_result(d->c, d->qfirst[0], 1, &ms, &smp, &n);
+  if (rc < 0) return fail(s, d->c, "vgpu_run", rc);
   d->loop_ms += ms;
-  unsigned long first = d->next, n = d->inflight;
-  if (d->volNodes) {
-    size_t off = 0;
-    for (int o = 0; o < s->numObs; o++)
-      if (d->vid[o] >= 0) {
-        const uint64_t vn = d->vnodes[d->vid[o]];
-        float *tmp = malloc(vn * 6 * sizeof(float) * n);
-        if (!tmp) { snprintf(s->err, sizeof(s->err), "volume observer staging"); return -1; }
-        vgpu_read_volume_observer(d->c, d->vid[o], tmp, n, NULL);
-        for (unsigned long q = 0; q < n; q++)
-          memcpy(d->vol + ((first + q) * d->volNodes + off) * 6, tmp + q * vn * 6, vn * 6 * sizeof(float));
-        free(tmp);
-        off += vn;
-      }
+  if (d->nbin) {
+    if (!smp && d->volNodes) { snprintf(s->err, sizeof(s->err), "volume observer samples missing"); return -1; }
+    for (uint64_t q = 0; q < n; q++) {
+      const unsigned long step = d->qfirst[0] + q;
+      vh_observe_binary_step(s, step, step * s->dt, smp + q * d->volNodes * 6);
+    }
   }
   d->next += n;
-  d->inflight = 0;
+  d->qfirst[0] = d->qfirst[1];
+  d->qn[0] = d->qn[1];
+  d->qlen--;
   return 0;
 }
 
@@ -353,7 +355,7 @@ void vh_dsolve_write(vh_dsolve *d, double **td)
     for (int o = 0; o < s->numObs; o++)
       if (d->pid[o] >= 0 && ot)
         for (int q = 0; q < 6; q++) pv[6 * o + q] = td[o][(n - s->startTimeStep) * 6 + q];
-    vh_observe_step(s, n, t, pv, d->vol ? d->vol + (size_t)n * d->volNodes * 6 : NULL);
+    vh_observe_step(s, n, t, pv, NULL);
   }
   for (int o = 0; o < s->numObs; o++) free(td[o]);
   free(td);
@@ -368,8 +370,10 @@ int vh_gpu_run(vh_solve *s, int prec, int device, unsigned long chunk, double *l
 {
   vh_dsolve *d = vh_dsolve_open(s, prec, device, chunk);
   if (!d) return -1;
-  while (d->next < d->NT) {
-    if (vh_dsolve_enqueue(d) < 0 || vh_dsolve_complete(d)) { vh_dsolve_close(d); return -1; }
+  while (!vh_dsolve_done(d)) {
+    long e;
+    while ((e = vh_dsolve_enqueue(d)) > 0) {}
+    if (e < 0 || vh_dsolve_complete(d)) { vh_dsolve_close(d); return -1; }
   }
   if (loop_ms) *loop_ms = d->loop_ms;
   double **td = NULL;

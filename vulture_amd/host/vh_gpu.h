@@ -23,6 +23,7 @@ long vh_dsolve_enqueue(vh_dsolve *d);
 int vh_dsolve_ready(vh_dsolve *d);
 int vh_dsolve_complete(vh_dsolve *d);
 unsigned long vh_dsolve_remaining(const vh_dsolve *d);
+int vh_dsolve_done(const vh_dsolve *d);
 int vh_dsolve_collect(vh_dsolve *d, double ***td);
 void vh_dsolve_write(vh_dsolve *d, double **td);
 double vh_dsolve_loop_ms(const vh_dsolve *d);

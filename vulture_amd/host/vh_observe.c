@@ -239,20 +239,62 @@ static void accumulate_dft(vh_solve *s, vh_observer *ob, real t, const real *v)
     }
 }
 
+/* Step n's excite.dat line and, inside the output window, its impulse.dat
+ * rows (observer.c:1164-1198): every binary observer's nodes in k, j, i
+ * order, each as three int indices and six float field values, from binVals
+ * (six floats per node in file order) or, when NULL, the host arrays. */
+static void write_binary_step(vh_solve *s, unsigned long n, real t, const float *binVals)
+{
+  int nbin = 0;
+  for (int o = 0; o < s->numObs; o++) nbin += s->obs[o].format == OF_BINARY;
+  if (!nbin) return;
+  if (s->exciteFile) fprintf(s->exciteFile, "%16.8e\n", vh_waveform_value(s, t, 0, 0.0));
+  if (n < s->startTimeStep || n > s->stopTimeStep) return;
+  const real *F[6] = {s->Ex, s->Ey, s->Ez, s->Hx, s->Hy, s->Hz};
+  size_t bpos = 0;
+  for (int o = 0; o < s->numObs; o++) {
+    vh_observer *ob = &s->obs[o];
+    if (ob->quantity == OQ_WF || ob->format != OF_BINARY) continue;
+    for (int k = ob->gbbox[ZLO]; k <= ob->gbbox[ZHI]; k += ob->step[2])
+      for (int j = ob->gbbox[YLO]; j <= ob->gbbox[YHI]; j += ob->step[1])
+        for (int i = ob->gbbox[XLO]; i <= ob->gbbox[XHI]; i += ob->step[0]) {
+          float out[6];
+          if (binVals) {
+            for (int c = 0; c < 6; c++) out[c] = binVals[6 * bpos + c];
+          } else {
+            size_t id = VH_IDX(s, i, j, k);
+            for (int c = 0; c < 6; c++) out[c] = F[c][id];
+          }
+          bpos++;
+          int io[3] = {i - s->gibox[XLO], j - s->gibox[YLO], k - s->gibox[ZLO]};
+          if (s->impulseFile) {
+            fwrite(io, sizeof(int), 3, s->impulseFile);
+            fwrite(out, sizeof(float), 6, s->impulseFile);
+          }
+        }
+  }
+}
+
+/* The binary part of a step on its own, for callers that stream volume
+ * samples as runs complete (vh_observe_step then skips it). */
+void vh_observe_binary_step(vh_solve *s, unsigned long n, real t, const float *binVals)
+{
+  write_binary_step(s, n, t, binVals);
+}
+
 /*
  * One call per time step, before the step's updates (simulation.c:167).
  * pointVals: [numObs][6] node values for EH point observers, or NULL to read
  * the host field arrays.  binVals: the binary observers' nodes, six floats
  * each, in file order, or NULL to read the host arrays.
+ * The binary part is left out when the caller streams it
+ * (s->binaryStreamed, vh_observe_binary_step).
  */
 void vh_observe_step(vh_solve *s, unsigned long n, real t, const double *pointVals, const float *binVals)
 {
   bool otValid = n >= s->startTimeStep && n <= s->stopTimeStep;
-  int nbin = 0;
-  for (int o = 0; o < s->numObs; o++) nbin += s->obs[o].format == OF_BINARY;
-  if (nbin && s->exciteFile) fprintf(s->exciteFile, "%16.8e\n", vh_waveform_value(s, t, 0, 0.0));
+  if (!s->binaryStreamed) write_binary_step(s, n, t, binVals);
   const real *F[6] = {s->Ex, s->Ey, s->Ez, s->Hx, s->Hy, s->Hz};
-  size_t bpos = 0;
   for (int o = 0; o < s->numObs; o++) {
     vh_observer *ob = &s->obs[o];
     real v[6];
@@ -265,27 +307,7 @@ void vh_observe_step(vh_solve *s, unsigned long n, real t, const double *pointVa
       }
       continue;
     }
-    if (!otValid) continue;
-    if (ob->format == OF_BINARY) {
-      for (int k = ob->gbbox[ZLO]; k <= ob->gbbox[ZHI]; k += ob->step[2])
-        for (int j = ob->gbbox[YLO]; j <= ob->gbbox[YHI]; j += ob->step[1])
-          for (int i = ob->gbbox[XLO]; i <= ob->gbbox[XHI]; i += ob->step[0]) {
-            float out[6];
-            if (binVals) {
-              for (int c = 0; c < 6; c++) out[c] = binVals[6 * bpos + c];
-            } else {
-              size_t id = VH_IDX(s, i, j, k);
-              for (int c = 0; c < 6; c++) out[c] = F[c][id];
-            }
-            bpos++;
-            int io[3] = {i - s->gibox[XLO], j - s->gibox[YLO], k - s->gibox[ZLO]};
-            if (s->impulseFile) {
-              fwrite(io, sizeof(int), 3, s->impulseFile);
-              fwrite(out, sizeof(float), 6, s->impulseFile);
-            }
-          }
-      continue;
-    }
+    if (!otValid || ob->format == OF_BINARY) continue;
     if (pointVals) {
       for (int c = 0; c < 6; c++) v[c] = pointVals[6 * o + c];
     } else {

@@ -221,11 +221,12 @@ static void *scheduler(void *arg)
       const int ready = vh_dsolve_ready(r->d);
       if (ready < 0) rc = -1;
       else if (ready == 1) {
+        /* finish the oldest chunk, then keep two in flight */
         rc = vh_dsolve_complete(r->d);
-        if (!rc && vh_dsolve_remaining(r->d) > 0) {
-          if (vh_dsolve_enqueue(r->d) < 0) rc = -1;
-          busy = 1;
-        } else if (!rc) {
+        long e = 0;
+        while (!rc && (e = vh_dsolve_enqueue(r->d)) > 0) busy = 1;
+        if (e < 0) rc = -1;
+        if (!rc && vh_dsolve_done(r->d)) {
           r->info.gpu_ms = vh_dsolve_loop_ms(r->d);
           double **td = NULL;
           vh_dsolve_collect(r->d, &td);
