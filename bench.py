@@ -154,25 +154,76 @@ def sibc_box(work, steps, warmup=5):
             "setup_s": round(setup_s, 2)}
 
 
-def cpu_baseline(mesh, steps, cells):
-    """The reference itself (its float build from its own sources, -O3 with
-    its OpenMP option, oracle/_ref/vulture_ref_omp) on `steps` steps of the
-    same mesh: wall time of the full run minus wall time of its set-up-only
-    run (-p).  The CPU restatement of the loop is timed beside it (cpu_port).
-    Both on up to 16 host threads."""
+def _fixture_dir(npz, dst, wf_src):
+    """Observer files under dst from an FP64-oracle fixture (arrays written by
+    tests/golden/make_fp64_fixtures.py), beside wf_src's waveform spectrum."""
+    import glob
+    import shutil
+    import numpy as np
+    fx = np.load(npz)
+    os.makedirs(dst, exist_ok=True)
+    for key in fx.files:
+        if key.startswith(("td/", "fd/")):
+            with open(os.path.join(dst, key[3:]), "w") as f:
+                for row in fx[key]:
+                    head = f"{int(row[0])} " if key.startswith("td/") else f"{float(row[0])!r} "
+                    f.write(head + " ".join(repr(float(x)) for x in row[1:]) + "\n")
+    for w in glob.glob(os.path.join(wf_src, "wf_*_fd.asc")):
+        shutil.copy(w, dst)
+    return str(fx["mesh_sha256"])
+
+
+def accuracy(work):
+    """The Debye-sphere volume for its full 3000 steps in both storage forms:
+    the FP32 observers against the FP64 device run (time-series relative L2
+    over all samples and components; max dB deviation of the spectra over the
+    band where the waveform spectrum lies within 60 dB of its peak, bins
+    below 1e-3 of the largest reference magnitude left out (fd_db) and kept
+    (fd_db_all)), and both forms against the FP64 oracle's series when its
+    fixture (tests/golden/fp64/debye_full.npz) is present."""
+    import hashlib
+    from vulture_amd import meshgen, run_mesh
+    from vulture_amd.obsfiles import compare_outputs
+    d = os.path.join(work, "acc")
+    mesh = meshgen.write("debye", d)
+    outs = {}
+    for prec in ("fp64", "fp32"):
+        outs[prec] = os.path.join(d, prec)
+        run_mesh(mesh, outs[prec], precision=prec)
+    keys = ("td", "fd_raw", "fd_db", "fd_db_all")
+    r = compare_outputs(outs["fp32"], outs["fp64"])
+    res = {"steps": 3000, "fp32_vs_fp64": {k: float(f"{r[k]:.3e}") for k in keys}}
+    fx = os.path.join(HERE, "tests", "golden", "fp64", "debye_full.npz")
+    if os.path.exists(fx):
+        sha = _fixture_dir(fx, os.path.join(d, "oracle"), outs["fp64"])
+        if sha == hashlib.sha256(open(mesh, "rb").read()).hexdigest():
+            for prec in ("fp64", "fp32"):
+                q = compare_outputs(outs[prec], os.path.join(d, "oracle"))
+                res[f"{prec}_vs_fp64_oracle"] = {k: float(f"{q[k]:.3e}") for k in keys}
+            res["oracle_observers_compared"] = q["files"]
+    return res
+
+
+def host_cpu():
+    """The host CPU model and the cores this process may use."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, len(os.sched_getaffinity(0))
+
+
+def time_reference(exe, mesh, steps, cells, threads):
+    """Loop seconds of a reference build on `steps` steps of `mesh`: wall time
+    of the full run minus that of its set-up-only run (-p), on `threads`
+    OpenMP threads.  Returns (Mcell-updates/s, full wall, set-up wall)."""
     import re
     import subprocess
-    import oracle
-    threads = min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
-    sec, pcells, n = oracle.time_mesh(mesh, steps=steps, warmup=2, threads=threads, precision="fp32",
-                                      outdir=os.path.join(os.path.dirname(mesh), "cpu"))
-    port = {"value": round(pcells * n / sec / 1e6, 2), "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
-            "sample": f"{n} timed steps (after 2 untimed) of the same 272^3 Debye mesh, FP32 CPU restatement "
-                      f"(-O3, OpenMP, {threads} threads): {sec:.2f} s"}
-    ref = os.path.join(HERE, "oracle", "_ref", "vulture_ref_omp")
-    if not os.path.exists(ref):
-        return port, None
-    d = os.path.join(os.path.dirname(mesh), "ref")
+    d = os.path.join(os.path.dirname(mesh), "ref_" + os.path.basename(exe))
     os.makedirs(d, exist_ok=True)
     text = re.sub(r"^NT \d+$", f"NT {steps}", open(mesh).read(), flags=re.M)
     rmesh = os.path.join(d, "ref.mesh")
@@ -181,17 +232,48 @@ def cpu_baseline(mesh, steps, cells):
     walls = []
     for opt in (["-p"], []):
         t0 = time.monotonic()
-        r = subprocess.run([ref, "-n", str(threads)] + opt + [rmesh], cwd=d, capture_output=True, text=True,
+        r = subprocess.run([exe, "-n", str(threads)] + opt + [rmesh], cwd=d, capture_output=True, text=True,
                            timeout=600)
         walls.append(time.monotonic() - t0)
         if r.returncode != 0:
-            return port, None
-    loop = walls[1] - walls[0]
-    return {"value": round(cells * steps / loop / 1e6, 2), "unit": "Mcell-updates/s", "cores": threads,
-            "kind": "reference",
-            "sample": f"the reference's own float build (-O3, its OpenMP option, {threads} threads) on {steps} "
-                      f"steps of the same 272^3 Debye mesh: full run {walls[1]:.2f} s minus set-up-only run "
-                      f"(-p) {walls[0]:.2f} s = {loop:.2f} s"}, port
+            return None
+    return cells * steps / (walls[1] - walls[0]) / 1e6, walls[1], walls[0]
+
+
+def cpu_baseline(mesh, steps, cells):
+    """The reference itself on `steps` steps of the same mesh, on every core
+    this process may use (OMP_NUM_THREADS when set: the job's share of the
+    host): its float build from its own sources with its own Release flags
+    (CMakeLists.txt:64,68: -Ofast -flto -funroll-loops -march=core2) and its
+    OpenMP option (oracle/_ref/vulture_ref_rel).  Beside it: the same sources
+    at -O3 -march=x86-64-v3 (vulture_ref_omp), giving the calibration ratio
+    between the two builds, and the CPU restatement of the loop (cpu_port)."""
+    import oracle
+    model, avail = host_cpu()
+    threads = int(os.environ.get("OMP_NUM_THREADS") or avail)
+    sec, pcells, n = oracle.time_mesh(mesh, steps=steps, warmup=2, threads=threads, precision="fp32",
+                                      outdir=os.path.join(os.path.dirname(mesh), "cpu"))
+    port = {"value": round(pcells * n / sec / 1e6, 2), "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{n} timed steps (after 2 untimed) of the same 272^3 Debye mesh, FP32 CPU restatement "
+                      f"(-O3, OpenMP, {threads} threads): {sec:.2f} s"}
+    rel = os.path.join(HERE, "oracle", "_ref", "vulture_ref_rel")
+    o3 = os.path.join(HERE, "oracle", "_ref", "vulture_ref_omp")
+    if not os.path.exists(rel):
+        return port, None
+    r = time_reference(rel, mesh, steps, cells, threads)
+    if r is None:
+        return port, None
+    base = {"value": round(r[0], 2), "unit": "Mcell-updates/s", "cores": threads, "kind": "reference",
+            "cpu": model, "host_cores": avail,
+            "sample": f"the reference's own float build with its Release flags and OpenMP option, {threads} "
+                      f"threads, on {steps} steps of the same 272^3 Debye mesh: full run {r[1]:.2f} s minus "
+                      f"set-up-only run (-p) {r[2]:.2f} s = {r[1] - r[2]:.2f} s"}
+    if os.path.exists(o3):
+        q = time_reference(o3, mesh, steps, cells, threads)
+        if q is not None:
+            base["o3_build"] = round(q[0], 2)
+            base["release_over_o3"] = round(r[0] / q[0], 3)
+    return base, port
 
 
 def main():
@@ -202,13 +284,14 @@ def main():
     ap.add_argument("--n", type=int, default=256, help="mesh cells per axis")
     ap.add_argument("--precision", choices=["both", "fp32", "fp64"], default="both")
     ap.add_argument("--cpu-steps", type=int, default=30, help="timed CPU baseline steps (0 skips it)")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the full-length FP32 vs FP64 accuracy run")
     ap.add_argument("--spheres", type=int, default=200, help="Debye spheres (workload variants for profiling)")
     ap.add_argument("--boundary", choices=["PML", "PEC", "MUR"], default="PML")
     ap.add_argument("--no-planewave", action="store_true", help="hard point source instead of the TF/SF plane wave")
-    ap.add_argument("--serve", type=int, default=24, help="requests in the serving sweep (0 skips it)")
+    ap.add_argument("--serve", type=int, default=32, help="requests in the serving sweep (0 skips it)")
     ap.add_argument("--serve-active", type=int, default=8, help="solves in flight at once when serving")
-    ap.add_argument("--serve-nmax", type=int, default=128, help="largest cells per axis of a serving request")
-    ap.add_argument("--serve-steps", type=int, nargs=2, default=[200, 800], help="step range of serving requests")
+    ap.add_argument("--serve-nmax", type=int, default=320, help="largest cells per axis of a serving request")
+    ap.add_argument("--serve-steps", type=int, nargs=2, default=[2000, 8000], help="step range of serving requests")
     ap.add_argument("--sibc-steps", type=int, default=30,
                     help="timed steps of the SIBC + Mur box at 384x384x256 in FP32 (0 skips it)")
     ap.add_argument("--sibc-only", action="store_true",
@@ -223,8 +306,13 @@ def main():
     if args.sibc_only:
         print(json.dumps(sibc_box(work, max(args.sibc_steps, 1))), flush=True)
         return 0
-    mesh = meshgen.write("debye", work, n=args.n, steps=args.warmup + args.steps, spheres=args.spheres,
-                         boundary=args.boundary, planewave=not args.no_planewave)
+    kw = dict(n=args.n, steps=args.warmup + args.steps, spheres=args.spheres, boundary=args.boundary,
+              planewave=not args.no_planewave)
+    # the device reads the spheres as one voxel object; the reference's CPU
+    # build reads the same spheres as one MB card per run (same cells, media
+    # and face masks: identical observer files)
+    mesh = meshgen.write("debye", os.path.join(work, "vox"), voxel=True, **kw)
+    mesh_cards = meshgen.write("debye", os.path.join(work, "cards"), **kw)
     res = {}
     for prec in (["fp64", "fp32"] if args.precision == "both" else [args.precision]):
         res[prec] = measure(mesh, prec, args.steps, args.warmup)
@@ -240,7 +328,7 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": head["prec"],
         "data": "synthetic",
         "config": {"workload": f"debye spheres {args.n}^3 cubic 2 mm, 8-cell UPML, 200 single-pole Debye "
-                               f"spheres, aux-grid TF/SF plane wave, 8 DFT observers (seed 3)",
+                               f"spheres (one voxel object), aux-grid TF/SF plane wave, 8 DFT observers (seed 3)",
                    "cells": st["cells"], "mesh_cells": st["mesh_cells"], "debye_sites": st["debye_sites"],
                    "device_bytes": st["bytes"], "coefficients": "16-bit classes" if st["indexed_media"] else
                    "per-cell arrays", "debye_fused_sites": st["debye_fused_sites"],
@@ -253,9 +341,14 @@ def main():
         out["fp64"] = {"value": round(f64["value"], 1), "ms_per_step": round(f64["ms_per_step"], 4),
                        "roofline": f64["roofline"], "step_roofline": f64["step_roofline"]}
         out["fp32_over_fp64"] = round(res["fp32"]["value"] / f64["value"], 3)
+    if not args.no_accuracy and args.n == 256 and args.spheres == 200 and args.boundary == "PML" and \
+            not args.no_planewave:
+        out["fp32_error"] = accuracy(work)
+        a = out["fp32_error"]["fp32_vs_fp64"]
+        print(f"# fp32 vs fp64 over 3000 steps: td {a['td']:.2e}, {a['fd_db']:.2e} dB", file=sys.stderr, flush=True)
     out["cpu_baseline"] = None
     if args.cpu_steps > 0:
-        base, port = cpu_baseline(mesh, args.cpu_steps, st["cells"])
+        base, port = cpu_baseline(mesh_cards, args.cpu_steps, st["cells"])
         out["cpu_baseline"] = base
         if port is not None:
             out["cpu_port"] = port
