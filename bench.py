@@ -235,6 +235,8 @@ def time_reference(exe, mesh, steps, cells, threads):
         r = subprocess.run([exe, "-n", str(threads)] + opt + [rmesh], cwd=d, capture_output=True, text=True,
                            timeout=600)
         walls.append(time.monotonic() - t0)
+        print(f"# cpu baseline: {os.path.basename(exe)} {' '.join(opt) or 'full run'} {walls[-1]:.1f} s",
+              file=sys.stderr, flush=True)
         if r.returncode != 0:
             return None
     return cells * steps / (walls[1] - walls[0]) / 1e6, walls[1], walls[0]
@@ -283,7 +285,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=256, help="mesh cells per axis")
     ap.add_argument("--precision", choices=["both", "fp32", "fp64"], default="both")
-    ap.add_argument("--cpu-steps", type=int, default=30, help="timed CPU baseline steps (0 skips it)")
+    ap.add_argument("--cpu-steps", type=int, default=100, help="timed CPU baseline steps (0 skips it)")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the full-length FP32 vs FP64 accuracy run")
     ap.add_argument("--spheres", type=int, default=200, help="Debye spheres (workload variants for profiling)")
     ap.add_argument("--boundary", choices=["PML", "PEC", "MUR"], default="PML")
@@ -360,7 +362,8 @@ def main():
         from vulture_amd import serve
         rep = serve.sweep(os.path.join(work, "serve"), count=args.serve, load=0.7, precision="fp32",
                           max_active=args.serve_active, chunk=200, nmin=32, nmax=args.serve_nmax,
-                          steps_min=args.serve_steps[0], steps_max=args.serve_steps[1], nfreq=64)
+                          steps_min=args.serve_steps[0], steps_max=args.serve_steps[1], nfreq=64,
+                          log=lambda m: print(f"# serving: {m}", file=sys.stderr, flush=True))
         rep["workload"] = (f"{args.serve} seeded requests: cells per axis log-uniform [32, {args.serve_nmax}], "
                            f"graded lines, 4-24 SIMPLE/DEBYE blocks, 8-cell UPML, TF/SF at random angles, "
                            f"{args.serve_steps[0]}-{args.serve_steps[1]} steps, 64-frequency DFT observers; "

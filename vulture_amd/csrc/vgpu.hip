@@ -231,6 +231,16 @@ extern "C" {
 
 static void destroy_ctx(vgpu_ctx *c);
 
+// The last run may have gone to a caller's stream (vgpu_run's stream
+// argument): wait for its end event as well as for the context's stream
+// before reading results or freeing memory.  An event never recorded
+// counts as complete.
+static void wait_last_run(Ctx *c)
+{
+  hipEventSynchronize(c->ev1);
+  hipStreamSynchronize(c->stream);
+}
+
 int vgpu_create(const vgpu_grid_desc *g, vgpu_prec prec, int device, vgpu_ctx **out)
 {
   if (!g || !out) return VGPU_EINVAL;
@@ -1395,7 +1405,7 @@ int vgpu_read_point_observer(vgpu_ctx *c, int id, double *td, uint64_t nsteps, d
 {
   if (!c || !c->finalized || id < 0 || id >= (int)c->pobs.size()) return VGPU_EINVAL;
   const DeviceCalls guard(g_capture_mu);
-  hipStreamSynchronize(c->stream);
+  wait_last_run(c);
   if (td) {
     const uint64_t n = std::min<uint64_t>(nsteps, c->totalSteps);
     hipMemcpy(td, (char *)c->d_otd + (size_t)id * c->totalSteps * 6 * sizeof(double), n * 6 * sizeof(double),
@@ -1413,7 +1423,7 @@ int vgpu_read_volume_observer(vgpu_ctx *c, int id, float *out, uint64_t nsteps, 
 {
   if (!c || !c->finalized || id < 0 || id >= (int)c->vobs.size()) return VGPU_EINVAL;
   const DeviceCalls guard(g_capture_mu);
-  hipStreamSynchronize(c->stream);
+  wait_last_run(c);
   const long long n0 = c->vobsNodeStart[id], n1 = c->vobsNodeStart[id + 1];
   if (nodes) *nodes = (uint64_t)(n1 - n0);
   if (!out) return VGPU_OK;
@@ -1457,7 +1467,7 @@ int vgpu_download_field(vgpu_ctx *c, int component, double *host)
 {
   if (!c || component < 0 || component > 5 || !host) return VGPU_EINVAL;
   const DeviceCalls guard(g_capture_mu);
-  hipStreamSynchronize(c->stream);
+  wait_last_run(c);
   std::vector<unsigned char> buf(c->ncell * c->esz);
   if (hipMemcpy(buf.data(), c->fld(c->par, component), buf.size(), hipMemcpyDeviceToHost) != hipSuccess)
     return set_error(c, VGPU_EDEVICE, "download failed");
@@ -1561,7 +1571,7 @@ void vgpu_destroy(vgpu_ctx *c)
 static void destroy_ctx(vgpu_ctx *c)
 {
   hipSetDevice(c->device);
-  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->stream) wait_last_run(c);
   for (void *p : c->allocs) hipFree(p);
   if (c->graphExec) hipGraphExecDestroy(c->graphExec);
   if (c->graph) hipGraphDestroy(c->graph);

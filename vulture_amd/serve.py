@@ -127,7 +127,7 @@ def sweep(workdir, count=40, load=0.7, seed=5, precision="fp32", max_active=8, c
             if info["status"] != 1:
                 raise VgpuError(f"request {q} failed alone: {info['err']}")
             iso.append(info)
-            if log and q % 10 == 0:
+            if log and q % 8 == 0:
                 log(f"alone {q}/{count}")
     service = np.array([i["t_done"] - i["t_submit"] for i in iso])
     work = np.array([i["cells"] * i["steps"] for i in iso], dtype=np.float64)
@@ -148,11 +148,17 @@ def sweep(workdir, count=40, load=0.7, seed=5, precision="fp32", max_active=8, c
             if info["status"] != 1:
                 raise VgpuError(f"request {q} failed in the mix: {info['err']}")
             mix.append(info)
+            if log and q % 8 == 0:
+                log(f"mixed {q}/{count}")
         span = max(i["t_done"] for i in mix) - min(i["t_submit"] for i in mix)
     # saturation: every request queued at once, for the sustained throughput
     with Server(precision=precision, max_active=max_active, chunk=chunk) as sv:
         ids = [sv.submit(mesh, os.path.join(workdir, "sat", f"{q:04d}")) for q, (mesh, _, _) in enumerate(reqs)]
-        sat = [sv.wait(rid) for rid in ids]
+        sat = []
+        for q, rid in enumerate(ids):
+            sat.append(sv.wait(rid))
+            if log and q % 8 == 0:
+                log(f"saturated {q}/{count}")
         sat_span = max(i["t_done"] for i in sat) - min(i["t_submit"] for i in sat)
     lat = np.array([i["t_done"] - i["t_submit"] for i in mix])
     ratio = lat / service
