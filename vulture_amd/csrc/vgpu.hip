@@ -302,13 +302,21 @@ int vgpu_create(const vgpu_grid_desc *g, vgpu_prec prec, int device, vgpu_ctx **
     hipEventCreate(&rr.start);
     hipEventCreate(&rr.end);
   }
+  // VGPU_FIELD_STAGGER=<bytes> starts component d's array d times that many
+  // bytes (rounded to 256) into its allocation: a probe of how the arrays'
+  // relative placement in the HBM channels affects the sweeps (default 0).
+  const char *sg = getenv("VGPU_FIELD_STAGGER");
+  const size_t stagger = sg ? ((size_t)strtoull(sg, nullptr, 10) + 255) / 256 * 256 : 0;
   for (int d = 0; d < 6; ++d) {
-    c->F[d] = dev_alloc(c, (c->ncell + FIELD_DISCARD) * c->esz);
-    c->F1[d] = c->leap ? dev_alloc(c, (c->ncell + FIELD_DISCARD) * c->esz) : c->F[d];
-    if (!c->F[d] || !c->F1[d]) {
+    const size_t off = (size_t)d * stagger, bytes = (c->ncell + FIELD_DISCARD) * c->esz + off;
+    void *f = dev_alloc(c, bytes);
+    void *f1 = c->leap ? dev_alloc(c, bytes) : f;
+    if (!f || !f1) {
       destroy_ctx(c);
       return VGPU_ENOMEM;
     }
+    c->F[d] = (char *)f + off;
+    c->F1[d] = (char *)f1 + off;
   }
   for (int d = 0; d < 3; ++d) {
     c->alpha[d] = dev_alloc(c, c->ncell * c->esz);

@@ -161,6 +161,18 @@ def sweep(workdir, count=40, load=0.7, seed=5, precision="fp32", max_active=8, c
                 log(f"saturated {q}/{count}")
         sat_span = max(i["t_done"] for i in sat) - min(i["t_submit"] for i in sat)
     lat = np.array([i["t_done"] - i["t_submit"] for i in mix])
+    # device efficiency of each request run alone (its loop's device time),
+    # by size: how much of the stream's work small grids hold, and how well
+    # the device runs them
+    gpu_s = np.array([i["gpu_ms"] for i in iso]) / 1e3
+    cells = np.array([i["cells"] for i in iso], dtype=np.float64)
+    by_size = []
+    for lo, hi in ((0, 5e5), (5e5, 4e6), (4e6, np.inf)):
+        m = (cells >= lo) & (cells < hi)
+        if m.any():
+            by_size.append({"cells": f"[{lo:.0e}, {hi:.0e})", "requests": int(m.sum()),
+                            "work_share": float(work[m].sum() / work.sum()),
+                            "device_mcells_per_s": float(work[m].sum() / gpu_s[m].sum() / 1e6)})
     ratio = lat / service
     same = [_same_outputs(os.path.join(workdir, "alone", f"{q:04d}"), os.path.join(workdir, "mix", f"{q:04d}")) and
             _same_outputs(os.path.join(workdir, "alone", f"{q:04d}"), os.path.join(workdir, "sat", f"{q:04d}"))
@@ -172,6 +184,8 @@ def sweep(workdir, count=40, load=0.7, seed=5, precision="fp32", max_active=8, c
         "p99_latency_over_isolated": float(np.percentile(ratio, 99)),
         "offered_mcells_per_s": float(work.sum() / span / 1e6),
         "isolated_mcells_per_s": float(work.sum() / service.sum() / 1e6),
+        "isolated_device_mcells_per_s": float(work.sum() / gpu_s.sum() / 1e6),
+        "isolated_by_size": by_size,
         "saturated_mcells_per_s": float(work.sum() / sat_span / 1e6),
         "saturated_over_isolated": float(service.sum() / sat_span),
         "alone_equals_mix": bool(all(same)), "mismatched": [q for q, s in enumerate(same) if not s],
