@@ -143,6 +143,8 @@ def sweep(workdir, count=40, load=0.7, seed=5, precision="fp32", max_active=8, c
             if dt > 0:
                 time.sleep(dt)
             ids.append(sv.submit(mesh, os.path.join(workdir, "mix", f"{q:04d}")))
+            if log and q % 8 == 0:
+                log(f"mixed: submitted {q}/{count}")
         for q, rid in enumerate(ids):
             info = sv.wait(rid)
             if info["status"] != 1:
