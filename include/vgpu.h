@@ -60,6 +60,10 @@ int vgpu_create(const vgpu_grid_desc *grid, vgpu_prec prec, int device, vgpu_ctx
 /* Update coefficients alpha/beta (E) and gamma (H), k-fastest [nx][ny][nz]
  * (src/grid.c:97-105 filled by setMediumOnGrid, src/grid.c:1726-1873). */
 int vgpu_set_media(vgpu_ctx *c, const double *alpha[3], const double *beta[3], const double *gamma[3]);
+/* The same from float arrays: the reference's own build keeps the coefficient
+ * globals in `real` = float (src/fdtd_types.h:36-38), and this takes them as
+ * they are, with no double copy on the host (each value is widened exactly). */
+int vgpu_set_media_f32(vgpu_ctx *c, const float *alpha[3], const float *beta[3], const float *gamma[3]);
 
 /* UPML slabs (src/pml.c:44-84, updates at 859-1176).  region_mask bit r set
  * when face r is PML; fplim the slab field limits; per-axis profile arrays

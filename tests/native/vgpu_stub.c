@@ -44,6 +44,8 @@ int vgpu_create(const vgpu_grid_desc *g, vgpu_prec prec, int device, vgpu_ctx **
 }
 int vgpu_set_media(vgpu_ctx *c, const double *a[3], const double *b[3], const double *g[3])
 { (void)c; (void)a; (void)b; (void)g; return 0; }
+int vgpu_set_media_f32(vgpu_ctx *c, const float *a[3], const float *b[3], const float *g[3])
+{ (void)c; (void)a; (void)b; (void)g; return 0; }
 int vgpu_set_pml(vgpu_ctx *c, const vgpu_pml_desc *p)
 {
   (void)c;

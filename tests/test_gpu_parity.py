@@ -252,6 +252,24 @@ def test_fused_leapfrog_workload(name, kw, tmp_path, monkeypatch):
     _check(_run_pair(mesh, tmp_path, "fp64"), "fp64")
 
 
+@pytest.mark.parametrize("name", ["closedbox_go04", "parplate1D_debyepml_xyz", "planewave_empty_xyz", "dgref_pml6"])
+def test_float_coefficient_entry_point(name, tmp_path, monkeypatch):
+    """vgpu_set_media_f32 (the coefficients as the reference's float build
+    holds them) gives the FP32 storage form the same observer files, byte for
+    byte, as vgpu_set_media with the same values in double: the FP32 form
+    rounds the coefficients to float either way."""
+    import filecmp
+    mesh = _stage(name, tmp_path / "in")
+    print(f"case {name} media f32", flush=True)
+    a, b = tmp_path / "double", tmp_path / "float"
+    vulture_amd.run_mesh(mesh, str(a), precision="fp32")
+    monkeypatch.setenv("VH_MEDIA_F32", "1")
+    vulture_amd.run_mesh(mesh, str(b), precision="fp32")
+    files = sorted(f for f in os.listdir(a) if f.endswith((".asc", ".dat")))
+    assert files and files == sorted(f for f in os.listdir(b) if f.endswith((".asc", ".dat")))
+    assert all(filecmp.cmp(a / f, b / f, shallow=False) for f in files)
+
+
 @pytest.mark.parametrize("name", ["parplate1D_debye_xyz", "parplate1D_debyepml_xyz", "parplate1D_debyeslab_zyx"])
 def test_fused_debye_layer(name, tmp_path, monkeypatch):
     """The first Debye layer applied inside the fast E kernel (numbered by a

@@ -55,7 +55,7 @@ template <typename V> static void *upload_vec(Ctx *c, const std::vector<V> &v)
 
 // Upload a host k-fastest [NX][NY][NZ] double array into an x-fastest device
 // array of the context's precision.
-static int upload_field(Ctx *c, void *dst, const double *src)
+template <typename S> static int upload_field(Ctx *c, void *dst, const S *src)
 {
   const size_t n = c->ncell;
   std::vector<unsigned char> buf(n * c->esz, 0);
@@ -64,7 +64,7 @@ static int upload_field(Ctx *c, void *dst, const double *src)
     for (int j = 0; j < c->NY; ++j) {
       const size_t row = ((size_t)k * c->NY + j) * c->PX;
       for (int i = 0; i < c->NX; ++i) {
-        const double v = src[((size_t)i * NY + j) * NZ + k];
+        const double v = (double)src[((size_t)i * NY + j) * NZ + k];
         if (c->esz == 8) ((double *)buf.data())[row + i] = v;
         else ((float *)buf.data())[row + i] = (float)v;
       }
@@ -86,8 +86,8 @@ template <int N> struct TupleHash {
   }
 };
 
-template <int N>
-static void build_classes(Ctx *c, const double *const src[N], std::vector<uint32_t> &cls, std::vector<double> &tab)
+template <int N, typename S>
+static void build_classes(Ctx *c, const S *const src[N], std::vector<uint32_t> &cls, std::vector<double> &tab)
 {
   cls.assign(c->ncell, 0);
   tab.clear();
@@ -100,7 +100,10 @@ static void build_classes(Ctx *c, const double *const src[N], std::vector<uint32
       for (int k = 0; k < c->NZ; ++k) {
         const size_t h = ((size_t)i * NY + j) * NZ + k;
         std::array<uint64_t, N> key;
-        for (int q = 0; q < N; ++q) memcpy(&key[q], &src[q][h], 8);
+        for (int q = 0; q < N; ++q) {
+          const double v = (double)src[q][h];  // exact for float input
+          memcpy(&key[q], &v, 8);
+        }
         uint32_t id;
         if (prevId != UINT32_MAX && key == prev) {
           id = prevId;
@@ -109,7 +112,7 @@ static void build_classes(Ctx *c, const double *const src[N], std::vector<uint32
           if (it == ids.end()) {
             id = (uint32_t)ids.size();
             ids.emplace(key, id);
-            for (int q = 0; q < N; ++q) tab.push_back(src[q][h]);
+            for (int q = 0; q < N; ++q) tab.push_back((double)src[q][h]);
           } else {
             id = it->second;
           }
@@ -226,6 +229,27 @@ static void *scratch_of_device(int device)
 // hold this lock shared, and a capture holds it alone.
 static std::shared_mutex g_capture_mu;
 using DeviceCalls = std::shared_lock<std::shared_mutex>;
+
+template <typename S> static int set_media(Ctx *c, const S *const alpha[3], const S *const beta[3],
+                                           const S *const gamma[3])
+{
+  if (!c || c->finalized) return VGPU_ESTATE;
+  for (int d = 0; d < 3; ++d)
+    if (!alpha[d] || !beta[d] || !gamma[d]) return set_error(c, VGPU_EINVAL, "missing coefficient array");
+  const DeviceCalls guard(g_capture_mu);
+  for (int d = 0; d < 3; ++d) {
+    int r;
+    if ((r = upload_field(c, c->alpha[d], alpha[d])) || (r = upload_field(c, c->beta[d], beta[d])) ||
+        (r = upload_field(c, c->gamma[d], gamma[d])))
+      return r;
+  }
+  const S *e6[6] = {alpha[0], beta[0], alpha[1], beta[1], alpha[2], beta[2]};
+  build_classes<6, S>(c, e6, c->ecls, c->etab);
+  build_classes<3, S>(c, gamma, c->hcls, c->htab);
+  c->edslot.assign(c->etab.size() / 6 * 3, -1);
+  c->media_set = true;
+  return VGPU_OK;
+}
 
 extern "C" {
 
@@ -347,20 +371,12 @@ int vgpu_create(const vgpu_grid_desc *g, vgpu_prec prec, int device, vgpu_ctx **
 
 int vgpu_set_media(vgpu_ctx *c, const double *alpha[3], const double *beta[3], const double *gamma[3])
 {
-  if (!c || c->finalized) return VGPU_ESTATE;
-  const DeviceCalls guard(g_capture_mu);
-  for (int d = 0; d < 3; ++d) {
-    int r;
-    if ((r = upload_field(c, c->alpha[d], alpha[d])) || (r = upload_field(c, c->beta[d], beta[d])) ||
-        (r = upload_field(c, c->gamma[d], gamma[d])))
-      return r;
-  }
-  const double *e6[6] = {alpha[0], beta[0], alpha[1], beta[1], alpha[2], beta[2]};
-  build_classes<6>(c, e6, c->ecls, c->etab);
-  build_classes<3>(c, gamma, c->hcls, c->htab);
-  c->edslot.assign(c->etab.size() / 6 * 3, -1);
-  c->media_set = true;
-  return VGPU_OK;
+  return set_media<double>(c, alpha, beta, gamma);
+}
+
+int vgpu_set_media_f32(vgpu_ctx *c, const float *alpha[3], const float *beta[3], const float *gamma[3])
+{
+  return set_media<float>(c, alpha, beta, gamma);
 }
 
 int vgpu_set_pml(vgpu_ctx *c, const vgpu_pml_desc *p)

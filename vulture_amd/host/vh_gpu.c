@@ -70,7 +70,27 @@ vgpu_ctx *vh_gpu_setup(vh_solve *s, int prec, int device, unsigned long chunk)
   const double *A[3] = {s->alphaEx, s->alphaEy, s->alphaEz};
   const double *B[3] = {s->betaEx, s->betaEy, s->betaEz};
   const double *G[3] = {s->gammaHx, s->gammaHy, s->gammaHz};
-  if ((rc = vgpu_set_media(c, A, B, G))) goto bad;
+  {
+    /* VH_MEDIA_F32=1 hands the coefficients over as float arrays, as the
+     * reference's float build holds them (the FP32 storage form rounds them to
+     * float anyway, so its results are the same bits either way) */
+    const char *mf = getenv("VH_MEDIA_F32");
+    if (mf && mf[0] == '1') {
+      float *buf = malloc(9 * s->ncell * sizeof(float));
+      if (!buf) { snprintf(s->err, sizeof(s->err), "out of host memory"); vgpu_destroy(c); return NULL; }
+      const double *src[9] = {A[0], A[1], A[2], B[0], B[1], B[2], G[0], G[1], G[2]};
+      for (int q = 0; q < 9; q++)
+        for (size_t n = 0; n < s->ncell; n++) buf[q * s->ncell + n] = (float)src[q][n];
+      const float *fa[3] = {buf, buf + s->ncell, buf + 2 * s->ncell};
+      const float *fb[3] = {buf + 3 * s->ncell, buf + 4 * s->ncell, buf + 5 * s->ncell};
+      const float *fg[3] = {buf + 6 * s->ncell, buf + 7 * s->ncell, buf + 8 * s->ncell};
+      rc = vgpu_set_media_f32(c, fa, fb, fg);
+      free(buf);
+    } else {
+      rc = vgpu_set_media(c, A, B, G);
+    }
+    if (rc) goto bad;
+  }
   {
     vgpu_pml_desc p;
     memset(&p, 0, sizeof(p));
