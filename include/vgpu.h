@@ -209,7 +209,7 @@ typedef struct {
   int leap;                    /* 1: fused leapfrog sweep (E then H in one pass over the fast box) */
   uint64_t fix_cells;          /* H cells of the fused box updated again after the E corrections */
   uint64_t fix_chunks;         /* the 64-cell row chunks holding them */
-  uint64_t graph_runs;         /* runs replayed from a captured graph (VGPU_GRAPHS=1) */
+  uint64_t graph_runs;         /* runs replayed from a captured graph (on unless VGPU_GRAPHS=0) */
 } vgpu_stats;
 int vgpu_get_stats(const vgpu_ctx *c, vgpu_stats *out);
 

@@ -6,7 +6,10 @@ observer file.  Bars:
   FP64 device  time series and raw field DFTs within 1e-12 relative L2;
   FP32 device  time series within 1e-4 relative L2 and spectra within
                0.05 dB over the band where the waveform spectrum lies within
-               60 dB of its peak.
+               60 dB of its peak, in bins whose reference magnitude is at
+               least 1e-3 of the observer's largest in-band one (nulls of a
+               cavity spectrum are rounding noise; compare_outputs also
+               reports the unfloored maximum, fd_db_all).
 The raw DFT is the waveform-normalised spectrum multiplied back by the
 waveform DFT: the normalisation divides by a spectrum that falls by many
 orders of magnitude, which would otherwise turn last-bit differences of the
