@@ -291,7 +291,8 @@ def main():
     ap.add_argument("--boundary", choices=["PML", "PEC", "MUR"], default="PML")
     ap.add_argument("--no-planewave", action="store_true", help="hard point source instead of the TF/SF plane wave")
     ap.add_argument("--serve", type=int, default=32, help="requests in the serving sweep (0 skips it)")
-    ap.add_argument("--serve-active", type=int, default=8, help="solves in flight at once when serving")
+    ap.add_argument("--serve-active", type=int, default=16, help="solves resident at once when serving")
+    ap.add_argument("--serve-workers", type=int, default=12, help="host threads setting up and writing solves")
     ap.add_argument("--serve-nmax", type=int, default=320, help="largest cells per axis of a serving request")
     ap.add_argument("--serve-steps", type=int, nargs=2, default=[2000, 8000], help="step range of serving requests")
     ap.add_argument("--sibc-steps", type=int, default=30,
@@ -361,17 +362,21 @@ def main():
     if args.serve > 0:
         from vulture_amd import serve
         rep = serve.sweep(os.path.join(work, "serve"), count=args.serve, load=0.7, precision="fp32",
-                          max_active=args.serve_active, chunk=200, nmin=32, nmax=args.serve_nmax,
+                          max_active=args.serve_active, workers=args.serve_workers, chunk=200, nmin=32,
+                          nmax=args.serve_nmax,
                           steps_min=args.serve_steps[0], steps_max=args.serve_steps[1], nfreq=64,
                           log=lambda m: print(f"# serving: {m}", file=sys.stderr, flush=True))
         rep["workload"] = (f"{args.serve} seeded requests: cells per axis log-uniform [32, {args.serve_nmax}], "
                            f"graded lines, 4-24 SIMPLE/DEBYE blocks, 8-cell UPML, TF/SF at random angles, "
                            f"{args.serve_steps[0]}-{args.serve_steps[1]} steps, 64-frequency DFT observers; "
-                           f"Poisson arrivals at 70% of isolated capacity; FP32")
+                           f"Poisson arrivals at 70% of isolated capacity; FP32; up to {args.serve_active} solves "
+                           f"resident, least work first, the 4 with least work left running two 200-step "
+                           f"graph-captured chunks each")
         out["serving"] = rep
         print(f"# serving: p50 {rep['p50_latency_s']:.3f} s p99 {rep['p99_latency_s']:.3f} s, "
               f"saturated {rep['saturated_mcells_per_s']:.0f} vs isolated {rep['isolated_mcells_per_s']:.0f} "
-              f"Mcell-updates/s", file=sys.stderr, flush=True)
+              f"Mcell-updates/s, p99 latency / isolated {rep['p99_latency_over_isolated']:.2f}", file=sys.stderr,
+              flush=True)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -13,6 +13,12 @@
  * alone: each solve has its own context and stream, and the step kernels
  * are the same ones.
  *
+ * Order: set-up solves are admitted least work (cells x steps) first, and of
+ * the active solves only the `running` ones with the least work left keep
+ * chunks in flight (two each); the others wait resident until they are among
+ * the least.  A short request that arrives behind long ones therefore runs
+ * almost as fast as alone, without stream priorities.
+ *
  * The reference has no counterpart (one solve per process, simulation.c).
  */
 #define _GNU_SOURCE
@@ -39,7 +45,7 @@ typedef struct {
 } request;
 
 struct vh_server {
-  int device, prec, fullPrecision, maxActive, nworkers, lookahead;
+  int device, prec, fullPrecision, maxActive, nworkers, lookahead, running;
   unsigned long chunk;
   size_t budget, used;
   pthread_mutex_t mu;
@@ -85,6 +91,18 @@ static size_t estimate_bytes(const vh_solve *s, int prec)
   }
   b += (size_t)s->numObs * (s->numTimeSteps * 6 * 8 + 12 * s->numFreq * 8);
   return b + b / 8 + ((size_t)64 << 20);
+}
+
+/* Work of a request (gobox cells x steps) and what is left of it. */
+static double work_of(const request *r) { return (double)r->info.cells * (double)r->info.steps; }
+
+static int cmp_work_left(const void *a, const void *b)
+{
+  const request *x = *(request *const *)a, *y = *(request *const *)b;
+  const double wx = (double)x->info.cells * (double)vh_dsolve_remaining(x->d);
+  const double wy = (double)y->info.cells * (double)vh_dsolve_remaining(y->d);
+  if (wx != wy) return wx < wy ? -1 : 1;
+  return x->info.id < y->info.id ? -1 : x->info.id > y->info.id;
 }
 
 static request *take(vh_server *sv, int from, int to)
@@ -197,14 +215,17 @@ static void *scheduler(void *arg)
       break;
     }
     int admitted = 0;
-    for (int q = 0; q < sv->nrq && sv->active < sv->maxActive; q++) {
-      request *r = sv->rq[q];
-      if (r->state == RQ_QUEUED || r->state == RQ_SETUP) break; /* keep arrival order */
-      if (r->state != RQ_READY) continue;
-      if (sv->used + r->est > sv->budget && sv->active > 0) break;
-      r->state = RQ_ADMITTED;
+    while (sv->active < sv->maxActive) {
+      /* the set-up solve with the least work (arrival order breaks ties) */
+      request *best = NULL;
+      for (int q = 0; q < sv->nrq; q++) {
+        request *r = sv->rq[q];
+        if (r->state == RQ_READY && (!best || work_of(r) < work_of(best))) best = r;
+      }
+      if (!best || (sv->used + best->est > sv->budget && sv->active > 0)) break;
+      best->state = RQ_ADMITTED;
       sv->ready--;
-      sv->used += r->est;
+      sv->used += best->est;
       sv->active++;
       admitted = 1;
     }
@@ -213,6 +234,8 @@ static void *scheduler(void *arg)
       if (sv->rq[q]->state == RQ_ACTIVE) act[n++] = sv->rq[q];
     if (admitted) pthread_cond_broadcast(&sv->cvWork);
     pthread_mutex_unlock(&sv->mu);
+    /* least work left first; only the first `running` get new chunks */
+    qsort(act, (size_t)n, sizeof(request *), cmp_work_left);
 
     int busy = 0;
     for (int q = 0; q < n; q++) {
@@ -221,10 +244,10 @@ static void *scheduler(void *arg)
       const int ready = vh_dsolve_ready(r->d);
       if (ready < 0) rc = -1;
       else if (ready == 1) {
-        /* finish the oldest chunk, then keep two in flight */
+        /* finish the oldest chunk, then keep two in flight if running */
         rc = vh_dsolve_complete(r->d);
         long e = 0;
-        while (!rc && (e = vh_dsolve_enqueue(r->d)) > 0) busy = 1;
+        while (!rc && q < sv->running && (e = vh_dsolve_enqueue(r->d)) > 0) busy = 1;
         if (e < 0) rc = -1;
         if (!rc && vh_dsolve_done(r->d)) {
           r->info.gpu_ms = vh_dsolve_loop_ms(r->d);
@@ -271,7 +294,15 @@ vh_server *vh_server_create(int device, int prec, int fullPrecision, double budg
   sv->budget = (size_t)budgetBytes;
   sv->maxActive = maxActive > 0 ? maxActive : 8;
   sv->nworkers = workers > 0 ? workers : 4;
-  sv->lookahead = sv->maxActive;
+  /* set-up solves kept ready beyond the active ones, so that a short request
+   * arriving behind long ones is set up and can be admitted first */
+  sv->lookahead = 2 * sv->maxActive;
+  /* solves with chunks in flight at once (the device's hardware queues are
+   * shared by fewer streams, and the rest wait resident); VH_SERVE_RUNNING
+   * overrides */
+  sv->running = sv->maxActive < 4 ? sv->maxActive : 4;
+  const char *rn = getenv("VH_SERVE_RUNNING");
+  if (rn && atoi(rn) > 0) sv->running = atoi(rn);
   sv->chunk = chunk ? chunk : 200;
   clock_gettime(CLOCK_MONOTONIC, &sv->t0);
   pthread_mutex_init(&sv->mu, NULL);

@@ -116,7 +116,8 @@ def _same_outputs(a, b):
                                             for f in fa)
 
 
-def sweep(workdir, count=40, load=0.7, seed=5, precision="fp32", max_active=8, chunk=200, log=None, **kw):
+def sweep(workdir, count=40, load=0.7, seed=5, precision="fp32", max_active=8, chunk=200, log=None, workers=8,
+          **kw):
     """Isolated pass, then the Poisson-arrival mixed pass; returns a report dict."""
     reqs = make_requests(os.path.join(workdir, "meshes"), count, seed=seed, **kw)
     rng = np.random.default_rng(seed + 1000)
@@ -134,7 +135,7 @@ def sweep(workdir, count=40, load=0.7, seed=5, precision="fp32", max_active=8, c
     rate = load / service.mean()                      # arrivals per second
     gaps = rng.exponential(1.0 / rate, size=count)
     mix = []
-    with Server(precision=precision, max_active=max_active, chunk=chunk) as sv:
+    with Server(precision=precision, max_active=max_active, chunk=chunk, workers=workers) as sv:
         t0 = time.monotonic()
         arrive = np.cumsum(gaps) - gaps[0]
         ids = []
@@ -154,7 +155,7 @@ def sweep(workdir, count=40, load=0.7, seed=5, precision="fp32", max_active=8, c
                 log(f"mixed {q}/{count}")
         span = max(i["t_done"] for i in mix) - min(i["t_submit"] for i in mix)
     # saturation: every request queued at once, for the sustained throughput
-    with Server(precision=precision, max_active=max_active, chunk=chunk) as sv:
+    with Server(precision=precision, max_active=max_active, chunk=chunk, workers=workers) as sv:
         ids = [sv.submit(mesh, os.path.join(workdir, "sat", f"{q:04d}")) for q, (mesh, _, _) in enumerate(reqs)]
         sat = []
         for q, rid in enumerate(ids):
@@ -163,6 +164,14 @@ def sweep(workdir, count=40, load=0.7, seed=5, precision="fp32", max_active=8, c
                 log(f"saturated {q}/{count}")
         sat_span = max(i["t_done"] for i in sat) - min(i["t_submit"] for i in sat)
     lat = np.array([i["t_done"] - i["t_submit"] for i in mix])
+    # where the mixed pass's time goes: waiting for and doing the host set-up,
+    # waiting for admission, on the device, writing the files
+    phase = {"setup": [i["t_ready"] - i["t_submit"] for i in mix],
+             "admission": [i["t_admit"] - i["t_ready"] for i in mix],
+             "device": [i["t_gpu_done"] - i["t_admit"] for i in mix],
+             "write": [i["t_done"] - i["t_gpu_done"] for i in mix]}
+    phases = {k: {"p50_s": float(np.percentile(v, 50)), "p99_s": float(np.percentile(v, 99))}
+              for k, v in phase.items()}
     # device efficiency of each request run alone (its loop's device time),
     # by size: how much of the stream's work small grids hold, and how well
     # the device runs them
@@ -184,6 +193,13 @@ def sweep(workdir, count=40, load=0.7, seed=5, precision="fp32", max_active=8, c
         "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
         "p50_isolated_s": float(np.percentile(service, 50)), "p99_isolated_s": float(np.percentile(service, 99)),
         "p99_latency_over_isolated": float(np.percentile(ratio, 99)),
+        "p50_latency_over_isolated": float(np.percentile(ratio, 50)),
+        "mixed_phases": phases,
+        "worst": [{"request": int(q), "cells": int(iso[q]["cells"]), "steps": int(iso[q]["steps"]),
+                   "isolated_s": float(service[q]), "latency_s": float(lat[q]),
+                   **{k: float(v[q]) for k, v in phase.items()}}
+                  for q in np.argsort(-ratio)[:3]],
+        "workers": workers,
         "offered_mcells_per_s": float(work.sum() / span / 1e6),
         "isolated_mcells_per_s": float(work.sum() / service.sum() / 1e6),
         "isolated_device_mcells_per_s": float(work.sum() / gpu_s.sum() / 1e6),
