@@ -16,8 +16,10 @@
  * Order: set-up solves are admitted least work (cells x steps) first, and of
  * the active solves only the `running` ones with the least work left keep
  * chunks in flight (two each); the others wait resident until they are among
- * the least.  A short request that arrives behind long ones therefore runs
- * almost as fast as alone, without stream priorities.
+ * the least.  With two running solves (the default) the two shortest
+ * remaining ones overlap on the device and every other waits, preempted at
+ * chunk boundaries: a short request that arrives behind long ones shares the
+ * device with one other solve at most, without stream priorities.
  *
  * The reference has no counterpart (one solve per process, simulation.c).
  */
@@ -46,6 +48,7 @@ typedef struct {
 
 struct vh_server {
   int device, prec, fullPrecision, maxActive, nworkers, lookahead, running;
+  int nextWorker, uploaders;  /* the first `uploaders` workers never set up */
   unsigned long chunk;
   size_t budget, used;
   pthread_mutex_t mu;
@@ -132,6 +135,9 @@ static void *worker(void *arg)
 {
   vh_server *sv = arg;
   pthread_mutex_lock(&sv->mu);
+  /* a few workers only upload and write, so an admitted solve never waits
+   * behind the parsing of large meshes */
+  const int me = sv->nextWorker++;
   while (!sv->stop) {
     request *r;
     if ((r = take(sv, RQ_WRITE, RQ_WRITING))) {
@@ -167,7 +173,7 @@ static void *worker(void *arg)
         r->info.t_admit = now(sv);
         r->state = RQ_ACTIVE;
       }
-    } else if (sv->ready < sv->lookahead && (r = take(sv, RQ_QUEUED, RQ_SETUP))) {
+    } else if (me >= sv->uploaders && sv->ready < sv->lookahead && (r = take(sv, RQ_QUEUED, RQ_SETUP))) {
       pthread_mutex_unlock(&sv->mu);
       vh_solve *s = vh_create();
       int rc = s ? 0 : -1;
@@ -297,12 +303,14 @@ vh_server *vh_server_create(int device, int prec, int fullPrecision, double budg
   /* set-up solves kept ready beyond the active ones, so that a short request
    * arriving behind long ones is set up and can be admitted first */
   sv->lookahead = 2 * sv->maxActive;
-  /* solves with chunks in flight at once (the device's hardware queues are
-   * shared by fewer streams, and the rest wait resident); VH_SERVE_RUNNING
-   * overrides */
-  sv->running = sv->maxActive < 4 ? sv->maxActive : 4;
+  /* solves with chunks in flight at once; the rest wait resident.  Two
+   * overlap on the device; one lowers the tail latency a little more and
+   * shares nothing, four raise throughput and the tail with it;
+   * VH_SERVE_RUNNING overrides */
+  sv->running = sv->maxActive < 2 ? sv->maxActive : 2;
   const char *rn = getenv("VH_SERVE_RUNNING");
   if (rn && atoi(rn) > 0) sv->running = atoi(rn);
+  sv->uploaders = sv->nworkers > 4 ? 2 : 0;
   sv->chunk = chunk ? chunk : 200;
   clock_gettime(CLOCK_MONOTONIC, &sv->t0);
   pthread_mutex_init(&sv->mu, NULL);
