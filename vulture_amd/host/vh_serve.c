@@ -108,6 +108,17 @@ static int cmp_work_left(const void *a, const void *b)
   return x->info.id < y->info.id ? -1 : x->info.id > y->info.id;
 }
 
+/* The request in state `from` with the least work (arrival order breaks
+ * ties), moved to state `to`: uploads go shortest first, as admission does. */
+static request *take_least(vh_server *sv, int from, int to)
+{
+  request *best = NULL;
+  for (int q = 0; q < sv->nrq; q++)
+    if (sv->rq[q]->state == from && (!best || work_of(sv->rq[q]) < work_of(best))) best = sv->rq[q];
+  if (best) best->state = to;
+  return best;
+}
+
 static request *take(vh_server *sv, int from, int to)
 {
   for (int q = 0; q < sv->nrq; q++)
@@ -153,7 +164,7 @@ static void *worker(void *arg)
       r->info.status = 1;
       r->state = RQ_DONE;
       pthread_cond_broadcast(&sv->cvDone);
-    } else if ((r = take(sv, RQ_ADMITTED, RQ_UPLOAD))) {
+    } else if ((r = take_least(sv, RQ_ADMITTED, RQ_UPLOAD))) {
       pthread_mutex_unlock(&sv->mu);
       vh_dsolve *d = vh_dsolve_open(r->s, sv->prec, sv->device, sv->chunk);
       if (d) vh_release_grid_arrays(r->s);
@@ -310,7 +321,7 @@ vh_server *vh_server_create(int device, int prec, int fullPrecision, double budg
   sv->running = sv->maxActive < 2 ? sv->maxActive : 2;
   const char *rn = getenv("VH_SERVE_RUNNING");
   if (rn && atoi(rn) > 0) sv->running = atoi(rn);
-  sv->uploaders = sv->nworkers > 4 ? 2 : 0;
+  sv->uploaders = sv->nworkers > 6 ? 3 : sv->nworkers > 4 ? 2 : 0;
   sv->chunk = chunk ? chunk : 200;
   clock_gettime(CLOCK_MONOTONIC, &sv->t0);
   pthread_mutex_init(&sv->mu, NULL);
