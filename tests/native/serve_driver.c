@@ -1,7 +1,9 @@
 /* Drives the solve server over the meshes named on the command line (test
- * infrastructure): submit all, wait, print one line per request. */
+ * infrastructure): submit all (SERVE_DRIVER_GAP_MS apart), wait, print one
+ * line per request. */
 #include <stdio.h>
 #include <stdlib.h>
+#include <unistd.h>
 #include "vh_serve.h"
 
 int main(int argc, char **argv)
@@ -14,7 +16,9 @@ int main(int argc, char **argv)
   int n = argc - 3;
   int *ids = calloc((size_t)n, sizeof(int));
   char dir[4096];
+  const char *gap = getenv("SERVE_DRIVER_GAP_MS");
   for (int q = 0; q < n; q++) {
+    if (q && gap) usleep((useconds_t)(1000 * atoi(gap)));
     snprintf(dir, sizeof(dir), "%s/%d", argv[2], q);
     ids[q] = vh_server_submit(sv, argv[3 + q], dir);
   }
@@ -22,7 +26,7 @@ int main(int argc, char **argv)
   for (int q = 0; q < n; q++) {
     vh_request_info info;
     int rc = vh_server_wait(sv, ids[q], &info);
-    printf("%d %d %lld %lu %s\n", q, rc, info.cells, info.steps, info.err);
+    printf("%d %d %lld %lu %.6f %s\n", q, rc, info.cells, info.steps, info.t_gpu_done, info.err);
     if (rc) bad++;
   }
   vh_server_destroy(sv);

@@ -68,3 +68,23 @@ def test_failed_request_is_reported_and_others_finish(driver, tmp_path):
     assert "ThreadSanitizer" not in r.stderr, r.stderr[-4000:]
     rcs = [l.split()[1] for l in r.stdout.strip().splitlines()]
     assert rcs == ["0", "-1", "0"], r.stdout
+
+
+def test_short_request_overtakes_long_one(driver, tmp_path):
+    """Shortest remaining work first: with one running solve, a short request
+    submitted 20 ms after a long one, while the long one runs, finishes its
+    device work first (in arrival order it would wait for the long one's 700
+    chunks)."""
+    big = tmp_path / "big.mesh"
+    big.write_text(meshgen.freespace(steps=5000, n=24, pml=4))
+    small = tmp_path / "small.mesh"
+    small.write_text(meshgen.freespace(steps=14, n=20, pml=4))
+    env = dict(os.environ, VH_SERVE_RUNNING="1", SERVE_DRIVER_GAP_MS="20", TSAN_OPTIONS="halt_on_error=1")
+    r = subprocess.run([driver, "2", str(tmp_path / "out"), str(big), str(small)], capture_output=True, text=True,
+                       timeout=300, env=env, cwd=str(tmp_path))
+    assert r.returncode == 0 and "ThreadSanitizer" not in r.stderr, r.stdout + r.stderr[-3000:]
+    rows = [l.split() for l in r.stdout.strip().splitlines()]
+    t_big, t_small = float(rows[0][4]), float(rows[1][4])
+    # preempting at the next chunk boundary, the short one is done long before
+    # the long one (taking turns, it would end just before or after it)
+    assert t_small < 0.5 * t_big, rows
