@@ -73,10 +73,10 @@ def test_failed_request_is_reported_and_others_finish(driver, tmp_path):
 def test_short_request_overtakes_long_one(driver, tmp_path):
     """Shortest remaining work first: with one running solve, a short request
     submitted 20 ms after a long one, while the long one runs, finishes its
-    device work first (in arrival order it would wait for the long one's 700
+    device work first (in arrival order it would wait for the long one's 2900
     chunks)."""
     big = tmp_path / "big.mesh"
-    big.write_text(meshgen.freespace(steps=5000, n=24, pml=4))
+    big.write_text(meshgen.freespace(steps=20000, n=24, pml=4))
     small = tmp_path / "small.mesh"
     small.write_text(meshgen.freespace(steps=14, n=20, pml=4))
     env = dict(os.environ, VH_SERVE_RUNNING="1", SERVE_DRIVER_GAP_MS="20", TSAN_OPTIONS="halt_on_error=1")
