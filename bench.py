@@ -291,7 +291,8 @@ def main():
     ap.add_argument("--boundary", choices=["PML", "PEC", "MUR"], default="PML")
     ap.add_argument("--no-planewave", action="store_true", help="hard point source instead of the TF/SF plane wave")
     ap.add_argument("--serve", type=int, default=32, help="requests in the serving sweep (0 skips it)")
-    ap.add_argument("--serve-active", type=int, default=16, help="solves resident at once when serving")
+    ap.add_argument("--serve-active", type=int, default=64,
+                    help="solves resident at once when serving (device memory permitting)")
     ap.add_argument("--serve-workers", type=int, default=12, help="host threads setting up and writing solves")
     ap.add_argument("--serve-nmax", type=int, default=320, help="largest cells per axis of a serving request")
     ap.add_argument("--serve-steps", type=int, nargs=2, default=[2000, 8000], help="step range of serving requests")

@@ -313,7 +313,7 @@ vh_server *vh_server_create(int device, int prec, int fullPrecision, double budg
   sv->nworkers = workers > 0 ? workers : 4;
   /* set-up solves kept ready beyond the active ones, so that a short request
    * arriving behind long ones is set up and can be admitted first */
-  sv->lookahead = 2 * sv->maxActive;
+  sv->lookahead = 2 * sv->maxActive < 32 ? 2 * sv->maxActive : 32;
   /* solves with chunks in flight at once; the rest wait resident.  Two
    * overlap on the device; one lowers the tail latency a little more and
    * shares nothing, four raise throughput and the tail with it;
