@@ -371,8 +371,9 @@ def main():
                            f"graded lines, 4-24 SIMPLE/DEBYE blocks, 8-cell UPML, TF/SF at random angles, "
                            f"{args.serve_steps[0]}-{args.serve_steps[1]} steps, 64-frequency DFT observers; "
                            f"Poisson arrivals at 70% of isolated capacity; FP32; up to {args.serve_active} solves "
-                           f"resident, least work first, the two with least work left running two 200-step "
-                           f"graph-captured chunks each")
+                           f"resident, least work first, the two with least work left running two "
+                           f"graph-captured chunks each (200 steps, fewer above 5 M cells: at most 1e9 "
+                           f"cell-updates a chunk)")
         out["serving"] = rep
         print(f"# serving: p50 {rep['p50_latency_s']:.3f} s p99 {rep['p99_latency_s']:.3f} s, "
               f"saturated {rep['saturated_mcells_per_s']:.0f} vs isolated {rep['isolated_mcells_per_s']:.0f} "

@@ -96,6 +96,20 @@ static size_t estimate_bytes(const vh_solve *s, int prec)
   return b + b / 8 + ((size_t)64 << 20);
 }
 
+/* Steps per chunk of a request: the server's chunk, or fewer for large grids,
+ * so that a chunk is at most about 1e9 cell-updates (about 40 ms on the
+ * device).  A running solve gives way to a shorter one at its next chunk
+ * boundary, and large grids would otherwise hold the device for a long chunk
+ * after losing their turn.  Even, so captured graphs replay on either field
+ * set; at least 20 steps. */
+static unsigned long chunk_of(const vh_server *sv, const request *r)
+{
+  const double cap = 1e9 / (double)(r->info.cells > 0 ? r->info.cells : 1);
+  unsigned long c = sv->chunk;
+  if ((double)c > cap) c = (unsigned long)cap & ~1ul;
+  return c < 20 ? 20 : c;
+}
+
 /* Work of a request (gobox cells x steps) and what is left of it. */
 static double work_of(const request *r) { return (double)r->info.cells * (double)r->info.steps; }
 
@@ -166,7 +180,7 @@ static void *worker(void *arg)
       pthread_cond_broadcast(&sv->cvDone);
     } else if ((r = take_least(sv, RQ_ADMITTED, RQ_UPLOAD))) {
       pthread_mutex_unlock(&sv->mu);
-      vh_dsolve *d = vh_dsolve_open(r->s, sv->prec, sv->device, sv->chunk);
+      vh_dsolve *d = vh_dsolve_open(r->s, sv->prec, sv->device, chunk_of(sv, r));
       if (d) vh_release_grid_arrays(r->s);
       pthread_mutex_lock(&sv->mu);
       if (!d) {
