@@ -26,7 +26,13 @@ float_fd_db record the error of the reference's own float arithmetic (the
 float oracle, byte-identical to the reference's float build) against FP64,
 the yardstick of the FP32 device run there.
 
+For a generated case, `--float-error case ...` runs the float oracle on the
+same mesh and adds float_td and float_fd_db against the stored FP64 arrays to
+its fixture (for configurations long and resonant enough that float rounding
+alone exceeds the FP32 bars).
+
 Usage: python tests/golden/make_fp64_fixtures.py [case ...]   (default: all)
+       python tests/golden/make_fp64_fixtures.py --float-error case ...
 """
 import hashlib
 import os
@@ -109,7 +115,52 @@ def make(case):
         shutil.rmtree(work, ignore_errors=True)
 
 
+def write_fixture_files(fx, dst, wf_src):
+    """The fixture's arrays as observer files under dst, beside the waveform
+    spectrum files of the run in wf_src (host-computed, the same in every run)."""
+    import glob
+    os.makedirs(dst, exist_ok=True)
+    for key in fx.files:
+        if key.startswith(("td/", "fd/")):
+            with open(os.path.join(dst, key[3:]), "w") as f:
+                for row in fx[key]:
+                    head = f"{int(row[0])} " if key.startswith("td/") else f"{float(row[0])!r} "
+                    f.write(head + " ".join(repr(float(x)) for x in row[1:]) + "\n")
+    for w in glob.glob(os.path.join(wf_src, "wf_*_fd.asc")):
+        shutil.copy(w, dst)
+
+
+def add_float_error(case):
+    from vulture_amd.obsfiles import compare_outputs
+    path = os.path.join(OUT, f"{case}.npz")
+    fx = np.load(path)
+    arrays = {k: fx[k] for k in fx.files}
+    work = tempfile.mkdtemp(prefix=f"f32_{case}_")
+    try:
+        mesh = stage(case, work)
+        if mesh_sha256(mesh) != str(fx["mesh_sha256"]):
+            raise SystemExit(f"{case}: the mesh generator changed since the fixture was made")
+        t0 = time.time()
+        out32 = os.path.join(work, "oracle32")
+        oracle.run_mesh(mesh, out32, precision="fp32", full_precision=True, threads=os.cpu_count() or 1,
+                        timeout=24 * 3600)
+        ref = os.path.join(work, "fixture")
+        write_fixture_files(fx, ref, out32)
+        r = compare_outputs(out32, ref)
+        arrays["float_td"] = np.array(r["td"])
+        arrays["float_fd_db"] = np.array(r["fd_db"])
+        np.savez_compressed(path, **arrays)
+        print(f"{case}: float oracle against FP64: td {r['td']:.3e}, fd_db {r['fd_db']:.3e} "
+              f"({r['files']} files), {time.time() - t0:.0f} s", flush=True)
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
 if __name__ == "__main__":
-    cases = sys.argv[1:] or REFERENCE + [c for c, _, _, _ in GENERATED]
-    for c in cases:
-        make(c)
+    if sys.argv[1:2] == ["--float-error"]:
+        for c in sys.argv[2:]:
+            add_float_error(c)
+    else:
+        cases = sys.argv[1:] or REFERENCE + [c for c, _, _, _ in GENERATED]
+        for c in cases:
+            make(c)
