@@ -12,10 +12,13 @@ precision, tests/test_oracle_reference.py):
 Bars: FP64 time series and raw DFTs within 1e-12 relative L2.  FP32 within
 1e-4 time-series relative L2 and 0.05 dB over the band (the waveform
 spectrum within 60 dB of its peak, bins above 1e-3 of the largest reference
-magnitude); on the reference meshes, whose fields are cancellation-dominated,
-within twice the reference's own float error where that is larger: there
-both float runs are rounding noise against FP64, two realizations of it
-(sgref_pml6: 1.2 dB for the float reference, 1.9 dB for the device).
+magnitude), or within twice the reference's own float error where the
+fixture records it and that is larger: on the reference meshes, whose fields
+are cancellation-dominated, both float runs are rounding noise against FP64,
+two realizations of it (sgref_pml6: 1.2 dB for the float reference, 1.9 dB
+for the device); in the SIBC + Mur box, a resonant cavity run for 6000
+steps, float rounding alone accumulates past 1e-4 (float_td, written by
+make_fp64_fixtures.py --float-error).
 
 The reference meshes observe small scattered fields next to a large incident
 one, so rounding in the last bit of the fields shows up in the observed

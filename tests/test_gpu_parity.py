@@ -273,22 +273,6 @@ def test_float_coefficient_entry_point(name, tmp_path, monkeypatch):
     assert all(filecmp.cmp(a / f, b / f, shallow=False) for f in files)
 
 
-@pytest.mark.parametrize("name", ["dgref_pml6", "parplate1D_debyepml_xyz", "planewave_pec_xyz"])
-def test_e_shell_six_waves(name, tmp_path, monkeypatch):
-    """The E shell compiled to fit six waves per SIMD (VGPU_ESHELL_WAVES=6)
-    computes the same operations: the same observer files, byte for byte,
-    in FP32."""
-    import filecmp
-    mesh = _stage(name, tmp_path / "in")
-    print(f"case {name} e-shell six waves", flush=True)
-    a, b = tmp_path / "default", tmp_path / "six"
-    vulture_amd.run_mesh(mesh, str(a), precision="fp32")
-    monkeypatch.setenv("VGPU_ESHELL_WAVES", "6")
-    vulture_amd.run_mesh(mesh, str(b), precision="fp32")
-    files = sorted(f for f in os.listdir(a) if f.endswith((".asc", ".dat")))
-    assert files and all(filecmp.cmp(a / f, b / f, shallow=False) for f in files)
-
-
 @pytest.mark.parametrize("name", ["parplate1D_debye_xyz", "parplate1D_debyepml_xyz", "parplate1D_debyeslab_zyx"])
 def test_fused_debye_layer(name, tmp_path, monkeypatch):
     """The first Debye layer applied inside the fast E kernel (numbered by a

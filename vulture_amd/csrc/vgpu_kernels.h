@@ -695,11 +695,8 @@ template <typename T, bool IDX, int C> struct EShellCell {
   }
 };
 
-// WPE: the waves per SIMD the compiler must fit (1 leaves it free: 86 VGPRs,
-// five waves in FP32; 6 fits 80 VGPRs with a few spills).
-template <typename T, bool IDX, bool CHK = false, int WPE = 1>
-__global__ __launch_bounds__(256, WPE) void k_e_shell(const ShellPiece *pcs, const ShellWork *work,
-                                                       const EArgs<T> *ap)
+template <typename T, bool IDX, bool CHK = false>
+__global__ __launch_bounds__(256) void k_e_shell(const ShellPiece *pcs, const ShellWork *work, const EArgs<T> *ap)
 {
   const ShellPiece *pc;
   int di, dj, dk0, dk1;

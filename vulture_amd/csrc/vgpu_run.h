@@ -471,15 +471,12 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
       else hipLaunchKernelGGL((k_e_fast<T, false, false>), eg1, vblk, 0, s, f);
     }
   };
-  const char *ew = getenv("VGPU_ESHELL_WAVES");  // tuning: 6 waves per SIMD for the E shell
-  const bool eshell6 = ew && atoi(ew) == 6;
   auto shell_e = [&](int q) {
     if (c->nEwork) {
       const dim3 g((unsigned)c->nEwork);
       const EArgs<T> *a = eap[q];
       if (chk && c->indexed) hipLaunchKernelGGL((k_e_shell<T, true, true>), g, dim3(256), 0, s, epc, ewk, a);
       else if (chk) hipLaunchKernelGGL((k_e_shell<T, false, true>), g, dim3(256), 0, s, epc, ewk, a);
-      else if (c->indexed && eshell6) hipLaunchKernelGGL((k_e_shell<T, true, false, 6>), g, dim3(256), 0, s, epc, ewk, a);
       else if (c->indexed) hipLaunchKernelGGL((k_e_shell<T, true>), g, dim3(256), 0, s, epc, ewk, a);
       else hipLaunchKernelGGL((k_e_shell<T, false>), g, dim3(256), 0, s, epc, ewk, a);
     }
