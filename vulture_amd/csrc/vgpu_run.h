@@ -477,7 +477,9 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
       const EArgs<T> *a = eap[q];
       if (chk && c->indexed) hipLaunchKernelGGL((k_e_shell<T, true, true>), g, dim3(256), 0, s, epc, ewk, a);
       else if (chk) hipLaunchKernelGGL((k_e_shell<T, false, true>), g, dim3(256), 0, s, epc, ewk, a);
+      else if (c->indexed && !c->leap) hipLaunchKernelGGL((k_e_shell<T, true, false, false>), g, dim3(256), 0, s, epc, ewk, a);
       else if (c->indexed) hipLaunchKernelGGL((k_e_shell<T, true>), g, dim3(256), 0, s, epc, ewk, a);
+      else if (!c->leap) hipLaunchKernelGGL((k_e_shell<T, false, false, false>), g, dim3(256), 0, s, epc, ewk, a);
       else hipLaunchKernelGGL((k_e_shell<T, false>), g, dim3(256), 0, s, epc, ewk, a);
     }
   };
