@@ -16,11 +16,10 @@
  * Order: set-up solves are admitted least work (cells x steps) first, and of
  * the active solves only the `running` ones with the least work left keep
  * chunks in flight (two each); the others wait resident until they are among
- * the least.  With two running solves (the default), the one with the least
- * work left and the oldest of the others overlap on the device and every
- * other waits, preempted at chunk boundaries: a short request that arrives
- * behind long ones shares the device with one other solve at most, without
- * stream priorities, and the longest requests are not starved.
+ * the least.  With two running solves (the default) the two shortest
+ * remaining ones overlap on the device and every other waits, preempted at
+ * chunk boundaries: a short request that arrives behind long ones shares the
+ * device with one other solve at most, without stream priorities.
  *
  * The reference has no counterpart (one solve per process, simulation.c).
  */
@@ -235,7 +234,7 @@ static void *worker(void *arg)
   return NULL;
 }
 
-/* Scheduler: admission least work first, chunk enqueueing, completion. */
+/* Scheduler: admission in arrival order, chunk enqueueing, completion. */
 static void *scheduler(void *arg)
 {
   vh_server *sv = arg;
@@ -266,19 +265,8 @@ static void *scheduler(void *arg)
       if (sv->rq[q]->state == RQ_ACTIVE) act[n++] = sv->rq[q];
     if (admitted) pthread_cond_broadcast(&sv->cvWork);
     pthread_mutex_unlock(&sv->mu);
-    /* least work left first; only the first `running` get new chunks.  With
-     * two or more running, the last running place goes to the oldest active
-     * solve instead, so that the longest requests, which every shorter one
-     * would otherwise overtake, keep moving too */
+    /* least work left first; only the first `running` get new chunks */
     qsort(act, (size_t)n, sizeof(request *), cmp_work_left);
-    if (sv->running >= 2 && n > sv->running) {
-      int old = sv->running - 1;
-      for (int q = sv->running; q < n; q++)
-        if (act[q]->info.id < act[old]->info.id) old = q;
-      request *t = act[sv->running - 1];
-      act[sv->running - 1] = act[old];
-      act[old] = t;
-    }
 
     int busy = 0;
     for (int q = 0; q < n; q++) {
