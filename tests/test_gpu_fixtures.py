@@ -17,8 +17,9 @@ fixture records it and that is larger: on the reference meshes, whose fields
 are cancellation-dominated, both float runs are rounding noise against FP64,
 two realizations of it (sgref_pml6: 1.2 dB for the float reference, 1.9 dB
 for the device); in the SIBC + Mur box, a resonant cavity run for 6000
-steps, float rounding alone accumulates past 1e-4 (float_td, written by
-make_fp64_fixtures.py --float-error).
+steps, the reference's float arithmetic drifts to 4.0e-3 and 0.11 dB from
+FP64 (float_td, written by make_fp64_fixtures.py --float-error), while the
+FP32 device form, with its FP64 SIBC state, stays at 3.3e-4 and 0.003 dB.
 
 The reference meshes observe small scattered fields next to a large incident
 one, so rounding in the last bit of the fields shows up in the observed
@@ -91,7 +92,9 @@ def test_device_against_fp64_fixture(case, precision, tmp_path):
     ftd = float(fx["float_td"]) if "float_td" in fx.files else 0.0
     fdb = float(fx["float_fd_db"]) if "float_fd_db" in fx.files else 0.0
     if precision == "fp64":
-        bar = max(1e-12, 4.0 * ftd * 2.0 ** -29)
+        # the rounding-level bar only where cancellation sets the level (the
+        # reflection meshes); elsewhere 1e-12 however long the run
+        bar = max(1e-12, 4.0 * ftd * 2.0 ** -29) if case.startswith(("sgref", "dgref")) else 1e-12
         assert r["td"] <= bar and r["fd_raw"] <= bar, (r, bar)
     else:
         assert r["td"] <= max(1e-4, 2.0 * ftd), (r, ftd)
