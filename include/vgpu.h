@@ -156,6 +156,21 @@ int vgpu_finalize(vgpu_ctx *c, uint64_t total_steps, uint64_t max_steps_per_run)
 int vgpu_run(vgpu_ctx *c, uint64_t first, uint64_t nsteps, void *stream);
 int vgpu_synchronize(vgpu_ctx *c);
 
+/* Advance up to VGPU_GROUP_MAX solves by nsteps each in lockstep, context i
+ * over steps [first[i], first[i]+nsteps): one packed launch per step kernel
+ * over all of them (fast and shell sweeps, Debye layers, corrections,
+ * observers) instead of one launch per solve, for grids too small to fill
+ * the device alone.  Each context ends as after vgpu_run(ctxs[i], first[i],
+ * nsteps, NULL), bit for bit, with its own run record (vgpu_run_result).
+ * Runs on the first context's stream after the work queued on every
+ * member's stream; each member's stream waits for the group's end.  Every
+ * member must use the separate sweeps (not VGPU_LEAP, not a fused Debye
+ * layer) and share device and precision.  No counterpart in the reference,
+ * which runs one solve per process (src/simulation.c:136-230); it serves
+ * many small solves on one GPU. */
+#define VGPU_GROUP_MAX 16
+int vgpu_run_group(vgpu_ctx *const *ctxs, int n, const uint64_t *first, uint64_t nsteps);
+
 /* Non-blocking completion test of the last vgpu_run: 1 done, 0 still
  * running, <0 device error.  No counterpart in the reference, whose loop is
  * synchronous; it lets one host thread keep many solves in flight. */

@@ -108,6 +108,12 @@ int vgpu_run(vgpu_ctx *c, uint64_t first, uint64_t n, void *s)
   c->polls = rand() % 4;
   return 0;
 }
+int vgpu_run_group(vgpu_ctx *const *cs, int n, const uint64_t *first, uint64_t nsteps)
+{
+  if (n < 1 || n > VGPU_GROUP_MAX) return VGPU_EINVAL;
+  for (int i = 0; i < n; i++) vgpu_run(cs[i], first[i], nsteps, NULL);
+  return 0;
+}
 int vgpu_run_result(vgpu_ctx *c, uint64_t first, int wait, double *ms, const float **samples, uint64_t *nsteps)
 {
   int q = -1;

@@ -88,3 +88,43 @@ def test_short_request_overtakes_long_one(driver, tmp_path):
     # preempting at the next chunk boundary, the short one is done long before
     # the long one (taking turns, it would end just before or after it)
     assert t_small < 0.5 * t_big, rows
+
+
+@pytest.fixture(scope="module")
+def group_driver(tmp_path_factory):
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    d = tmp_path_factory.mktemp("group")
+    exe = str(d / "group_driver")
+    cmd = ["gcc", "-std=gnu99", "-O1", "-g", "-fsanitize=address,undefined", "-DVH_REAL=double", "-I" + HOST,
+           "-Wno-stringop-truncation", "-Wno-format-truncation", "-Wno-unused-result"]
+    cmd += [os.path.join(HOST, f) for f in SRC]
+    cmd += [os.path.join(NATIVE, "vgpu_stub.c"), os.path.join(NATIVE, "group_driver.c"), "-lm", "-lpthread", "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        pytest.fail(r.stderr[-2000:])
+    return exe
+
+
+def test_group_run_host_side(group_driver, tmp_path):
+    """Members of different lengths leave the group as they finish; every
+    member's files are written; more than the group maximum is refused."""
+    meshes = []
+    rng = np.random.default_rng(9)
+    for q in range(5):
+        text, _, _ = meshgen.serving_request(rng, nmin=12, nmax=20, steps_min=30, steps_max=90, nfreq=8)
+        p = tmp_path / f"g{q}.mesh"
+        p.write_text(text)
+        meshes.append(str(p))
+    for name in ("closedbox_go04", "hardsrc_jmsxy"):
+        shutil.copytree(os.path.join(GOLDEN, "ref_inputs", name), tmp_path / name)
+        meshes.append(str(tmp_path / name / f"{name}.mesh"))
+    r = subprocess.run([group_driver, "16", str(tmp_path / "out")] + meshes, capture_output=True, text=True,
+                       timeout=600, cwd=str(tmp_path))
+    assert r.returncode == 0 and r.stdout.startswith("0 ok"), r.stdout[-2000:] + r.stderr[-4000:]
+    for q in range(len(meshes)):
+        assert any(f.endswith(".asc") for f in os.listdir(tmp_path / "out" / str(q))), q
+    assert os.path.getsize(tmp_path / "out" / str(len(meshes) - 1) / "impulse.dat") > 0
+    r = subprocess.run([group_driver, "16", str(tmp_path / "out2")] + [meshes[0]] * 17, capture_output=True,
+                       text=True, timeout=600, cwd=str(tmp_path))
+    assert r.returncode == 1 and "1 to 16" in r.stdout, r.stdout + r.stderr[-2000:]

@@ -52,6 +52,11 @@ def load_host():
         f.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_long,
                       ctypes.c_ulong, ctypes.POINTER(ctypes.c_double), ctypes.c_char_p, ctypes.c_int]
         f.restype = ctypes.c_int
+        f = _host.vh_run_meshes_gpu_group
+        f.argtypes = [ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_char_p), ctypes.c_int, ctypes.c_int,
+                      ctypes.c_int, ctypes.c_int, ctypes.c_ulong, ctypes.POINTER(ctypes.c_double), ctypes.c_char_p,
+                      ctypes.c_int]
+        f.restype = ctypes.c_int
         f = _host.vh_gpu_open
         f.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_long, ctypes.c_ulong,
                       ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p), ctypes.c_char_p, ctypes.c_int]
@@ -105,6 +110,36 @@ def run_mesh(mesh, outdir, precision="fp64", full_precision=True, steps=-1, chun
     err = ctypes.create_string_buffer(1024)
     rc = host.vh_run_mesh_gpu(os.fsencode(mesh), os.fsencode(outdir), 1 if precision == "fp32" else 0, device,
                               1 if full_precision else 0, steps, chunk, ctypes.byref(ms), err, len(err))
+    if rc != 0:
+        raise VgpuError(err.value.decode(errors="replace"))
+    return ms.value
+
+
+GROUP_MAX = 16  # VGPU_GROUP_MAX (include/vgpu.h)
+
+
+def run_meshes_group(meshes, outdirs, precision="fp32", full_precision=True, chunk=0, device=0):
+    """Run several Vulture meshes on the GPU together, as one packed group.
+
+    Every step kernel is launched once over all of them (vgpu_run_group)
+    instead of once per mesh, for grids too small to fill the device alone;
+    each mesh's observer files come out as run_mesh writes them.  At most
+    GROUP_MAX meshes.  Returns the host's wall time of the time loop in ms.
+    """
+    if precision not in ("fp64", "fp32"):
+        raise ValueError("precision must be 'fp64' or 'fp32'")
+    if len(meshes) != len(outdirs) or not 1 <= len(meshes) <= GROUP_MAX:
+        raise ValueError(f"1 to {GROUP_MAX} meshes, each with an output directory")
+    host = load_host()
+    for d in outdirs:
+        os.makedirs(d, exist_ok=True)
+    n = len(meshes)
+    m = (ctypes.c_char_p * n)(*[os.fsencode(x) for x in meshes])
+    o = (ctypes.c_char_p * n)(*[os.fsencode(x) for x in outdirs])
+    ms = ctypes.c_double(0.0)
+    err = ctypes.create_string_buffer(1024)
+    rc = host.vh_run_meshes_gpu_group(m, o, n, 1 if precision == "fp32" else 0, device, 1 if full_precision else 0,
+                                      chunk, ctypes.byref(ms), err, len(err))
     if rc != 0:
         raise VgpuError(err.value.decode(errors="replace"))
     return ms.value

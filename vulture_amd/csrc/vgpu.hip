@@ -1296,6 +1296,8 @@ int vgpu_finalize(vgpu_ctx *c, uint64_t total_steps, uint64_t max_steps_per_run)
       (r = build_observers(c)) || (r = build_leap(c)))
     return r;
   if ((r = c->prec == VGPU_FP64 ? prepare_sweeps<double>(c) : prepare_sweeps<float>(c))) return r;
+  if (!c->leap && !c->debyeFused && (r = c->prec == VGPU_FP64 ? upload_pack<double>(c) : upload_pack<float>(c)))
+    return r;
   // set-up copies are synchronous and its kernels ran on the solve's stream
   if (hipStreamSynchronize(c->stream) != hipSuccess || hipGetLastError() != hipSuccess)
     return set_error(c, VGPU_EDEVICE, "device error during set-up");
@@ -1334,42 +1336,35 @@ static void capture_graph(Ctx *c, uint64_t nsteps)
   c->graphPar = c->par;
 }
 
-int vgpu_run(vgpu_ctx *c, uint64_t first, uint64_t nsteps, void *stream)
+static int check_run(Ctx *c, uint64_t first, uint64_t nsteps)
 {
   if (!c || !c->finalized) return VGPU_ESTATE;
   if (nsteps > c->maxRun && c->vnodes) return set_error(c, VGPU_EINVAL, "run longer than the observer staging");
   if (!c->pobs.empty() && first + nsteps > c->totalSteps) return set_error(c, VGPU_EINVAL, "run past total steps");
-  hipSetDevice(c->device);
-  (void)hipGetLastError();  // launch errors reported below belong to this run only
-  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
-  Ctx::RunRec &rr = c->runs[c->runCount & 1];
+  return VGPU_OK;
+}
+
+// A run's start events on stream s.
+static void open_run(Ctx *c, hipStream_t s)
+{
   hipEventRecord(c->ev0, s);
-  hipEventRecord(rr.start, s);
+  hipEventRecord(c->runs[c->runCount & 1].start, s);
+}
+
+// The step counter, on stream s, when a run does not continue the last one.
+static void set_step(Ctx *c, uint64_t first, hipStream_t s)
+{
   if (c->stepHost != first)
     hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, s, (unsigned long long *)c->d_step,
                        (unsigned long long)first);
-  // Runs of the staging length are replayed from a graph captured once,
-  // when the solve is long enough to replay it at least twice (a graph of an
-  // even number of steps starts and ends on the same field set, so it
-  // replays wherever that set is current); VGPU_GRAPHS=0 launches every run
-  // directly.  The capture holds g_capture_mu alone, so no other thread
-  // synchronizes the device while it runs.
-  const char *ge = getenv("VGPU_GRAPHS");
-  const bool graphable = !(ge && ge[0] == '0') && s == c->stream && c->evE.empty() && nsteps >= 2 &&
-                         nsteps % 2 == 0 && nsteps == c->maxRun && c->totalSteps >= 2 * nsteps;
-  int r = VGPU_OK;
-  if (graphable && !c->graphExec && !c->graphTried) capture_graph(c, nsteps);
-  if (graphable && c->graphExec && c->graphSteps == nsteps && c->graphPar == c->par) {
-    if (hipGraphLaunch(c->graphExec, s) != hipSuccess)
-      r = set_error(c, VGPU_EDEVICE, "graph launch failed: %s", hipGetErrorString(hipGetLastError()));
-    ++c->graphRuns;
-    c->lastRunTimed = false;
-  } else {
-    r = c->prec == VGPU_FP64 ? run_steps<double>(c, nsteps, s, true) : run_steps<float>(c, nsteps, s, true);
-    c->lastRunTimed = true;
-  }
-  // this run's volume samples to its slot of the pinned ring, after its
-  // kernels on the same stream (rows wrap at the end of the device ring)
+}
+
+// A run's close on stream s, after its kernels: its volume samples to its
+// slot of the pinned ring (rows wrap at the end of the device ring), its end
+// events and its record.
+static void close_run(Ctx *c, uint64_t first, uint64_t nsteps, hipStream_t s)
+{
+  Ctx::RunRec &rr = c->runs[c->runCount & 1];
   if (c->vnodes && nsteps) {
     const size_t row = (size_t)c->vnodes * 6;
     float *dst = c->h_vring + (size_t)(c->runCount & 1) * c->maxRun * row;
@@ -1388,6 +1383,79 @@ int vgpu_run(vgpu_ctx *c, uint64_t first, uint64_t nsteps, void *stream)
   c->stepHost = first + nsteps;
   c->lastRunFirst = first;
   c->lastRunSteps = nsteps;
+}
+
+int vgpu_run(vgpu_ctx *c, uint64_t first, uint64_t nsteps, void *stream)
+{
+  int r = check_run(c, first, nsteps);
+  if (r) return r;
+  hipSetDevice(c->device);
+  (void)hipGetLastError();  // launch errors reported below belong to this run only
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  open_run(c, s);
+  set_step(c, first, s);
+  // Runs of the staging length are replayed from a graph captured once,
+  // when the solve is long enough to replay it at least twice (a graph of an
+  // even number of steps starts and ends on the same field set, so it
+  // replays wherever that set is current); VGPU_GRAPHS=0 launches every run
+  // directly.  The capture holds g_capture_mu alone, so no other thread
+  // synchronizes the device while it runs.
+  const char *ge = getenv("VGPU_GRAPHS");
+  const bool graphable = !(ge && ge[0] == '0') && s == c->stream && c->evE.empty() && nsteps >= 2 &&
+                         nsteps % 2 == 0 && nsteps == c->maxRun && c->totalSteps >= 2 * nsteps;
+  if (graphable && !c->graphExec && !c->graphTried) capture_graph(c, nsteps);
+  if (graphable && c->graphExec && c->graphSteps == nsteps && c->graphPar == c->par) {
+    if (hipGraphLaunch(c->graphExec, s) != hipSuccess)
+      r = set_error(c, VGPU_EDEVICE, "graph launch failed: %s", hipGetErrorString(hipGetLastError()));
+    ++c->graphRuns;
+    c->lastRunTimed = false;
+  } else {
+    r = c->prec == VGPU_FP64 ? run_steps<double>(c, nsteps, s, true) : run_steps<float>(c, nsteps, s, true);
+    c->lastRunTimed = true;
+  }
+  close_run(c, first, nsteps, s);
+  return r;
+}
+
+int vgpu_run_group(vgpu_ctx *const *ctxs, int n, const uint64_t *first, uint64_t nsteps)
+{
+  if (!ctxs || !first || n < 1 || n > VGPU_GROUP_MAX) return VGPU_EINVAL;
+  for (int i = 0; i < n; ++i) {
+    Ctx *c = ctxs[i];
+    int r = check_run(c, first[i], nsteps);
+    if (r) return r;
+    if (!c->d_pack) return set_error(c, VGPU_EINVAL, "group runs need the separate sweeps");
+    if (c->device != ctxs[0]->device || c->prec != ctxs[0]->prec)
+      return set_error(c, VGPU_EINVAL, "group members differ in device or precision");
+    for (int j = 0; j < i; ++j)
+      if (ctxs[j] == c) return set_error(c, VGPU_EINVAL, "context twice in one group");
+  }
+  hipSetDevice(ctxs[0]->device);
+  (void)hipGetLastError();
+  // The group runs on the first member's stream, after the work already
+  // queued on every member's stream; each member's stream then waits for the
+  // group's end before its own later work.
+  hipStream_t s = ctxs[0]->stream;
+  for (int i = 0; i < n; ++i) {
+    Ctx *c = ctxs[i];
+    open_run(c, c->stream);
+    if (i) hipStreamWaitEvent(s, c->runs[c->runCount & 1].start, 0);
+  }
+  for (int i = 0; i < n; ++i) set_step(ctxs[i], first[i], s);
+  Ctx *cs[VGPU_GROUP_MAX];
+  for (int i = 0; i < n; ++i) cs[i] = ctxs[i];
+  const int r = ctxs[0]->prec == VGPU_FP64 ? run_group_steps<double>(cs, n, nsteps, s)
+                                           : run_group_steps<float>(cs, n, nsteps, s);
+  // the first member closes last, so its end follows all of the group's work
+  // on its stream
+  for (int k = 1; k <= n; ++k) {
+    const int i = k % n;
+    Ctx *c = ctxs[i];
+    c->lastRunTimed = false;
+    hipEvent_t end = c->runs[c->runCount & 1].end;
+    close_run(c, first[i], nsteps, s);
+    if (i) hipStreamWaitEvent(c->stream, end, 0);
+  }
   return r;
 }
 

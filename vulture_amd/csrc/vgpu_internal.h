@@ -388,6 +388,41 @@ template <typename T> struct VolObsArgs {
   unsigned long long ring;   // staging rows: two runs' worth
 };
 
+// ---- packed launches over several solves (vgpu_run_group) ----
+
+constexpr int PACK_MAX = VGPU_GROUP_MAX;  // solves in one group run
+
+// One solve's by-value argument blocks for the packed launches, for field set
+// 0 (the one the separate sweeps read and write), kept in device memory.  A
+// packed launch passes the step offset itself.
+template <typename T> struct PackArgs {
+  FastE<T> fe;
+  FastH<T> fh;
+  const ShellPiece *epc, *hpc;
+  const ShellWork *ewk, *hwk;
+  const EArgs<T> *ea;
+  const HArgs<T> *ha;
+  DebyeArgs<T> da;
+  CorrArgs<T> ce, ch;    // E and H corrections
+  ObsArgs<T> oa;
+  AuxArgs<T> xa;
+  VolObsArgs<T> va;
+  int headGx;            // x blocks of the solve's own k_head launch
+};
+
+// Argument of a packed launch: the solves it covers, in launch order, and
+// the first launch block (or block row) of each: solve d owns blocks
+// start[d] .. start[d+1]-1 and runs them as its own launch would run blocks
+// 0 .. start[d+1]-start[d]-1.
+struct PackMap {
+  int n;                         // solves in this launch
+  int soff;                      // step offset within the run
+  unsigned start[PACK_MAX + 1];
+  const void *args[PACK_MAX];    // each solve's PackArgs
+  long long base[PACK_MAX];      // Debye launches: first site of the layer
+  long long cnt[PACK_MAX];       // Debye: sites in the layer; H corrections: correction blocks
+};
+
 template <typename T> struct GhostArgs {
   int NX, NY, NZ, PX;
   long long plane;
@@ -568,6 +603,7 @@ struct Ctx {
   // device step counter (read by the time-dependent kernels) and the host's
   // view of its value once the enqueued work has run
   void *d_step = nullptr;
+  void *d_pack = nullptr;  // PackArgs for group runs (separate sweeps only)
   uint64_t stepHost = 0;
   // a captured graph of graphSteps whole steps (replayed by vgpu_run)
   hipGraph_t graph = nullptr;
@@ -613,6 +649,11 @@ template <typename T>
 int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool timeKernels, const VisitRun *chk = nullptr);
 // Uploads the sweeps' argument blocks (once, at finalize).
 template <typename T> int prepare_sweeps(Ctx *c);
+// Uploads the context's argument blocks for packed launches (once, at finalize).
+template <typename T> int upload_pack(Ctx *c);
+// Enqueues nsteps steps of n contexts in lockstep on stream s, with packed
+// launches, then advances each one's step counter.
+template <typename T> int run_group_steps(Ctx *const *cs, int n, uint64_t nsteps, hipStream_t s);
 // Update-once check of the field sweeps against the outer field limits.  Its
 // buffers come from `scratch` (scratchBytes long) when given, else they are
 // allocated for the call.

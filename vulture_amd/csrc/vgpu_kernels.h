@@ -151,11 +151,12 @@ VG_DEV T debye_site(T e, T beta, long long g, int m, T *lastE_, T *J_, long long
 // padded to a multiple of eight and launch index b runs logical block
 // (b % 8) * (n / 8) + b / 8: every XCD sweeps one contiguous range of blocks
 // (a z range of the domain), where neighbouring rows and planes share its L2.
-VG_DEV unsigned xcd_block()
+VG_DEV unsigned xcd_block(unsigned b, unsigned n)
 {
-  const unsigned per = gridDim.x / 8;
-  return (blockIdx.x % 8) * per + blockIdx.x / 8;
+  const unsigned per = n / 8;
+  return (b % 8) * per + b / 8;
 }
+VG_DEV unsigned xcd_block() { return xcd_block(blockIdx.x, gridDim.x); }
 
 // Layer-0 Debye correction inside the fast E kernel (debye.c:243-307).  The
 // chunk's first site comes from dbase (uniform over the wavefront, so rows
@@ -201,12 +202,11 @@ VG_DEV T e_fast(const FastE<T> &f, int bx, int j, int k, bool vx, long long id, 
 // instead of storing them, and leaves every field and auxiliary untouched.
 VG_DEV void visit(unsigned *v, long long id) { atomicAdd(v + id, 1u); }
 
-// Fast E kernel: inner grid (grid.c:709-797) and, with FUSE, the first Debye
-// layer.
-template <typename T, bool IDX, bool FUSE, bool CHK = false>
-__global__ __launch_bounds__(512) void k_e_fast(FastE<T> f)
+// Fast E tile: inner grid (grid.c:709-797) and, with FUSE, the first Debye
+// layer.  L is the logical block (in XCD order).
+template <typename T, bool IDX, bool FUSE, bool CHK>
+VG_DEV void e_fast_tile(const FastE<T> &f, unsigned L)
 {
-  const unsigned L = xcd_block();
   if (L >= (unsigned)(f.nchx * f.gy * f.gz)) return;
   const int bx = (int)(L % f.nchx), by = (int)((L / f.nchx) % f.gy), bz = (int)(L / (f.nchx * f.gy));
   const int i = f.box[0] + bx * 64 + (int)threadIdx.x;
@@ -250,10 +250,15 @@ __global__ __launch_bounds__(512) void k_e_fast(FastE<T> f)
   }
 }
 
-// Fast H kernel: inner grid (grid.c:800-888).
-template <typename T, bool IDX, bool CHK = false> __global__ __launch_bounds__(512) void k_h_fast(FastH<T> f)
+template <typename T, bool IDX, bool FUSE, bool CHK = false>
+__global__ __launch_bounds__(512) void k_e_fast(FastE<T> f)
 {
-  const unsigned L = xcd_block();
+  e_fast_tile<T, IDX, FUSE, CHK>(f, xcd_block());
+}
+
+// Fast H tile: inner grid (grid.c:800-888).
+template <typename T, bool IDX, bool CHK> VG_DEV void h_fast_tile(const FastH<T> &f, unsigned L)
+{
   if (L >= (unsigned)(f.nchx * f.gy * f.gz)) return;
   const int bx = (int)(L % f.nchx), by = (int)((L / f.nchx) % f.gy), bz = (int)(L / (f.nchx * f.gy));
   const int i = f.box[0] + bx * 64 + (int)threadIdx.x;
@@ -308,6 +313,11 @@ template <typename T, bool IDX, bool CHK = false> __global__ __launch_bounds__(5
     ex = exp1;
     ey = eyp1;
   }
+}
+
+template <typename T, bool IDX, bool CHK = false> __global__ __launch_bounds__(512) void k_h_fast(FastH<T> f)
+{
+  h_fast_tile<T, IDX, CHK>(f, xcd_block());
 }
 
 // ===========================================================================
@@ -554,10 +564,10 @@ template <typename T, bool IDX> __global__ __launch_bounds__(256) void k_h_fix(F
 // registers, and each plane issues all its loads before its first store.  The
 // launch order of the blocks (ShellWork) gives each XCD one contiguous eighth
 // of every piece.
-VG_DEV bool shell_column(const ShellPiece *pcs, const ShellWork *work, const ShellPiece *&pc, int &di, int &dj,
-                         int &dk0, int &dk1)
+VG_DEV bool shell_column(const ShellPiece *pcs, const ShellWork *work, unsigned b, const ShellPiece *&pc, int &di,
+                         int &dj, int &dk0, int &dk1)
 {
-  const ShellWork w = work[blockIdx.x];
+  const ShellWork w = work[b];
   if (w.piece < 0) return false;
   pc = pcs + w.piece;
   const int zc = w.block / pc->nxyb, xyb = w.block - zc * pc->nxyb;
@@ -698,12 +708,12 @@ template <typename T, bool IDX, int C> struct EShellCell {
   }
 };
 
-template <typename T, bool IDX, bool CHK = false, bool SEP = true>
-__global__ __launch_bounds__(256) void k_e_shell(const ShellPiece *pcs, const ShellWork *work, const EArgs<T> *ap)
+template <typename T, bool IDX, bool CHK, bool SEP>
+VG_DEV void e_shell_column(const ShellPiece *pcs, const ShellWork *work, const EArgs<T> *ap, unsigned b)
 {
   const ShellPiece *pc;
   int di, dj, dk0, dk1;
-  if (!shell_column(pcs, work, pc, di, dj, dk0, dk1)) return;
+  if (!shell_column(pcs, work, b, pc, di, dj, dk0, dk1)) return;
   const EArgs<T> &a = *ap;
   const EShellPtrs<T> g(a, SEP);
   const int NY = a.NY;
@@ -754,6 +764,12 @@ __global__ __launch_bounds__(256) void k_e_shell(const ShellPiece *pcs, const Sh
     rg[1].q += rg[1].psz;
     rg[2].q += rg[2].psz;
   }
+}
+
+template <typename T, bool IDX, bool CHK = false, bool SEP = true>
+__global__ __launch_bounds__(256) void k_e_shell(const ShellPiece *pcs, const ShellWork *work, const EArgs<T> *ap)
+{
+  e_shell_column<T, IDX, CHK, SEP>(pcs, work, ap, blockIdx.x);
 }
 
 // The H shell's arrays as global pointers.
@@ -828,12 +844,12 @@ template <typename T, bool IDX, int C> struct HShellCell {
   }
 };
 
-template <typename T, bool IDX, bool CHK = false>
-__global__ __launch_bounds__(256) void k_h_shell(const ShellPiece *pcs, const ShellWork *work, const HArgs<T> *ap)
+template <typename T, bool IDX, bool CHK>
+VG_DEV void h_shell_column(const ShellPiece *pcs, const ShellWork *work, const HArgs<T> *ap, unsigned b)
 {
   const ShellPiece *pc;
   int di, dj, dk0, dk1;
-  if (!shell_column(pcs, work, pc, di, dj, dk0, dk1)) return;
+  if (!shell_column(pcs, work, b, pc, di, dj, dk0, dk1)) return;
   const HArgs<T> &a = *ap;
   const HShellPtrs<T> q(a);
   const int NX = a.NX, NY = a.NY, NZ = a.NZ;
@@ -883,6 +899,12 @@ __global__ __launch_bounds__(256) void k_h_shell(const ShellPiece *pcs, const Sh
     rg[1].q += rg[1].psz;
     rg[2].q += rg[2].psz;
   }
+}
+
+template <typename T, bool IDX, bool CHK = false>
+__global__ __launch_bounds__(256) void k_h_shell(const ShellPiece *pcs, const ShellWork *work, const HArgs<T> *ap)
+{
+  h_shell_column<T, IDX, CHK>(pcs, work, ap, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -961,17 +983,21 @@ static __global__ void k_visit_tally(const unsigned *vis, int NX, int NY, int NZ
 // ---------------------------------------------------------------------------
 // Debye ADE correction of one layer of sites (debye.c:243-307)
 // ---------------------------------------------------------------------------
-template <typename T> __global__ void k_debye(DebyeArgs<T> a)
+template <typename T> VG_DEV void debye_layer_site(const DebyeArgs<T> &a, long long base, long long n, long long s)
 {
-  const long long s = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (s >= a.n) return;
-  const long long g = a.base + s;
+  if (s >= n) return;
+  const long long g = base + s;
   const long long key = a.key[g];
   const int c = (int)(key & 3), m = (int)(key >> 42);
   const long long id = (key >> 2) & ((1LL << 40) - 1);
   T *E = a.E[c];
   E[id] = debye_site<T>(E[id], a.sbeta[g], g, m, a.lastE, a.J, a.stride, a.dalpha, a.dbeta, a.npoles, a.maxPoles,
                         a.cplx);
+}
+
+template <typename T> __global__ void k_debye(DebyeArgs<T> a)
+{
+  debye_layer_site(a, a.base, a.n, blockIdx.x * (long long)blockDim.x + threadIdx.x);
 }
 
 // Each Debye site's beta, gathered once: from the coefficient classes when
@@ -1283,11 +1309,10 @@ __global__ void k_corr_gather(long long nrows, const long long *key, int shift, 
   if (rowalpha) rowalpha[r] = alpha ? alpha[c][id] : T(1);
 }
 
-template <typename T> VG_DEV void corr_rows(const CorrArgs<T> &a, double pi)
+template <typename T> VG_DEV void corr_rows(const CorrArgs<T> &a, double pi, long long r, int soff)
 {
-  const long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (r >= a.nrows) return;
-  const double t = ((double)(*a.step + (unsigned long long)a.soff) + a.half) * a.dt;
+  const double t = ((double)(*a.step + (unsigned long long)soff) + a.half) * a.dt;
   const long long key = a.rowkey[r];
   const long long id = key >> 2;
   T *F = a.F[key & 3];
@@ -1324,14 +1349,23 @@ template <typename T> VG_DEV void corr_rows(const CorrArgs<T> &a, double pi)
   F[id] = f;
 }
 
-template <typename T> __global__ void k_corr(CorrArgs<T> a, double pi) { corr_rows(a, pi); }
+template <typename T> __global__ void k_corr(CorrArgs<T> a, double pi)
+{
+  corr_rows(a, pi, blockIdx.x * (long long)blockDim.x + threadIdx.x, a.soff);
+}
 
 // H corrections with the aux grids' H update in the blocks from corrBlocks
 // on: it writes aux H, which the H corrections do not read (they read aux E).
+template <typename T>
+VG_DEV void corr_aux_block(const CorrArgs<T> &a, double pi, const AuxArgs<T> &xa, int corrBlocks, int b, int soff)
+{
+  if (b >= corrBlocks) aux_h_block(xa, b - corrBlocks);
+  else corr_rows(a, pi, b * (long long)blockDim.x + threadIdx.x, soff);
+}
+
 template <typename T> __global__ void k_corr_aux(CorrArgs<T> a, double pi, AuxArgs<T> xa, int corrBlocks)
 {
-  if ((int)blockIdx.x >= corrBlocks) aux_h_block(xa, (int)blockIdx.x - corrBlocks);
-  else corr_rows(a, pi);
+  corr_aux_block(a, pi, xa, corrBlocks, (int)blockIdx.x, a.soff);
 }
 
 // ---------------------------------------------------------------------------
@@ -1369,16 +1403,19 @@ template <typename T> VG_DEV void observe_block(const ObsArgs<T> &a, int o)
 // aux E update reads the aux H of the previous step and is read only by the
 // H corrections, so it may run before the E sweep instead of after it
 // (planewave.c:576-590 runs it just before the E corrections).
-template <typename T> __global__ void k_head(ObsArgs<T> oa, AuxArgs<T> xa, double pi)
+template <typename T> VG_DEV void head_block(const ObsArgs<T> &oa, const AuxArgs<T> &xa, double pi, int y)
 {
-  const int y = blockIdx.y;
   if (y < oa.nobs) observe_block(oa, y);
   else if (blockIdx.x == 0) aux_e_block(xa, y - oa.nobs, pi);
 }
 
-template <typename T> __global__ void k_observe_volume(VolObsArgs<T> a)
+template <typename T> __global__ void k_head(ObsArgs<T> oa, AuxArgs<T> xa, double pi)
 {
-  const long long nd = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  head_block(oa, xa, pi, (int)blockIdx.y);
+}
+
+template <typename T> VG_DEV void observe_volume_node(const VolObsArgs<T> &a, long long nd)
+{
   if (nd >= a.nnodes) return;
   const int o = a.nodeobs[nd];
   const unsigned long long n = *a.step + (unsigned long long)a.soff;
@@ -1386,6 +1423,11 @@ template <typename T> __global__ void k_observe_volume(VolObsArgs<T> a)
   const long long id = a.nodecell[nd];
   float *out = a.stage + ((long long)(n % a.ring) * a.nnodes + nd) * 6;
   for (int c = 0; c < 6; ++c) out[c] = (float)a.F[c][id];
+}
+
+template <typename T> __global__ void k_observe_volume(VolObsArgs<T> a)
+{
+  observe_volume_node(a, blockIdx.x * (long long)blockDim.x + threadIdx.x);
 }
 
 // Ghost plane copy for one face (surface.c:569-812).
@@ -1403,6 +1445,117 @@ template <typename T> __global__ void k_ghost(GhostArgs<T> a)
   const long long di = ((long long)d3[2] * a.NY + d3[1]) * a.PX + d3[0];
   const long long si = ((long long)s3[2] * a.NY + s3[1]) * a.PX + s3[0];
   a.F[di] = a.sign * a.F[si];
+}
+
+// ---------------------------------------------------------------------------
+// Packed launches: one launch of a step kernel over several solves of a group
+// run.  A block finds its solve among the map's block ranges (a scan over at
+// most PACK_MAX uniform entries), reads that solve's argument block through
+// the constant address space (scalar loads) and runs the same code its own
+// launch would, with its block number in that launch; block ranges of the
+// fast kernels are multiples of eight, so each solve keeps its XCD order.
+// ---------------------------------------------------------------------------
+VG_DEV int pack_member(const PackMap &m, unsigned b)
+{
+  int d = 0;
+  while (d + 1 < m.n && b >= m.start[d + 1]) ++d;
+  return d;
+}
+
+template <typename T> VG_DEV const VG_CONST PackArgs<T> *pack_args(const PackMap &m, int d)
+{
+  return cp(static_cast<const PackArgs<T> *>(m.args[d]));
+}
+
+// An argument block copied out of the constant address space, word by word.
+template <typename S> VG_DEV S ld_args(const VG_CONST S *p)
+{
+  static_assert(sizeof(S) % 4 == 0, "argument blocks are whole words");
+  S out;
+  const VG_CONST unsigned *w = reinterpret_cast<const VG_CONST unsigned *>(p);
+  unsigned *o = reinterpret_cast<unsigned *>(&out);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(S) / 4); ++i) o[i] = w[i];
+  return out;
+}
+
+template <typename T, bool IDX> __global__ __launch_bounds__(512) void k_e_fast_pk(PackMap m)
+{
+  const int d = pack_member(m, blockIdx.x);
+  const FastE<T> f = ld_args(&pack_args<T>(m, d)->fe);
+  e_fast_tile<T, IDX, false, false>(f, xcd_block(blockIdx.x - m.start[d], m.start[d + 1] - m.start[d]));
+}
+
+template <typename T, bool IDX> __global__ __launch_bounds__(512) void k_h_fast_pk(PackMap m)
+{
+  const int d = pack_member(m, blockIdx.x);
+  const FastH<T> f = ld_args(&pack_args<T>(m, d)->fh);
+  h_fast_tile<T, IDX, false>(f, xcd_block(blockIdx.x - m.start[d], m.start[d + 1] - m.start[d]));
+}
+
+template <typename T, bool IDX> __global__ __launch_bounds__(256) void k_e_shell_pk(PackMap m)
+{
+  const int d = pack_member(m, blockIdx.x);
+  const VG_CONST PackArgs<T> *a = pack_args<T>(m, d);
+  e_shell_column<T, IDX, false, false>(a->epc, a->ewk, a->ea, blockIdx.x - m.start[d]);
+}
+
+template <typename T, bool IDX> __global__ __launch_bounds__(256) void k_h_shell_pk(PackMap m)
+{
+  const int d = pack_member(m, blockIdx.x);
+  const VG_CONST PackArgs<T> *a = pack_args<T>(m, d);
+  h_shell_column<T, IDX, false>(a->hpc, a->hwk, a->ha, blockIdx.x - m.start[d]);
+}
+
+// The Debye and correction kernels pick a component's array by an index read
+// from memory; from a copy in registers that would be an indexed read of
+// scratch, so they read their argument blocks in place.
+template <typename T> VG_DEV const PackArgs<T> &pack_ref(const PackMap &m, int d)
+{
+  return *static_cast<const PackArgs<T> *>(m.args[d]);
+}
+
+template <typename T> __global__ void k_debye_pk(PackMap m)
+{
+  const int d = pack_member(m, blockIdx.x);
+  debye_layer_site(pack_ref<T>(m, d).da, m.base[d], m.cnt[d],
+                   (blockIdx.x - m.start[d]) * (long long)blockDim.x + threadIdx.x);
+}
+
+template <typename T> __global__ void k_corr_e_pk(PackMap m, double pi)
+{
+  const int d = pack_member(m, blockIdx.x);
+  corr_rows(pack_ref<T>(m, d).ce, pi, (blockIdx.x - m.start[d]) * (long long)blockDim.x + threadIdx.x, m.soff);
+}
+
+// H corrections, and the aux grids' H update in each solve's blocks from
+// cnt[d] on (none when the solve has no aux grid).
+template <typename T> __global__ void k_corr_h_pk(PackMap m, double pi)
+{
+  const int d = pack_member(m, blockIdx.x);
+  const PackArgs<T> &p = pack_ref<T>(m, d);
+  corr_aux_block(p.ch, pi, p.xa, (int)m.cnt[d], (int)(blockIdx.x - m.start[d]), m.soff);
+}
+
+// Head of a step: block rows are the solves' own k_head rows; x blocks past
+// a solve's own launch width return.
+template <typename T> __global__ void k_head_pk(PackMap m, double pi)
+{
+  const int d = pack_member(m, blockIdx.y);
+  const VG_CONST PackArgs<T> *p = pack_args<T>(m, d);
+  if ((int)blockIdx.x >= p->headGx) return;
+  ObsArgs<T> oa = ld_args(&p->oa);
+  AuxArgs<T> xa = ld_args(&p->xa);
+  oa.soff = xa.soff = m.soff;
+  head_block(oa, xa, pi, (int)(blockIdx.y - m.start[d]));
+}
+
+template <typename T> __global__ void k_observe_volume_pk(PackMap m)
+{
+  const int d = pack_member(m, blockIdx.x);
+  VolObsArgs<T> a = ld_args(&pack_args<T>(m, d)->va);
+  a.soff = m.soff;
+  observe_volume_node(a, (blockIdx.x - m.start[d]) * (long long)blockDim.x + threadIdx.x);
 }
 
 // Device step counter: set before a run that does not continue the previous

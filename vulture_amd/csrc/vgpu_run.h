@@ -400,26 +400,48 @@ template <typename T> int prepare_sweeps(Ctx *c)
   return VGPU_OK;
 }
 
-// The loop (simulation.c:136-230).  With chk, one step of the field sweeps
-// only (Mur E, the E sweep, the H sweep), in their update-counting form.
-// With the fused sweep, step q of the run reads field set c->par ^ (q & 1)
-// and writes the other, and the caller flips c->par by the parity of nsteps
-// once the run is enqueued; the separate sweeps read and write set 0.
-template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool timeKernels, const VisitRun *chk)
+// Argument blocks of one solve's step launches, for both field sets (index:
+// the set a step reads), built on the host before a run.
+template <typename T> struct StepArgs {
+  EArgs<T> ea[2];
+  HArgs<T> ha[2];
+  dim3 vblk, egrid, hgrid, eg1, hg1;   // fast kernels: block, tile grid, padded 1-D launch
+  FastE<T> fe[2];
+  FastH<T> fh[2];
+  const EArgs<T> *eap[2];              // argument blocks in device memory (shells)
+  const HArgs<T> *hap[2];
+  const ShellPiece *epc, *hpc;
+  const ShellWork *ewk, *hwk;
+  LeapArgs<T> la[2];
+  FixArgs<T> fx[2];
+  dim3 lgrid, lblk;
+  MurArgs<T> ma[2];
+  dim3 murGrid;
+  DebyeArgs<T> da[2];
+  SibcEArgs<T> se[2];
+  SibcHArgs<T> sh[2];
+  CorrArgs<T> ce[2][2];                // [set][E, H]
+  AuxArgs<T> xa;
+  bool anyAux;
+  ObsArgs<T> oa[2];
+  int maxnf;
+  VolObsArgs<T> va[2];
+  unsigned long long *stepp;           // device step counter
+};
+
+template <typename T> void fill_step_args(Ctx *c, StepArgs<T> &A, const VisitRun *chk)
 {
-  const double pi = 4.0 * atan(1.0);
   const double dt = c->grid.dt;
-  timeKernels = timeKernels && !c->evE.empty();
   // argument sets by the field set a step reads (index 0/1)
-  EArgs<T> ea[2] = {};
-  HArgs<T> ha[2] = {};
+  auto &ea = A.ea;
+  auto &ha = A.ha;
   for (int q = 0; q < 2; ++q) fill_volume_args<T>(c, ea[q], ha[q], q);
   // Fast kernels: 64x4 tiles (64 x ty) over the fast boxes, marching z in chunks of 4
   // planes (measured best from 272^3 to 386x386x258; 16k blocks and more at those sizes).
   // rows per block (tuning override VGPU_FAST_TY: 1..8)
   int ty = 4;
   if (const char *ev = getenv("VGPU_FAST_TY")) ty = std::max(1, std::min(atoi(ev), 8));
-  const dim3 vblk(64, ty, 1);
+  A.vblk = dim3(64, ty, 1);
   auto fast_grid = [&](const int *fb, int &kchunk) {
     if (fb[0] > fb[1] || fb[2] > fb[3] || fb[4] > fb[5]) return dim3(0, 0, 0);
     const int gx = (fb[1] - fb[0] + 64) / 64, gy = (fb[3] - fb[2] + ty) / ty, nz = fb[5] - fb[4] + 1;
@@ -428,14 +450,14 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
       kchunk = std::max(1, std::min(atoi(ev), nz));
     return dim3(gx, gy, (nz + kchunk - 1) / kchunk);
   };
-  FastE<T> fe[2] = {};
-  FastH<T> fh[2] = {};
+  auto &fe = A.fe;
+  auto &fh = A.fh;
   int ekc = 1, hkc = 1;
-  const dim3 egrid = fast_grid(c->fastE, ekc), hgrid = fast_grid(c->fastH, hkc);
+  const dim3 egrid = A.egrid = fast_grid(c->fastE, ekc), hgrid = A.hgrid = fast_grid(c->fastH, hkc);
   // 1-D launches padded to a multiple of eight blocks (XCD order, xcd_block)
   auto pad8 = [](unsigned long long n) { return dim3((unsigned)((n + 7) / 8 * 8)); };
-  const dim3 eg1 = pad8((unsigned long long)egrid.x * egrid.y * egrid.z);
-  const dim3 hg1 = pad8((unsigned long long)hgrid.x * hgrid.y * hgrid.z);
+  A.eg1 = pad8((unsigned long long)egrid.x * egrid.y * egrid.z);
+  A.hg1 = pad8((unsigned long long)hgrid.x * hgrid.y * hgrid.z);
   for (int q = 0; q < 2; ++q) {
     FastE<T> &e = fe[q];
     FastH<T> &h = fh[q];
@@ -456,14 +478,243 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
     if (chk)
       for (int d = 0; d < 3; ++d) { e.vis[d] = chk->vis[d]; h.vis[d] = chk->vis[3 + d]; }
   }
-  const EArgs<T> *eap[2];
-  const HArgs<T> *hap[2];
+  auto &eap = A.eap;
+  auto &hap = A.hap;
   for (int q = 0; q < 2; ++q) {
     eap[q] = P<EArgs<T>>(chk ? chk->eargs : c->d_eargs[q]);
     hap[q] = P<HArgs<T>>(chk ? chk->hargs : c->d_hargs[q]);
   }
-  const ShellPiece *epc = P<ShellPiece>(c->d_epcs), *hpc = P<ShellPiece>(c->d_hpcs);
-  const ShellWork *ewk = P<ShellWork>(c->d_ework), *hwk = P<ShellWork>(c->d_hwork);
+  A.epc = P<ShellPiece>(c->d_epcs);
+  A.hpc = P<ShellPiece>(c->d_hpcs);
+  A.ewk = P<ShellWork>(c->d_ework);
+  A.hwk = P<ShellWork>(c->d_hwork);
+  // fused leapfrog sweep: 4 waves of R rows per block, 63-cell x tiles, kz-plane z chunks
+  auto &la = A.la;
+  auto &fx = A.fx;
+  dim3 &lgrid = A.lgrid;
+  lgrid = dim3(0);
+  A.lblk = dim3(64, 4, 1);
+  if (c->leap) {
+    const int *fb = c->fastE;
+    if (fb[0] <= fb[1] && fb[2] <= fb[3] && fb[4] <= fb[5]) {
+      const int R = c->leapR, kz = std::min(c->leapKZ, fb[5] - fb[4] + 1);
+      const int ntx = (fb[1] - fb[0] + 63) / 63, ngy = (fb[3] - fb[2] + 4 * R) / (4 * R), ntz = (fb[5] - fb[4] + kz) / kz;
+      lgrid = pad8((unsigned long long)ntx * ngy * ntz);
+      for (int q = 0; q < 2; ++q) {
+        LeapArgs<T> &l = la[q];
+        l.NX = c->NX; l.NY = c->NY; l.NZ = c->NZ; l.PX = c->PX; l.plane = c->plane;
+        for (int f = 0; f < 6; ++f) l.box[f] = fb[f];
+        l.ntx = ntx; l.ngy = ngy; l.ntz = ntz; l.kz = kz;
+        for (int d = 0; d < 3; ++d) {
+          l.E[d] = ea[q].E[d]; l.Ein[d] = ea[q].Ein[d]; l.H[d] = ha[q].H[d]; l.Hin[d] = ha[q].Hin[d];
+          l.idh[d] = ea[q].idh[d]; l.ide[d] = ha[q].ide[d];
+          l.alpha[d] = ea[q].alpha[d]; l.beta[d] = ea[q].beta[d]; l.gamma[d] = ha[q].gamma[d];
+        }
+        l.ecls = ea[q].ecls; l.etab = ea[q].etab; l.hcls = ha[q].hcls; l.htab = ha[q].htab;
+        l.discard = (int)c->ncell;
+        if (chk)
+          for (int d = 0; d < 6; ++d) l.vis[d] = chk->vis[d];
+        FixArgs<T> &x = fx[q];
+        x.n = c->nFix; x.ent = P<FixEnt>(c->d_fix);
+        x.NY = c->NY; x.PX = c->PX; x.plane = c->plane;
+        for (int d = 0; d < 3; ++d) {
+          x.H[d] = ha[q].H[d]; x.Hin[d] = ha[q].Hin[d]; x.E[d] = ha[q].E[d]; x.ide[d] = ha[q].ide[d];
+          x.gamma[d] = ha[q].gamma[d];
+        }
+        x.hcls = ha[q].hcls; x.htab = ha[q].htab;
+      }
+    }
+  }
+  auto &ma = A.ma;
+  int murmax1 = 1, murmax2 = 1;
+  if (c->murmask) {
+    for (int q = 0; q < 2; ++q) {
+      MurArgs<T> &m = ma[q];
+      m.NX = c->NX; m.NY = c->NY; m.PX = c->PX; m.plane = c->plane;
+      for (int d = 0; d < 3; ++d) {
+        m.E[d] = ea[q].E[d]; m.Ein[d] = ea[q].Ein[d]; m.H[d] = ea[q].H[d];
+        m.alpha[d] = P<T>(c->alpha[d]); m.beta[d] = P<T>(c->beta[d]); m.idh[d] = P<T>(c->idh[d]);
+      }
+      if (c->indexed) { m.ecls = ea[q].ecls; m.etab = ea[q].etab; }
+      if (chk)
+        for (int d = 0; d < 3; ++d) m.vis[d] = chk->vis[d];
+    }
+    static const int order[6][2] = {{1, 2}, {1, 2}, {0, 2}, {0, 2}, {1, 0}, {1, 0}};
+    for (int f = 0; f < 6; ++f) {
+      for (int q = 0; q < 2; ++q) ma[q].zeta[f] = T(c->zeta[f]);
+      for (int u = 0; u < 2; ++u) {
+        const int slot = 2 * f + u;
+        if ((c->murmask >> f) & 1u) {
+          const int comp = order[f][u];
+          for (int q = 0; q < 2; ++q) {
+            ma[q].comp[slot] = comp;
+            for (int e = 0; e < 6; ++e) ma[q].box[slot][e] = c->fmlim[f][comp][e];
+          }
+          const int ax = f >> 1, a1 = ax == 0 ? 1 : 0, a2 = ax == 2 ? 1 : 2;
+          const int *b = ma[0].box[slot];
+          murmax1 = std::max(murmax1, b[2 * a1 + 1] - b[2 * a1] + 1);
+          murmax2 = std::max(murmax2, b[2 * a2 + 1] - b[2 * a2] + 1);
+        } else {
+          for (int q = 0; q < 2; ++q) ma[q].comp[slot] = -1;
+        }
+      }
+    }
+  }
+  A.murGrid = dim3((murmax1 + 15) / 16, (murmax2 + 15) / 16, 12);
+
+  auto &da = A.da;
+  if (c->debyeSites) {
+    for (int q = 0; q < 2; ++q) {
+      DebyeArgs<T> &a = da[q];
+      a.key = P<long long>(c->d_dkey); a.sbeta = P<T>(c->d_dsbeta);
+      for (int d = 0; d < 3; ++d) a.E[d] = ea[q].E[d];
+      a.lastE = P<T>(c->d_dlast); a.J = P<T>(c->d_dJ); a.stride = c->debyeSites;
+      a.dalpha = P<double>(c->d_dalpha); a.dbeta = P<double>(c->d_dbeta); a.npoles = P<int>(c->d_dnp);
+      a.maxPoles = c->debyeMaxPoles; a.cplx = c->debyeCplx;
+    }
+  }
+
+  auto &se = A.se;
+  auto &sh = A.sh;
+  if (c->sibcFaces) {
+    for (int q = 0; q < 2; ++q) {
+      SibcEArgs<T> &e = se[q];
+      SibcHArgs<T> &h = sh[q];
+      e.nfaces = c->sibcFaces; e.ijk = P<int>(c->d_sijk); e.surf = P<int>(c->d_ssurf);
+      e.blocks = P<SibcBlock>(c->d_sblocks);
+      e.adjA = P<unsigned char>(c->d_sadjA); e.adjB = P<unsigned char>(c->d_sadjB);
+      e.normal = P<int>(c->d_snormal); e.A = P<double>(c->d_sA); e.B = P<double>(c->d_sB);
+      e.filtset = P<int>(c->d_sset); e.npoles = P<int>(c->d_snp); e.asymp = P<double>(c->d_sasymp);
+      e.q = P<double>(c->d_sq); e.maxPoles = c->sibcMaxPoles; e.zeta = P<double>(c->d_szeta);
+      e.real = c->sibcReal ? 1 : 0;
+      e.factored = c->sibcFactored ? 1 : 0;
+      e.fbase = P<int>(c->d_sfbase); e.fratio = P<double>(c->d_sfratio);
+      e.old = P<double>(c->d_sold); e.Etan = P<T>(c->d_setan);
+      // SIBC E runs after the E sweep: it zeroes the new tangential E and
+      // reads the old H (sibc.c:444-586)
+      for (int d = 0; d < 3; ++d) { e.E[d] = ea[q].E[d]; e.H[d] = ea[q].H[d]; }
+      e.NY = c->NY; e.PX = c->PX; e.plane = c->plane;
+      h.nrows = c->sibcHRows; h.rowkey = P<long long>(c->d_shkey);
+      h.rowstart = P<int>(c->d_shstart); h.rowg = P<T>(c->d_shg);
+      h.ent = P<SibcHEnt>(c->d_shent); h.Etan = P<T>(c->d_setan);
+      for (int d = 0; d < 3; ++d) { h.H[d] = ha[q].H[d]; h.ide[d] = P<T>(c->ide[d]); }
+    }
+  }
+
+  auto &ce = A.ce;
+  for (int q = 0; q < 2; ++q)
+    for (int e = 0; e < 2; ++e) {
+      CorrArgs<T> &a = ce[q][e];
+      a = CorrArgs<T>{};
+      a.nrows = c->corrRows[e];
+      a.rowkey = P<long long>(c->d_ckey[e]);
+      a.rowstart = P<int>(c->d_cstart[e]); a.ent = P<CorrEnt>(c->d_cent[e]);
+      a.rowcoef = P<T>(c->d_ccoef[e]); a.rowalpha = P<T>(c->d_calpha[e]);
+      a.electric = e == 0;
+      a.wf = P<WaveformDev>(c->d_wf); a.src = P<SourceDev>(c->d_src); a.pw = P<PwDev>(c->d_pw);
+      a.aux = P<T>(c->d_aux); a.auxmax = c->auxmax; a.eta0 = c->grid.eta0;
+      // the E list corrects the new E, the H list the new H
+      for (int d = 0; d < 3; ++d) a.F[d] = e == 0 ? ea[q].E[d] : ha[q].H[d];
+    }
+
+  auto &xa = A.xa;
+  bool &anyAux = A.anyAux;
+  anyAux = false;
+  for (auto &p : c->pw) anyAux |= p.d.use_aux != 0;
+  if (anyAux) {
+    xa.npw = (int)c->pw.size(); xa.pw = P<PwDev>(c->d_pw); xa.aux = P<T>(c->d_aux);
+    xa.pml = P<T>(c->d_auxpml); xa.prof = P<T>(c->d_auxprof); xa.auxmax = c->auxmax;
+    xa.wf = P<WaveformDev>(c->d_wf);
+  }
+
+  // observers sample the fields a step starts from: the set it reads
+  auto &oa = A.oa;
+  int &maxnf = A.maxnf;
+  maxnf = 1;
+  if (!c->pobs.empty()) {
+    for (int q = 0; q < 2; ++q) {
+      ObsArgs<T> &o = oa[q];
+      o.nobs = (int)c->pobs.size(); o.cell = P<long long>(c->d_ocell);
+      o.first = P<unsigned long long>(c->d_ofirst); o.last = P<unsigned long long>(c->d_olast);
+      o.nfreq = P<int>(c->d_onf); o.dftoff = P<long long>(c->d_ooff); o.omega = P<double>(c->d_oomega);
+      o.td = P<double>(c->d_otd); o.tdsteps = c->totalSteps; o.dre = P<double>(c->d_odre);
+      o.dim = P<double>(c->d_odim);
+      for (int d = 0; d < 6; ++d) o.F[d] = P<T>(c->fld(q, d));
+    }
+    for (auto &o : c->pobs) maxnf = std::max(maxnf, (int)o.omega.size());
+  }
+  auto &va = A.va;
+  if (c->vnodes) {
+    for (int q = 0; q < 2; ++q) {
+      VolObsArgs<T> &v = va[q];
+      v.nodecell = P<long long>(c->d_vcell); v.nodeobs = P<int>(c->d_vobs); v.nnodes = c->vnodes;
+      v.first = P<unsigned long long>(c->d_vfirst); v.last = P<unsigned long long>(c->d_vlast);
+      v.stage = P<float>(c->d_vstage);
+      v.ring = c->vring;
+      for (int d = 0; d < 6; ++d) v.F[d] = P<T>(c->fld(q, d));
+    }
+  }
+
+  // Time-dependent kernels read the step number from the device counter
+  // (step n = counter + offset within this run), so the same launches can be
+  // captured once in a graph and replayed for later runs.
+  unsigned long long *stepp = A.stepp = P<unsigned long long>(c->d_step);
+  for (int q = 0; q < 2; ++q) {
+    oa[q].step = stepp; oa[q].dt = dt;
+    va[q].step = stepp;
+    for (int e = 0; e < 2; ++e) { ce[q][e].step = stepp; ce[q][e].dt = dt; ce[q][e].half = e == 0 ? 0.0 : 0.5; }
+  }
+  xa.step = stepp; xa.dt = dt;
+}
+
+// Ghost planes of one half-step (surface.c:569-812), copied within the field
+// set the step writes.
+template <typename T> void launch_ghosts(Ctx *c, int electric, int q, hipStream_t s)
+{
+  for (int f = 0; f < 6; ++f) {
+    const int t = c->ghost[f];
+    if (!t) continue;
+    const int a = f >> 1;
+    const int inner = (f & 1) ? c->grid.gobox[f] - 1 : c->grid.gobox[f];
+    const int opp = (f & 1) ? c->grid.gobox[f - 1] : c->grid.gobox[f + 1] - 1;
+    const int dst = (f & 1) ? c->grid.gobox[f] : c->grid.gobox[f] - 1;
+    GhostArgs<T> g{};
+    g.NX = c->NX; g.NY = c->NY; g.NZ = c->NZ; g.PX = c->PX; g.plane = c->plane;
+    g.axis = a; g.dst = dst;
+    for (int d = 0; d < 3; ++d) { g.lo[d] = c->grid.gobox[2 * d] - 1; g.hi[d] = c->grid.gobox[2 * d + 1]; }
+    const int a1 = a == 0 ? 1 : 0, a2 = a == 2 ? 1 : 2;
+    const dim3 gb(16, 16), gg((g.hi[a1] - g.lo[a1] + 16) / 16, (g.hi[a2] - g.lo[a2] + 16) / 16);
+    int comps[2], nc;
+    if (electric) { comps[0] = a; nc = 1; }
+    else { comps[0] = 3 + (a + 1) % 3; comps[1] = 3 + (a + 2) % 3; nc = 2; }
+    // E: PEC copy, PMC negate, periodic copy from the opposite side; H: PMC negate, periodic, PEC copy.
+    g.src = t == 3 ? opp : inner;
+    g.sign = (t == 2) ? T(-1) : T(1);
+    for (int u = 0; u < nc; ++u) {
+      g.F = P<T>(c->fld(c->next(q), comps[u]));
+      hipLaunchKernelGGL(k_ghost<T>, gg, gb, 0, s, g);
+    }
+  }
+}
+
+// The loop (simulation.c:136-230).  With chk, one step of the field sweeps
+// only (Mur E, the E sweep, the H sweep), in their update-counting form.
+// With the fused sweep, step q of the run reads field set c->par ^ (q & 1)
+// and writes the other, and the caller flips c->par by the parity of nsteps
+// once the run is enqueued; the separate sweeps read and write set 0.
+template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool timeKernels, const VisitRun *chk)
+{
+  const double pi = 4.0 * atan(1.0);
+  timeKernels = timeKernels && !c->evE.empty();
+  StepArgs<T> A{};
+  fill_step_args<T>(c, A, chk);
+  const auto &fe = A.fe;
+  const auto &fh = A.fh;
+  const dim3 egrid = A.egrid, hgrid = A.hgrid, eg1 = A.eg1, hg1 = A.hg1, vblk = A.vblk;
+  const auto &eap = A.eap;
+  const auto &hap = A.hap;
+  const ShellPiece *epc = A.epc, *hpc = A.hpc;
+  const ShellWork *ewk = A.ewk, *hwk = A.hwk;
   auto fast_e = [&](int q) {
     const FastE<T> &f = fe[q];
     if (egrid.x && chk) {
@@ -508,42 +759,9 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
     }
   };
 
-  // fused leapfrog sweep: 4 waves of R rows per block, 63-cell x tiles, kz-plane z chunks
-  LeapArgs<T> la[2] = {};
-  FixArgs<T> fx[2] = {};
-  dim3 lgrid(0);
-  const dim3 lblk(64, 4, 1);
-  if (c->leap) {
-    const int *fb = c->fastE;
-    if (fb[0] <= fb[1] && fb[2] <= fb[3] && fb[4] <= fb[5]) {
-      const int R = c->leapR, kz = std::min(c->leapKZ, fb[5] - fb[4] + 1);
-      const int ntx = (fb[1] - fb[0] + 63) / 63, ngy = (fb[3] - fb[2] + 4 * R) / (4 * R), ntz = (fb[5] - fb[4] + kz) / kz;
-      lgrid = pad8((unsigned long long)ntx * ngy * ntz);
-      for (int q = 0; q < 2; ++q) {
-        LeapArgs<T> &l = la[q];
-        l.NX = c->NX; l.NY = c->NY; l.NZ = c->NZ; l.PX = c->PX; l.plane = c->plane;
-        for (int f = 0; f < 6; ++f) l.box[f] = fb[f];
-        l.ntx = ntx; l.ngy = ngy; l.ntz = ntz; l.kz = kz;
-        for (int d = 0; d < 3; ++d) {
-          l.E[d] = ea[q].E[d]; l.Ein[d] = ea[q].Ein[d]; l.H[d] = ha[q].H[d]; l.Hin[d] = ha[q].Hin[d];
-          l.idh[d] = ea[q].idh[d]; l.ide[d] = ha[q].ide[d];
-          l.alpha[d] = ea[q].alpha[d]; l.beta[d] = ea[q].beta[d]; l.gamma[d] = ha[q].gamma[d];
-        }
-        l.ecls = ea[q].ecls; l.etab = ea[q].etab; l.hcls = ha[q].hcls; l.htab = ha[q].htab;
-        l.discard = (int)c->ncell;
-        if (chk)
-          for (int d = 0; d < 6; ++d) l.vis[d] = chk->vis[d];
-        FixArgs<T> &x = fx[q];
-        x.n = c->nFix; x.ent = P<FixEnt>(c->d_fix);
-        x.NY = c->NY; x.PX = c->PX; x.plane = c->plane;
-        for (int d = 0; d < 3; ++d) {
-          x.H[d] = ha[q].H[d]; x.Hin[d] = ha[q].Hin[d]; x.E[d] = ha[q].E[d]; x.ide[d] = ha[q].ide[d];
-          x.gamma[d] = ha[q].gamma[d];
-        }
-        x.hcls = ha[q].hcls; x.htab = ha[q].htab;
-      }
-    }
-  }
+  const auto &la = A.la;
+  const auto &fx = A.fx;
+  const dim3 lgrid = A.lgrid, lblk = A.lblk;
   auto leap = [&](int q) {
     if (!lgrid.x) return;
     const LeapArgs<T> &l = la[q];
@@ -575,43 +793,8 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
     else hipLaunchKernelGGL((k_h_fix<T, false>), g, lblk, 0, s, fx[q]);
   };
 
-  MurArgs<T> ma[2] = {};
-  int murmax1 = 1, murmax2 = 1;
-  if (c->murmask) {
-    for (int q = 0; q < 2; ++q) {
-      MurArgs<T> &m = ma[q];
-      m.NX = c->NX; m.NY = c->NY; m.PX = c->PX; m.plane = c->plane;
-      for (int d = 0; d < 3; ++d) {
-        m.E[d] = ea[q].E[d]; m.Ein[d] = ea[q].Ein[d]; m.H[d] = ea[q].H[d];
-        m.alpha[d] = P<T>(c->alpha[d]); m.beta[d] = P<T>(c->beta[d]); m.idh[d] = P<T>(c->idh[d]);
-      }
-      if (c->indexed) { m.ecls = ea[q].ecls; m.etab = ea[q].etab; }
-      if (chk)
-        for (int d = 0; d < 3; ++d) m.vis[d] = chk->vis[d];
-    }
-    static const int order[6][2] = {{1, 2}, {1, 2}, {0, 2}, {0, 2}, {1, 0}, {1, 0}};
-    for (int f = 0; f < 6; ++f) {
-      for (int q = 0; q < 2; ++q) ma[q].zeta[f] = T(c->zeta[f]);
-      for (int u = 0; u < 2; ++u) {
-        const int slot = 2 * f + u;
-        if ((c->murmask >> f) & 1u) {
-          const int comp = order[f][u];
-          for (int q = 0; q < 2; ++q) {
-            ma[q].comp[slot] = comp;
-            for (int e = 0; e < 6; ++e) ma[q].box[slot][e] = c->fmlim[f][comp][e];
-          }
-          const int ax = f >> 1, a1 = ax == 0 ? 1 : 0, a2 = ax == 2 ? 1 : 2;
-          const int *b = ma[0].box[slot];
-          murmax1 = std::max(murmax1, b[2 * a1 + 1] - b[2 * a1] + 1);
-          murmax2 = std::max(murmax2, b[2 * a2 + 1] - b[2 * a2] + 1);
-        } else {
-          for (int q = 0; q < 2; ++q) ma[q].comp[slot] = -1;
-        }
-      }
-    }
-  }
-  const dim3 murGrid((murmax1 + 15) / 16, (murmax2 + 15) / 16, 12);
-
+  const auto &ma = A.ma;
+  const dim3 murGrid = A.murGrid;
   if (chk) {
     const int q = c->par;
     if (c->murmask) hipLaunchKernelGGL((k_mur_e<T, true>), murGrid, dim3(16, 16), 0, s, ma[q]);
@@ -630,135 +813,17 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
     return VGPU_OK;
   }
 
-  DebyeArgs<T> da[2] = {};
-  if (c->debyeSites) {
-    for (int q = 0; q < 2; ++q) {
-      DebyeArgs<T> &a = da[q];
-      a.key = P<long long>(c->d_dkey); a.sbeta = P<T>(c->d_dsbeta);
-      for (int d = 0; d < 3; ++d) a.E[d] = ea[q].E[d];
-      a.lastE = P<T>(c->d_dlast); a.J = P<T>(c->d_dJ); a.stride = c->debyeSites;
-      a.dalpha = P<double>(c->d_dalpha); a.dbeta = P<double>(c->d_dbeta); a.npoles = P<int>(c->d_dnp);
-      a.maxPoles = c->debyeMaxPoles; a.cplx = c->debyeCplx;
-    }
-  }
-
-  SibcEArgs<T> se[2] = {};
-  SibcHArgs<T> sh[2] = {};
-  if (c->sibcFaces) {
-    for (int q = 0; q < 2; ++q) {
-      SibcEArgs<T> &e = se[q];
-      SibcHArgs<T> &h = sh[q];
-      e.nfaces = c->sibcFaces; e.ijk = P<int>(c->d_sijk); e.surf = P<int>(c->d_ssurf);
-      e.blocks = P<SibcBlock>(c->d_sblocks);
-      e.adjA = P<unsigned char>(c->d_sadjA); e.adjB = P<unsigned char>(c->d_sadjB);
-      e.normal = P<int>(c->d_snormal); e.A = P<double>(c->d_sA); e.B = P<double>(c->d_sB);
-      e.filtset = P<int>(c->d_sset); e.npoles = P<int>(c->d_snp); e.asymp = P<double>(c->d_sasymp);
-      e.q = P<double>(c->d_sq); e.maxPoles = c->sibcMaxPoles; e.zeta = P<double>(c->d_szeta);
-      e.real = c->sibcReal ? 1 : 0;
-      e.factored = c->sibcFactored ? 1 : 0;
-      e.fbase = P<int>(c->d_sfbase); e.fratio = P<double>(c->d_sfratio);
-      e.old = P<double>(c->d_sold); e.Etan = P<T>(c->d_setan);
-      // SIBC E runs after the E sweep: it zeroes the new tangential E and
-      // reads the old H (sibc.c:444-586)
-      for (int d = 0; d < 3; ++d) { e.E[d] = ea[q].E[d]; e.H[d] = ea[q].H[d]; }
-      e.NY = c->NY; e.PX = c->PX; e.plane = c->plane;
-      h.nrows = c->sibcHRows; h.rowkey = P<long long>(c->d_shkey);
-      h.rowstart = P<int>(c->d_shstart); h.rowg = P<T>(c->d_shg);
-      h.ent = P<SibcHEnt>(c->d_shent); h.Etan = P<T>(c->d_setan);
-      for (int d = 0; d < 3; ++d) { h.H[d] = ha[q].H[d]; h.ide[d] = P<T>(c->ide[d]); }
-    }
-  }
-
-  CorrArgs<T> ce[2][2];
-  for (int q = 0; q < 2; ++q)
-    for (int e = 0; e < 2; ++e) {
-      CorrArgs<T> &a = ce[q][e];
-      a = CorrArgs<T>{};
-      a.nrows = c->corrRows[e];
-      a.rowkey = P<long long>(c->d_ckey[e]);
-      a.rowstart = P<int>(c->d_cstart[e]); a.ent = P<CorrEnt>(c->d_cent[e]);
-      a.rowcoef = P<T>(c->d_ccoef[e]); a.rowalpha = P<T>(c->d_calpha[e]);
-      a.electric = e == 0;
-      a.wf = P<WaveformDev>(c->d_wf); a.src = P<SourceDev>(c->d_src); a.pw = P<PwDev>(c->d_pw);
-      a.aux = P<T>(c->d_aux); a.auxmax = c->auxmax; a.eta0 = c->grid.eta0;
-      // the E list corrects the new E, the H list the new H
-      for (int d = 0; d < 3; ++d) a.F[d] = e == 0 ? ea[q].E[d] : ha[q].H[d];
-    }
-
-  AuxArgs<T> xa{};
-  bool anyAux = false;
-  for (auto &p : c->pw) anyAux |= p.d.use_aux != 0;
-  if (anyAux) {
-    xa.npw = (int)c->pw.size(); xa.pw = P<PwDev>(c->d_pw); xa.aux = P<T>(c->d_aux);
-    xa.pml = P<T>(c->d_auxpml); xa.prof = P<T>(c->d_auxprof); xa.auxmax = c->auxmax;
-    xa.wf = P<WaveformDev>(c->d_wf);
-  }
-
-  // observers sample the fields a step starts from: the set it reads
-  ObsArgs<T> oa[2] = {};
-  int maxnf = 1;
-  if (!c->pobs.empty()) {
-    for (int q = 0; q < 2; ++q) {
-      ObsArgs<T> &o = oa[q];
-      o.nobs = (int)c->pobs.size(); o.cell = P<long long>(c->d_ocell);
-      o.first = P<unsigned long long>(c->d_ofirst); o.last = P<unsigned long long>(c->d_olast);
-      o.nfreq = P<int>(c->d_onf); o.dftoff = P<long long>(c->d_ooff); o.omega = P<double>(c->d_oomega);
-      o.td = P<double>(c->d_otd); o.tdsteps = c->totalSteps; o.dre = P<double>(c->d_odre);
-      o.dim = P<double>(c->d_odim);
-      for (int d = 0; d < 6; ++d) o.F[d] = P<T>(c->fld(q, d));
-    }
-    for (auto &o : c->pobs) maxnf = std::max(maxnf, (int)o.omega.size());
-  }
-  VolObsArgs<T> va[2] = {};
-  if (c->vnodes) {
-    for (int q = 0; q < 2; ++q) {
-      VolObsArgs<T> &v = va[q];
-      v.nodecell = P<long long>(c->d_vcell); v.nodeobs = P<int>(c->d_vobs); v.nnodes = c->vnodes;
-      v.first = P<unsigned long long>(c->d_vfirst); v.last = P<unsigned long long>(c->d_vlast);
-      v.stage = P<float>(c->d_vstage);
-      v.ring = c->vring;
-      for (int d = 0; d < 6; ++d) v.F[d] = P<T>(c->fld(q, d));
-    }
-  }
-
-  // ghosts copy within the new set
-  auto ghost = [&](int electric, int q) {
-    for (int f = 0; f < 6; ++f) {
-      const int t = c->ghost[f];
-      if (!t) continue;
-      const int a = f >> 1;
-      const int inner = (f & 1) ? c->grid.gobox[f] - 1 : c->grid.gobox[f];
-      const int opp = (f & 1) ? c->grid.gobox[f - 1] : c->grid.gobox[f + 1] - 1;
-      const int dst = (f & 1) ? c->grid.gobox[f] : c->grid.gobox[f] - 1;
-      GhostArgs<T> g{};
-      g.NX = c->NX; g.NY = c->NY; g.NZ = c->NZ; g.PX = c->PX; g.plane = c->plane;
-      g.axis = a; g.dst = dst;
-      for (int d = 0; d < 3; ++d) { g.lo[d] = c->grid.gobox[2 * d] - 1; g.hi[d] = c->grid.gobox[2 * d + 1]; }
-      const int a1 = a == 0 ? 1 : 0, a2 = a == 2 ? 1 : 2;
-      const dim3 gb(16, 16), gg((g.hi[a1] - g.lo[a1] + 16) / 16, (g.hi[a2] - g.lo[a2] + 16) / 16);
-      int comps[2], nc;
-      if (electric) { comps[0] = a; nc = 1; }
-      else { comps[0] = 3 + (a + 1) % 3; comps[1] = 3 + (a + 2) % 3; nc = 2; }
-      // E: PEC copy, PMC negate, periodic copy from the opposite side; H: PMC negate, periodic, PEC copy.
-      g.src = t == 3 ? opp : inner;
-      g.sign = (t == 2) ? T(-1) : T(1);
-      for (int u = 0; u < nc; ++u) {
-        g.F = P<T>(c->fld(c->next(q), comps[u]));
-        hipLaunchKernelGGL(k_ghost<T>, gg, gb, 0, s, g);
-      }
-    }
-  };
-
-  // Time-dependent kernels read the step number from the device counter
-  // (step n = counter + offset within this run), so the same launches can be
-  // captured once in a graph and replayed for later runs.
-  unsigned long long *stepp = P<unsigned long long>(c->d_step);
-  for (int q = 0; q < 2; ++q) {
-    oa[q].step = stepp; oa[q].dt = dt;
-    va[q].step = stepp;
-    for (int e = 0; e < 2; ++e) { ce[q][e].step = stepp; ce[q][e].dt = dt; ce[q][e].half = e == 0 ? 0.0 : 0.5; }
-  }
-  xa.step = stepp; xa.dt = dt;
+  auto &da = A.da;
+  const auto &se = A.se;
+  const auto &sh = A.sh;
+  auto &ce = A.ce;
+  auto &xa = A.xa;
+  const bool anyAux = A.anyAux;
+  auto &oa = A.oa;
+  const int maxnf = A.maxnf;
+  auto &va = A.va;
+  unsigned long long *stepp = A.stepp;
+  auto ghost = [&](int electric, int q) { launch_ghosts<T>(c, electric, q, s); };
   int ne = 0, nh = 0;
   for (uint64_t n = 0; n < nsteps; ++n) {
     const int soff = (int)n;
@@ -820,6 +885,147 @@ template <typename T> int run_steps(Ctx *c, uint64_t nsteps, hipStream_t s, bool
   hipLaunchKernelGGL(k_advance_step, dim3(1), dim3(64), 0, s, stepp, (unsigned long long)nsteps);
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return set_error(c, VGPU_EDEVICE, "kernel launch failed: %s", hipGetErrorString(err));
+  return VGPU_OK;
+}
+
+// A solve's argument blocks for packed launches (field set 0), uploaded once
+// after set-up.  Contexts with the fused sweep take no part in group runs.
+template <typename T> int upload_pack(Ctx *c)
+{
+  StepArgs<T> A{};
+  fill_step_args<T>(c, A, nullptr);
+  PackArgs<T> p{};
+  p.fe = A.fe[0];
+  p.fh = A.fh[0];
+  p.epc = A.epc; p.hpc = A.hpc; p.ewk = A.ewk; p.hwk = A.hwk;
+  p.ea = A.eap[0]; p.ha = A.hap[0];
+  p.da = A.da[0];
+  p.ce = A.ce[0][0];
+  p.ch = A.ce[0][1];
+  p.oa = A.oa[0];
+  p.xa = A.xa;
+  p.va = A.va[0];
+  p.headGx = A.oa[0].nobs ? (A.maxnf + 63) / 64 : 1;
+  c->d_pack = dev_alloc(c, sizeof(p));
+  if (!c->d_pack) return set_error(c, VGPU_ENOMEM, "group-run arguments");
+  if (hipMemcpy(c->d_pack, &p, sizeof(p), hipMemcpyHostToDevice) != hipSuccess)
+    return set_error(c, VGPU_EDEVICE, "group-run arguments: %s", hipGetErrorString(hipGetLastError()));
+  return VGPU_OK;
+}
+
+// Steps of several solves in lockstep on one stream (vgpu_run_group): the
+// same kernels in the same order per solve as run_steps, with the fast and
+// shell sweeps, Debye layers, corrections and observers of all solves in one
+// packed launch each per half-step; Mur, SIBC and ghost launches stay one
+// per solve.  Solves share no memory, so only each solve's own order
+// matters, and every block does what it does in the solve's own launch.
+// Every context uses the separate sweeps (set 0) and has its PackArgs.
+template <typename T> int run_group_steps(Ctx *const *cs, int n, uint64_t nsteps, hipStream_t s)
+{
+  const double pi = 4.0 * atan(1.0);
+  std::vector<StepArgs<T>> A(n);
+  for (int i = 0; i < n; ++i) fill_step_args<T>(cs[i], A[i], nullptr);
+  // One packed launch: the solves with a positive block count, in order.
+  auto plan = [&](auto blocks) {
+    PackMap m{};
+    unsigned tot = 0;
+    for (int i = 0; i < n; ++i) {
+      long long base = 0, cnt = 0;
+      const long long k = blocks(i, base, cnt);
+      if (k <= 0) continue;
+      m.args[m.n] = cs[i]->d_pack;
+      m.base[m.n] = base;
+      m.cnt[m.n] = cnt;
+      m.start[m.n++] = tot;
+      tot += (unsigned)k;
+    }
+    m.start[m.n] = tot;
+    return m;
+  };
+  PackMap fastE[2], fastH[2], shellE[2], shellH[2];
+  for (int x = 0; x < 2; ++x) {
+    const bool idx = x == 0;
+    fastE[x] = plan([&](int i, long long &, long long &) {
+      return cs[i]->indexed == idx && A[i].egrid.x ? (long long)A[i].eg1.x : 0LL; });
+    fastH[x] = plan([&](int i, long long &, long long &) {
+      return cs[i]->indexed == idx && A[i].hgrid.x ? (long long)A[i].hg1.x : 0LL; });
+    shellE[x] = plan([&](int i, long long &, long long &) { return cs[i]->indexed == idx ? (long long)cs[i]->nEwork : 0LL; });
+    shellH[x] = plan([&](int i, long long &, long long &) { return cs[i]->indexed == idx ? (long long)cs[i]->nHwork : 0LL; });
+  }
+  unsigned headGx = 1;
+  for (int i = 0; i < n; ++i)
+    if (A[i].oa[0].nobs) headGx = std::max(headGx, (unsigned)((A[i].maxnf + 63) / 64));
+  PackMap head = plan([&](int i, long long &, long long &) {
+    return A[i].oa[0].nobs || A[i].anyAux ? (long long)A[i].oa[0].nobs + (A[i].anyAux ? A[i].xa.npw : 0) : 0LL; });
+  PackMap vol = plan([&](int i, long long &, long long &) { return (A[i].va[0].nnodes + 255) / 256; });
+  PackMap corrE = plan([&](int i, long long &, long long &) { return (A[i].ce[0][0].nrows + 255) / 256; });
+  PackMap corrH = plan([&](int i, long long &, long long &cnt) {
+    cnt = (A[i].ce[0][1].nrows + 255) / 256;
+    return A[i].ce[0][1].nrows || A[i].anyAux ? cnt + (A[i].anyAux ? A[i].xa.npw : 0) : 0LL; });
+  // Debye launches of each solve (run_steps' order: layers 0 and 1 together,
+  // then one per layer), packed by launch number
+  std::vector<std::vector<std::pair<long long, long long>>> dl(n);
+  size_t ndl = 0;
+  for (int i = 0; i < n; ++i) {
+    const Ctx *c = cs[i];
+    for (int l = 0; l < c->debyeLayers;) {
+      const int l1 = (l == 0 && c->debyeLayers > 1) ? 2 : l + 1;
+      const long long b = c->debyeLayerStart[l], k = c->debyeLayerStart[l1] - b;
+      l = l1;
+      if (k) dl[i].push_back({b, k});
+    }
+    ndl = std::max(ndl, dl[i].size());
+  }
+  std::vector<PackMap> debye(ndl);
+  for (size_t j = 0; j < ndl; ++j)
+    debye[j] = plan([&](int i, long long &base, long long &cnt) {
+      if (j >= dl[i].size()) return 0LL;
+      base = dl[i][j].first;
+      cnt = dl[i][j].second;
+      return (cnt + 255) / 256; });
+  const dim3 vblk = A[0].vblk;
+  for (uint64_t st = 0; st < nsteps; ++st) {
+    const int soff = (int)st;
+    if (head.n) {
+      head.soff = soff;
+      hipLaunchKernelGGL(k_head_pk<T>, dim3(headGx, head.start[head.n]), dim3(64), 0, s, head, pi);
+    }
+    if (vol.n) {
+      vol.soff = soff;
+      hipLaunchKernelGGL(k_observe_volume_pk<T>, dim3(vol.start[vol.n]), dim3(256), 0, s, vol);
+    }
+    for (int i = 0; i < n; ++i)
+      if (cs[i]->murmask) hipLaunchKernelGGL(k_mur_e<T>, A[i].murGrid, dim3(16, 16), 0, s, A[i].ma[0]);
+    if (fastE[0].n) hipLaunchKernelGGL((k_e_fast_pk<T, true>), dim3(fastE[0].start[fastE[0].n]), vblk, 0, s, fastE[0]);
+    if (fastE[1].n) hipLaunchKernelGGL((k_e_fast_pk<T, false>), dim3(fastE[1].start[fastE[1].n]), vblk, 0, s, fastE[1]);
+    if (shellE[0].n) hipLaunchKernelGGL((k_e_shell_pk<T, true>), dim3(shellE[0].start[shellE[0].n]), dim3(256), 0, s, shellE[0]);
+    if (shellE[1].n) hipLaunchKernelGGL((k_e_shell_pk<T, false>), dim3(shellE[1].start[shellE[1].n]), dim3(256), 0, s, shellE[1]);
+    for (const PackMap &m : debye) hipLaunchKernelGGL(k_debye_pk<T>, dim3(m.start[m.n]), dim3(256), 0, s, m);
+    for (int i = 0; i < n; ++i)
+      if (cs[i]->sibcFaces)
+        hipLaunchKernelGGL(k_sibc_e<T>, dim3((unsigned)cs[i]->sibcBlocks), dim3(128), 0, s, A[i].se[0]);
+    if (corrE.n) {
+      corrE.soff = soff;
+      hipLaunchKernelGGL(k_corr_e_pk<T>, dim3(corrE.start[corrE.n]), dim3(256), 0, s, corrE, pi);
+    }
+    for (int i = 0; i < n; ++i) launch_ghosts<T>(cs[i], 1, 0, s);
+    if (fastH[0].n) hipLaunchKernelGGL((k_h_fast_pk<T, true>), dim3(fastH[0].start[fastH[0].n]), vblk, 0, s, fastH[0]);
+    if (fastH[1].n) hipLaunchKernelGGL((k_h_fast_pk<T, false>), dim3(fastH[1].start[fastH[1].n]), vblk, 0, s, fastH[1]);
+    if (shellH[0].n) hipLaunchKernelGGL((k_h_shell_pk<T, true>), dim3(shellH[0].start[shellH[0].n]), dim3(256), 0, s, shellH[0]);
+    if (shellH[1].n) hipLaunchKernelGGL((k_h_shell_pk<T, false>), dim3(shellH[1].start[shellH[1].n]), dim3(256), 0, s, shellH[1]);
+    for (int i = 0; i < n; ++i)
+      if (cs[i]->sibcFaces)
+        hipLaunchKernelGGL(k_sibc_h<T>, dim3((unsigned)((A[i].sh[0].nrows + 255) / 256)), dim3(256), 0, s, A[i].sh[0]);
+    if (corrH.n) {
+      corrH.soff = soff;
+      hipLaunchKernelGGL(k_corr_h_pk<T>, dim3(corrH.start[corrH.n]), dim3(256), 0, s, corrH, pi);
+    }
+    for (int i = 0; i < n; ++i) launch_ghosts<T>(cs[i], 0, 0, s);
+  }
+  for (int i = 0; i < n; ++i)
+    hipLaunchKernelGGL(k_advance_step, dim3(1), dim3(64), 0, s, A[i].stepp, (unsigned long long)nsteps);
+  const hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return set_error(cs[0], VGPU_EDEVICE, "kernel launch failed: %s", hipGetErrorString(err));
   return VGPU_OK;
 }
 

@@ -7,6 +7,7 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 #include "vh_internal.h"
 #include "vh_gpu.h"
 
@@ -296,6 +297,42 @@ long vh_dsolve_enqueue(vh_dsolve *d)
   return (long)n;
 }
 
+/* Enqueue the next chunk of n solves together, in lockstep, with one packed
+ * launch per step kernel over all of them (vgpu_run_group): each advances by
+ * the steps the member with the fewest left still has, at most the chunk.
+ * Each member's chunk is then completed on its own (vh_dsolve_complete), as
+ * after vh_dsolve_enqueue.  Returns the steps enqueued (0 when a member has
+ * two chunks in flight or no steps left), <0 on error. */
+long vh_dsolve_enqueue_group(vh_dsolve **ds, int n)
+{
+  if (n < 1 || n > VGPU_GROUP_MAX) return -1;
+  vgpu_ctx *cs[VGPU_GROUP_MAX];
+  uint64_t first[VGPU_GROUP_MAX];
+  unsigned long k = 0;
+  for (int i = 0; i < n; i++) {
+    vh_dsolve *d = ds[i];
+    if (d->qlen >= 2 || d->enq >= d->NT) return 0;
+    const unsigned long left = d->NT - d->enq < d->chunk ? d->NT - d->enq : d->chunk;
+    if (i == 0 || left < k) k = left;
+    cs[i] = d->c;
+    first[i] = d->enq;
+  }
+  int rc = vgpu_run_group(cs, n, first, k);
+  if (rc) {
+    for (int i = 0; i < n; i++)  /* the member the device layer named */
+      if (vgpu_last_error(cs[i])[0]) return fail(ds[i]->s, cs[i], "vgpu_run_group", rc);
+    return fail(ds[0]->s, cs[0], "vgpu_run_group", rc);
+  }
+  for (int i = 0; i < n; i++) {
+    vh_dsolve *d = ds[i];
+    d->qfirst[d->qlen] = d->enq;
+    d->qn[d->qlen] = k;
+    d->qlen++;
+    d->enq += k;
+  }
+  return (long)k;
+}
+
 /* 1 when the oldest chunk in flight has finished (or none is in flight), 0
  * while it runs, <0 on a device error. */
 int vh_dsolve_ready(vh_dsolve *d)
@@ -403,6 +440,48 @@ int vh_gpu_run(vh_solve *s, int prec, int device, unsigned long chunk, double *l
   return 0;
 }
 
+/* Run n set-up solves to their ends together, packed (vh_dsolve_enqueue_group),
+ * and write each one's observer files; a solve leaves the group when all its
+ * steps are enqueued.  loop_ms receives the host's wall time of the loop, from
+ * the first enqueue to the last completion.  On error the message is in the
+ * failing solve's err. */
+int vh_gpu_run_group(vh_solve **ss, int n, int prec, int device, unsigned long chunk, double *loop_ms)
+{
+  if (n < 1 || n > VGPU_GROUP_MAX) return -1;
+  vh_dsolve *ds[VGPU_GROUP_MAX] = {0};
+  int rc = 0;
+  for (int i = 0; i < n && !rc; i++)
+    if (!(ds[i] = vh_dsolve_open(ss[i], prec, device, chunk))) rc = -1;
+  struct timespec t0, t1;
+  clock_gettime(CLOCK_MONOTONIC, &t0);
+  while (!rc) {
+    for (;;) {
+      vh_dsolve *live[VGPU_GROUP_MAX];
+      int m = 0;
+      for (int i = 0; i < n; i++)
+        if (ds[i]->enq < ds[i]->NT) live[m++] = ds[i];
+      if (!m) break;
+      const long e = vh_dsolve_enqueue_group(live, m);
+      if (e < 0) rc = -1;
+      if (e <= 0) break;
+    }
+    for (int i = 0; i < n && !rc; i++)
+      if (ds[i]->qlen && vh_dsolve_complete(ds[i])) rc = -1;
+    int all = 1;
+    for (int i = 0; i < n; i++) all = all && vh_dsolve_done(ds[i]);
+    if (all) break;
+  }
+  clock_gettime(CLOCK_MONOTONIC, &t1);
+  if (loop_ms) *loop_ms = (t1.tv_sec - t0.tv_sec) * 1e3 + (t1.tv_nsec - t0.tv_nsec) * 1e-6;
+  for (int i = 0; i < n && !rc; i++) {
+    double **td = NULL;
+    vh_dsolve_collect(ds[i], &td);
+    vh_dsolve_write(ds[i], td);
+  }
+  for (int i = 0; i < n; i++) vh_dsolve_close(ds[i]);
+  return rc;
+}
+
 /* Parse, set up, run on the device and write observer files: the whole
  * solver behind one call (the reference's main(), vulture.c:77-197, with the
  * time loop on the GPU).  Returns 0 on success. */
@@ -420,6 +499,43 @@ int vh_run_mesh_gpu(const char *mesh, const char *outdir, int prec, int device, 
   if (!rc) rc = vh_gpu_run(s, prec, device, chunk, loop_ms);
   if (rc && err) snprintf(err, (size_t)errlen, "%s", vh_error(s));
   vh_destroy(s);
+  return rc;
+}
+
+/* vh_run_mesh_gpu for n meshes run together as one packed group
+ * (vh_gpu_run_group).  err receives the first failing mesh's message. */
+int vh_run_meshes_gpu_group(const char *const *meshes, const char *const *outdirs, int n, int prec, int device,
+                            int fullPrecision, unsigned long chunk, double *loop_ms, char *err, int errlen)
+{
+  if (n < 1 || n > VGPU_GROUP_MAX) {
+    if (err) snprintf(err, (size_t)errlen, "a group holds 1 to %d solves, not %d", VGPU_GROUP_MAX, n);
+    return -1;
+  }
+  vh_solve *ss[VGPU_GROUP_MAX] = {0};
+  int rc = 0;
+  for (int i = 0; i < n && !rc; i++) {
+    vh_solve *s = ss[i] = vh_create();
+    if (!s) {
+      rc = -1;
+      break;
+    }
+    vh_set_dirs(s, NULL, outdirs[i]);
+    s->fullPrecision = fullPrecision;
+    rc = vh_read_mesh(s, meshes[i]);
+    if (!rc) rc = vh_init(s);
+    if (!rc) rc = vh_open_observers(s);
+  }
+  if (!rc) rc = vh_gpu_run_group(ss, n, prec, device, chunk, loop_ms);
+  if (rc && err) {
+    snprintf(err, (size_t)errlen, "group run failed");
+    for (int i = 0; i < n; i++)
+      if (ss[i] && vh_error(ss[i])[0]) {
+        snprintf(err, (size_t)errlen, "%s: %s", meshes[i], vh_error(ss[i]));
+        break;
+      }
+  }
+  for (int i = 0; i < n; i++)
+    if (ss[i]) vh_destroy(ss[i]);
   return rc;
 }
 

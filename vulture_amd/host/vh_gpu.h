@@ -17,9 +17,16 @@ int vh_gpu_run(vh_solve *s, int prec, int device, unsigned long chunk, double *l
 int vh_run_mesh_gpu(const char *mesh, const char *outdir, int prec, int device, int fullPrecision, long steps,
                     unsigned long chunk, double *loop_ms, char *err, int errlen);
 
+/* Several small solves run together, one packed launch per step kernel
+ * over all of them (vgpu_run_group); at most VGPU_GROUP_MAX. */
+int vh_gpu_run_group(vh_solve **ss, int n, int prec, int device, unsigned long chunk, double *loop_ms);
+int vh_run_meshes_gpu_group(const char *const *meshes, const char *const *outdirs, int n, int prec, int device,
+                            int fullPrecision, unsigned long chunk, double *loop_ms, char *err, int errlen);
+
 typedef struct vh_dsolve vh_dsolve;
 vh_dsolve *vh_dsolve_open(vh_solve *s, int prec, int device, unsigned long chunk);
 long vh_dsolve_enqueue(vh_dsolve *d);
+long vh_dsolve_enqueue_group(vh_dsolve **ds, int n);
 int vh_dsolve_ready(vh_dsolve *d);
 int vh_dsolve_complete(vh_dsolve *d);
 unsigned long vh_dsolve_remaining(const vh_dsolve *d);

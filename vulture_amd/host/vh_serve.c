@@ -21,6 +21,13 @@
  * chunk boundaries: a short request that arrives behind long ones shares the
  * device with one other solve at most, without stream priorities.
  *
+ * Packing: a running place whose solve is small (under packCells gobox
+ * cells, a grid whose own launches cannot fill the device) takes up to
+ * VGPU_GROUP_MAX of the small solves next in that order with it, and they
+ * step together in lockstep, one packed launch per step kernel over all of
+ * them (vh_dsolve_enqueue_group, vgpu_run_group).  Each still completes its
+ * chunks on its own, with the same results as alone.
+ *
  * The reference has no counterpart (one solve per process, simulation.c).
  */
 #define _GNU_SOURCE
@@ -48,6 +55,7 @@ typedef struct {
 
 struct vh_server {
   int device, prec, fullPrecision, maxActive, nworkers, lookahead, running;
+  long long packCells;  /* solves of fewer gobox cells share packed launches (0: none) */
   int nextWorker, uploaders;  /* the first `uploaders` workers never set up */
   unsigned long chunk;
   size_t budget, used;
@@ -234,11 +242,28 @@ static void *worker(void *arg)
   return NULL;
 }
 
+/* A solve that failed on the device: its slot and memory are freed and its
+ * request fails with the solve's message. */
+static void drop_failed(vh_server *sv, request *r)
+{
+  pthread_mutex_lock(&sv->mu);
+  sv->used -= r->est;
+  sv->active--;
+  fail_request(sv, r, vh_error(r->s));
+  vh_dsolve_close(r->d);
+  r->d = NULL;
+  vh_destroy(r->s);
+  r->s = NULL;
+  pthread_cond_broadcast(&sv->cvWork);
+  pthread_mutex_unlock(&sv->mu);
+}
+
 /* Scheduler: admission in arrival order, chunk enqueueing, completion. */
 static void *scheduler(void *arg)
 {
   vh_server *sv = arg;
   request **act = calloc((size_t)sv->maxActive + 1, sizeof(request *));
+  char *live = calloc((size_t)sv->maxActive + 1, 1);
   for (;;) {
     pthread_mutex_lock(&sv->mu);
     if (sv->stop) {
@@ -269,17 +294,16 @@ static void *scheduler(void *arg)
     qsort(act, (size_t)n, sizeof(request *), cmp_work_left);
 
     int busy = 0;
+    /* each solve's oldest chunk, once finished; a solve that is done goes to
+     * its writer, one that failed is dropped */
     for (int q = 0; q < n; q++) {
       request *r = act[q];
       int rc = 0;
       const int ready = vh_dsolve_ready(r->d);
+      live[q] = 0;
       if (ready < 0) rc = -1;
       else if (ready == 1) {
-        /* finish the oldest chunk, then keep two in flight if running */
         rc = vh_dsolve_complete(r->d);
-        long e = 0;
-        while (!rc && q < sv->running && (e = vh_dsolve_enqueue(r->d)) > 0) busy = 1;
-        if (e < 0) rc = -1;
         if (!rc && vh_dsolve_done(r->d)) {
           r->info.gpu_ms = vh_dsolve_loop_ms(r->d);
           double **td = NULL;
@@ -293,24 +317,50 @@ static void *scheduler(void *arg)
           pthread_cond_broadcast(&sv->cvWork);
           pthread_mutex_unlock(&sv->mu);
           busy = 1;
+          continue;
         }
       }
       if (rc) {
-        pthread_mutex_lock(&sv->mu);
-        sv->used -= r->est;
-        sv->active--;
-        fail_request(sv, r, vh_error(r->s));
-        vh_dsolve_close(r->d);
-        r->d = NULL;
-        vh_destroy(r->s);
-        r->s = NULL;
-        pthread_cond_broadcast(&sv->cvWork);
-        pthread_mutex_unlock(&sv->mu);
+        drop_failed(sv, r);
+        continue;
       }
+      live[q] = ready == 1 ? 1 : 2;  /* 2: oldest chunk still running */
+    }
+    /* new chunks for the first `running` places in least-work-left order: a
+     * place is one solve, whose oldest chunk has finished, or a pack of up to
+     * VGPU_GROUP_MAX small solves taken in that order, stepped together with
+     * packed launches (vh_dsolve_enqueue_group); two chunks stay in flight */
+    for (int q = 0, places = 0; q < n && places < sv->running; q++) {
+      if (!live[q]) continue;
+      request *r = act[q];
+      long e = 0;
+      places++;
+      if (r->info.cells < sv->packCells) {
+        vh_dsolve *pk[VGPU_GROUP_MAX];
+        int idx[VGPU_GROUP_MAX], m = 0;
+        for (int u = q; u < n && m < VGPU_GROUP_MAX; u++)
+          if (live[u] && act[u]->info.cells < sv->packCells && vh_dsolve_remaining(act[u]->d) > 0) {
+            idx[m] = u;
+            pk[m++] = act[u]->d;
+          }
+        if (m > 1) {
+          for (int u = 0; u < m; u++) live[idx[u]] = 0;
+          while ((e = vh_dsolve_enqueue_group(pk, m)) > 0) busy = 1;
+          if (e < 0)
+            for (int u = 0; u < m; u++) drop_failed(sv, act[idx[u]]);
+          continue;
+        }
+      }
+      const int ready = live[q] == 1;
+      live[q] = 0;
+      if (ready)
+        while ((e = vh_dsolve_enqueue(r->d)) > 0) busy = 1;
+      if (e < 0) drop_failed(sv, r);
     }
     if (!busy) usleep(n ? 20 : 200);
   }
   free(act);
+  free(live);
   return NULL;
 }
 
@@ -336,6 +386,11 @@ vh_server *vh_server_create(int device, int prec, int fullPrecision, double budg
   const char *rn = getenv("VH_SERVE_RUNNING");
   if (rn && atoi(rn) > 0) sv->running = atoi(rn);
   sv->uploaders = sv->nworkers > 6 ? 3 : sv->nworkers > 4 ? 2 : 0;
+  /* grids too small to fill the device alone share packed launches;
+   * VH_SERVE_PACK_CELLS overrides (0 turns packing off) */
+  sv->packCells = 1000000;
+  const char *pc = getenv("VH_SERVE_PACK_CELLS");
+  if (pc) sv->packCells = atoll(pc);
   sv->chunk = chunk ? chunk : 200;
   clock_gettime(CLOCK_MONOTONIC, &sv->t0);
   pthread_mutex_init(&sv->mu, NULL);
