@@ -154,6 +154,49 @@ def sibc_box(work, steps, warmup=5):
             "setup_s": round(setup_s, 2)}
 
 
+def packed(work, count, steps=2000):
+    """Small serving requests (32-72 cells per axis before the 8-cell UPML,
+    `steps` steps, FP32) run one at a time, in chunks of 200 steps replayed
+    from captured graphs, and then all together as one packed group
+    (vgpu_run_group, chunks of 200 steps replayed from a captured graph).
+    Solo throughput counts the device time of each solve's loop; the group's
+    counts the host's wall time of its loop, so the comparison favours solo.
+    Every solve's files must be byte-identical in the two runs."""
+    import filecmp
+    import numpy as np
+    import vulture_amd
+    from vulture_amd import meshgen
+    rng = np.random.default_rng(21)
+    d = os.path.join(work, "packed")
+    os.makedirs(d, exist_ok=True)
+    meshes, cells = [], 0
+    for q in range(count):
+        text, n, _ = meshgen.serving_request(rng, nmin=32, nmax=72, steps_min=steps, steps_max=steps)
+        meshes.append(os.path.join(d, f"r{q}.mesh"))
+        with open(meshes[-1], "w") as f:
+            f.write(text)
+        cells += (n[0] + 2) * (n[1] + 2) * (n[2] + 2)
+    solo_ms = 0.0
+    for q, m in enumerate(meshes):
+        solo_ms += vulture_amd.run_mesh(m, os.path.join(d, f"solo{q}"), precision="fp32", chunk=200)
+    group_ms = vulture_amd.run_meshes_group(meshes, [os.path.join(d, f"group{q}") for q in range(count)],
+                                            precision="fp32", chunk=200)
+    same = True
+    for q in range(count):
+        a, b = os.path.join(d, f"solo{q}"), os.path.join(d, f"group{q}")
+        fa = sorted(os.listdir(a))
+        same = same and fa == sorted(os.listdir(b)) and all(
+            filecmp.cmp(os.path.join(a, f), os.path.join(b, f), shallow=False) for f in fa)
+    work_cu = cells * steps
+    return {"workload": f"{count} serving requests, cells per axis log-uniform [32, 72] + 8-cell UPML, graded "
+                        f"lines, SIMPLE/DEBYE blocks, TF/SF, {steps} steps (seed 21)", "dtype": "fp32",
+            "requests": count, "mean_cells": round(cells / count), "solo_loop_s": round(solo_ms * 1e-3, 4),
+            "group_loop_s": round(group_ms * 1e-3, 4),
+            "solo_device_mcells_per_s": round(work_cu / (solo_ms * 1e-3) / 1e6, 1),
+            "group_mcells_per_s": round(work_cu / (group_ms * 1e-3) / 1e6, 1),
+            "group_over_solo": round(solo_ms / group_ms, 3), "files_identical": same}
+
+
 def _fixture_dir(npz, dst, wf_src):
     """Observer files under dst from an FP64-oracle fixture (arrays written by
     tests/golden/make_fp64_fixtures.py), beside wf_src's waveform spectrum."""
@@ -291,6 +334,8 @@ def main():
     ap.add_argument("--boundary", choices=["PML", "PEC", "MUR"], default="PML")
     ap.add_argument("--no-planewave", action="store_true", help="hard point source instead of the TF/SF plane wave")
     ap.add_argument("--serve", type=int, default=32, help="requests in the serving sweep (0 skips it)")
+    ap.add_argument("--packed", type=int, default=16,
+                    help="small requests run solo and as one packed group (0 skips it; at most 16)")
     ap.add_argument("--serve-active", type=int, default=64,
                     help="solves resident at once when serving (device memory permitting)")
     ap.add_argument("--serve-workers", type=int, default=12, help="host threads setting up and writing solves")
@@ -360,6 +405,11 @@ def main():
         out["sibc_box"] = sibc_box(work, args.sibc_steps)
         print(f"# sibc_box: {out['sibc_box']['value']:.1f} Mcell-updates/s, {out['sibc_box']['ms_per_step']:.3f} "
               f"ms/step", file=sys.stderr, flush=True)
+    if args.packed > 0:
+        out["packed"] = packed(work, min(args.packed, 16))
+        print(f"# packed: {out['packed']['group_mcells_per_s']:.0f} Mcell-updates/s as one group against "
+              f"{out['packed']['solo_device_mcells_per_s']:.0f} one at a time, files identical "
+              f"{out['packed']['files_identical']}", file=sys.stderr, flush=True)
     if args.serve > 0:
         from vulture_amd import serve
         rep = serve.sweep(os.path.join(work, "serve"), count=args.serve, load=0.7, precision="fp32",
@@ -373,7 +423,8 @@ def main():
                            f"Poisson arrivals at 70% of isolated capacity; FP32; up to {args.serve_active} solves "
                            f"resident, least work first, the two with least work left running two "
                            f"graph-captured chunks each (200 steps, fewer above 5 M cells: at most 1e9 "
-                           f"cell-updates a chunk)")
+                           f"cell-updates a chunk); a running solve under 1 M cells steps with up to 15 more "
+                           f"such solves in one packed group (vgpu_run_group)")
         out["serving"] = rep
         print(f"# serving: p50 {rep['p50_latency_s']:.3f} s p99 {rep['p99_latency_s']:.3f} s, "
               f"saturated {rep['saturated_mcells_per_s']:.0f} vs isolated {rep['isolated_mcells_per_s']:.0f} "

@@ -11,7 +11,8 @@ TF/SF with the aux grid and the analytic incident field, hard, soft, lumped
 and surface sources, impulse.dat), in both precisions, in one group; and
 eight serving requests (graded lines, SIMPLE and Debye blocks, TF/SF) of
 different sizes and step counts in FP32, run in chunks of 100 steps, so the
-group shrinks as its shorter solves finish.
+group shrinks as its shorter solves finish.  Chunked runs repeat a group,
+which then replays from a captured graph.
 """
 import filecmp
 import os
@@ -56,10 +57,12 @@ def _solo_and_group(meshes, tmp_path, precision, chunk=0):
     return {os.path.basename(m): _same_files(s, g) for m, s, g in zip(meshes, solo, grp)}
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_reference_meshes_grouped_equal_solo(precision, tmp_path):
+@pytest.mark.parametrize("precision,chunk", [("fp64", 0), ("fp32", 0), ("fp64", 50), ("fp32", 50)])
+def test_reference_meshes_grouped_equal_solo(precision, chunk, tmp_path):
+    """chunk 0: one group run per membership; chunk 50: repeated runs of the
+    same group, captured into a graph at the second and replayed after."""
     meshes = [_stage(n, tmp_path / "in" / n) for n in MIXED]
-    differ = _solo_and_group(meshes, tmp_path, precision)
+    differ = _solo_and_group(meshes, tmp_path, precision, chunk=chunk)
     assert not any(differ.values()), {k: v for k, v in differ.items() if v}
 
 

@@ -14,6 +14,7 @@
 #include <cstring>
 #include <array>
 #include <map>
+#include <atomic>
 #include <mutex>
 #include <shared_mutex>
 #include <unordered_map>
@@ -275,6 +276,8 @@ int vgpu_create(const vgpu_grid_desc *g, vgpu_prec prec, int device, vgpu_ctx **
   // this device's no-allocation scratch address, with the device current
   if (scratch_setup(device) != VGPU_OK) return VGPU_EDEVICE;
   vgpu_ctx *c = new vgpu_ctx();
+  static std::atomic<uint64_t> serials{0};
+  c->serial = ++serials;
   c->prec = prec;
   c->device = device;
   c->esz = prec == VGPU_FP64 ? 8 : 4;
@@ -1444,8 +1447,50 @@ int vgpu_run_group(vgpu_ctx *const *ctxs, int n, const uint64_t *first, uint64_t
   for (int i = 0; i < n; ++i) set_step(ctxs[i], first[i], s);
   Ctx *cs[VGPU_GROUP_MAX];
   for (int i = 0; i < n; ++i) cs[i] = ctxs[i];
-  const int r = ctxs[0]->prec == VGPU_FP64 ? run_group_steps<double>(cs, n, nsteps, s)
-                                           : run_group_steps<float>(cs, n, nsteps, s);
+  // The second run in a row of the same group and length is captured into a
+  // graph (held by the first member), and later ones replay it; a group that
+  // changes is launched directly.  Serials identify members, so a graph never
+  // outlives the contexts it names in a way that could match again.
+  Ctx *lead = cs[0];
+  std::vector<uint64_t> key(n);
+  for (int i = 0; i < n; ++i) key[i] = cs[i]->serial;
+  const char *ge = getenv("VGPU_GRAPHS");
+  const bool same = key == lead->groupKey && nsteps == lead->groupSteps;
+  if (!same) {
+    if (lead->groupExec) hipGraphExecDestroy(lead->groupExec);
+    if (lead->groupGraph) hipGraphDestroy(lead->groupGraph);
+    lead->groupExec = nullptr;
+    lead->groupGraph = nullptr;
+    lead->groupKey = key;
+    lead->groupSteps = nsteps;
+    lead->groupTried = false;
+  } else if (!lead->groupTried && !(ge && ge[0] == '0')) {
+    std::unique_lock<std::shared_mutex> alone(g_capture_mu);
+    lead->groupTried = true;
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+      const int rc = lead->prec == VGPU_FP64 ? run_group_steps<double>(cs, n, nsteps, s)
+                                             : run_group_steps<float>(cs, n, nsteps, s);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(s, &g);
+      hipGraphExec_t x = nullptr;
+      if (rc == VGPU_OK && e == hipSuccess && g && hipGraphInstantiate(&x, g, nullptr, nullptr, 0) == hipSuccess &&
+          x) {
+        lead->groupGraph = g;
+        lead->groupExec = x;
+      } else if (g) {
+        hipGraphDestroy(g);
+      }
+    }
+    (void)hipGetLastError();
+  }
+  int r = VGPU_OK;
+  if (lead->groupExec) {
+    if (hipGraphLaunch(lead->groupExec, s) != hipSuccess)
+      r = set_error(lead, VGPU_EDEVICE, "graph launch failed: %s", hipGetErrorString(hipGetLastError()));
+    ++lead->graphRuns;
+  } else {
+    r = lead->prec == VGPU_FP64 ? run_group_steps<double>(cs, n, nsteps, s) : run_group_steps<float>(cs, n, nsteps, s);
+  }
   // the first member closes last, so its end follows all of the group's work
   // on its stream
   for (int k = 1; k <= n; ++k) {
@@ -1667,6 +1712,8 @@ static void destroy_ctx(vgpu_ctx *c)
   for (void *p : c->allocs) hipFree(p);
   if (c->graphExec) hipGraphExecDestroy(c->graphExec);
   if (c->graph) hipGraphDestroy(c->graph);
+  if (c->groupExec) hipGraphExecDestroy(c->groupExec);
+  if (c->groupGraph) hipGraphDestroy(c->groupGraph);
   for (auto e : c->evE) hipEventDestroy(e);
   for (auto e : c->evH) hipEventDestroy(e);
   if (c->ev0) hipEventDestroy(c->ev0);

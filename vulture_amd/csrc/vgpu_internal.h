@@ -612,6 +612,15 @@ struct Ctx {
   int graphPar = 0;                          // field set current when the graph was captured
   uint64_t graphRuns = 0;                    // runs replayed from it
   bool graphTried = false, lastRunTimed = false;
+  // group runs led by this context: the last group's members (context
+  // serials, never reused) and length, and a graph captured once the same
+  // group runs twice in a row
+  uint64_t serial = 0;
+  std::vector<uint64_t> groupKey;
+  uint64_t groupSteps = 0;
+  hipGraph_t groupGraph = nullptr;
+  hipGraphExec_t groupExec = nullptr;
+  bool groupTried = false;
   void *d_ocell = nullptr, *d_ofirst = nullptr, *d_olast = nullptr, *d_onf = nullptr, *d_ooff = nullptr;
   void *d_oomega = nullptr, *d_otd = nullptr, *d_odre = nullptr, *d_odim = nullptr;
   long long vnodes = 0;
