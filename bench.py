@@ -424,7 +424,8 @@ def main():
                            f"resident, least work first, the two with least work left running two "
                            f"graph-captured chunks each (200 steps, fewer above 5 M cells: at most 1e9 "
                            f"cell-updates a chunk); a running solve under 1 M cells steps with up to 15 more "
-                           f"such solves in one packed group (vgpu_run_group)")
+                           f"such solves in one packed group (vgpu_run_group), and no larger solve runs "
+                           f"beside such a group")
         out["serving"] = rep
         print(f"# serving: p50 {rep['p50_latency_s']:.3f} s p99 {rep['p99_latency_s']:.3f} s, "
               f"saturated {rep['saturated_mcells_per_s']:.0f} vs isolated {rep['isolated_mcells_per_s']:.0f} "

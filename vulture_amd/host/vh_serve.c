@@ -26,7 +26,8 @@
  * VGPU_GROUP_MAX of the small solves next in that order with it, and they
  * step together in lockstep, one packed launch per step kernel over all of
  * them (vh_dsolve_enqueue_group, vgpu_run_group).  Each still completes its
- * chunks on its own, with the same results as alone.
+ * chunks on its own, with the same results as alone.  While the first
+ * running place is small, the other places go to small solves only.
  *
  * The reference has no counterpart (one solve per process, simulation.c).
  */
@@ -56,6 +57,7 @@ typedef struct {
 struct vh_server {
   int device, prec, fullPrecision, maxActive, nworkers, lookahead, running;
   long long packCells;  /* solves of fewer gobox cells share packed launches (0: none) */
+  int smallAlone;       /* a running small solve shares the device with small ones only */
   int nextWorker, uploaders;  /* the first `uploaders` workers never set up */
   unsigned long chunk;
   size_t budget, used;
@@ -330,10 +332,16 @@ static void *scheduler(void *arg)
      * place is one solve, whose oldest chunk has finished, or a pack of up to
      * VGPU_GROUP_MAX small solves taken in that order, stepped together with
      * packed launches (vh_dsolve_enqueue_group); two chunks stay in flight */
+    int firstSmall = 0;
     for (int q = 0, places = 0; q < n && places < sv->running; q++) {
       if (!live[q]) continue;
       request *r = act[q];
       long e = 0;
+      /* with smallAlone, small solves (latency-bound, slowed most by a large
+       * grid's long kernels between theirs) share the device only with each
+       * other */
+      if (places == 0) firstSmall = r->info.cells < sv->packCells;
+      else if (sv->smallAlone && firstSmall && r->info.cells >= sv->packCells) break;
       places++;
       if (r->info.cells < sv->packCells) {
         vh_dsolve *pk[VGPU_GROUP_MAX];
@@ -391,6 +399,12 @@ vh_server *vh_server_create(int device, int prec, int fullPrecision, double budg
   sv->packCells = 1000000;
   const char *pc = getenv("VH_SERVE_PACK_CELLS");
   if (pc) sv->packCells = atoll(pc);
+  /* a running small solve shares the device with small ones only: a large
+   * grid's long kernels between its short ones slow it most (32 requests:
+   * p99 latency / isolated 2.07 -> 1.91 at the same throughput);
+   * VH_SERVE_SMALL_ALONE=0 lets it share with any */
+  const char *sa = getenv("VH_SERVE_SMALL_ALONE");
+  sv->smallAlone = !(sa && atoi(sa) == 0);
   sv->chunk = chunk ? chunk : 200;
   clock_gettime(CLOCK_MONOTONIC, &sv->t0);
   pthread_mutex_init(&sv->mu, NULL);
