@@ -536,6 +536,10 @@ struct Ctx {
   void *d_epcs = nullptr, *d_ework = nullptr;   // E shell: pieces, launch blocks
   void *d_hpcs = nullptr, *d_hwork = nullptr;   // H shell
   int nEwork = 0, nHwork = 0, nEpcs = 0, nHpcs = 0;
+  // the same shells at the most planes per block, for the packed launches of
+  // group runs, which hold many grids' blocks
+  void *d_epcsPk = nullptr, *d_eworkPk = nullptr, *d_hpcsPk = nullptr, *d_hworkPk = nullptr;
+  int nEworkPk = 0, nHworkPk = 0;
   long long shellCells[2] = {};              // cells of the E and H shell pieces
   void *d_dbase = nullptr;                   // [3][NZ][NY][NCH] + 1: first layer-0 site
 

@@ -112,7 +112,7 @@ template <typename T> static void run_aux(const HArgs<T> &a, int r, int C, void 
 // each XCD gets one contiguous eighth of every piece's blocks.
 template <int NB, typename A>
 static int build_shell(const A &a, const int (*in)[NB][6], const int *fb, const int N[3], std::vector<ShellPiece> &pcs,
-                       std::vector<ShellWork> &work, long long &cells, std::string &err, int lw = 0)
+                       std::vector<ShellWork> &work, long long &cells, std::string &err, int lw = 0, int kcPlan = 0)
 {
   const bool hasFast = fb[0] <= fb[1] && fb[2] <= fb[3] && fb[4] <= fb[5];
   std::vector<int> cut[3];
@@ -259,6 +259,7 @@ static int build_shell(const A &a, const int (*in)[NB][6], const int *fb, const 
     rawCells += (long long)(pc.box[1] - pc.box[0] + 1) * (pc.box[3] - pc.box[2] + 1) * (pc.box[5] - pc.box[4] + 1);
   int kcmax = (int)std::max(1LL, std::min((long long)SHELL_KC, rawCells / (256LL * 512)));
   if (const char *ev = getenv("VGPU_SHELL_KC")) kcmax = std::max(1, std::min(atoi(ev), 64));  // tuning override
+  if (kcPlan > 0) kcmax = kcPlan;  // a plan for launches shared with other grids (group runs)
   cells = 0;
   pcs.clear();
   for (ShellPiece pc : raw) {
@@ -309,6 +310,24 @@ static int build_shell(const A &a, const int (*in)[NB][6], const int *fb, const 
   return 0;
 }
 
+// The E and H shells of a context (build_shell); kcPlan > 0 fixes the planes
+// per block.
+template <typename T>
+static int plan_shells(Ctx *c, int kcPlan, std::vector<ShellPiece> &ep, std::vector<ShellWork> &ew,
+                       std::vector<ShellPiece> &hp, std::vector<ShellWork> &hw, long long cells[2], std::string &err)
+{
+  EArgs<T> ea{};
+  HArgs<T> ha{};
+  fill_volume_args<T>(c, ea, ha, 0);
+  const int N[3] = {c->NX, c->NY, c->NZ};
+  int ein[3][1][6];
+  for (int C = 0; C < 3; ++C)
+    for (int q = 0; q < 6; ++q) ein[C][0][q] = ea.in[C][q];
+  const int lw = getenv("VGPU_SHELL_ROWS") && getenv("VGPU_SHELL_ROWS")[0] == '0' ? 0 : (int)(128 / sizeof(T));
+  return build_shell<1>(ea, ein, c->fastE, N, ep, ew, cells[0], err, lw, kcPlan) ||
+         build_shell<3>(ha, ha.in, c->fastH, N, hp, hw, cells[1], err, lw, kcPlan);
+}
+
 // Once per context (at finalize): the sweeps' argument blocks go to device
 // memory, where the shell kernels and the fused Debye step read them.
 template <typename T> int prepare_sweeps(Ctx *c)
@@ -323,17 +342,10 @@ template <typename T> int prepare_sweeps(Ctx *c)
   HArgs<T> ha{}, ha1{};
   fill_volume_args<T>(c, ea, ha, 0);
   fill_volume_args<T>(c, ea1, ha1, 1);
-  const int N[3] = {c->NX, c->NY, c->NZ};
   std::vector<ShellPiece> ep, hp;
   std::vector<ShellWork> ew, hw;
   std::string err;
-  int ein[3][1][6];
-  for (int C = 0; C < 3; ++C)
-    for (int q = 0; q < 6; ++q) ein[C][0][q] = ea.in[C][q];
-  const int lw = getenv("VGPU_SHELL_ROWS") && getenv("VGPU_SHELL_ROWS")[0] == '0' ? 0 : (int)(128 / sizeof(T));
-  if (build_shell<1>(ea, ein, c->fastE, N, ep, ew, c->shellCells[0], err, lw) ||
-      build_shell<3>(ha, ha.in, c->fastH, N, hp, hw, c->shellCells[1], err, lw))
-    return set_error(c, VGPU_EINVAL, "%s", err.c_str());
+  if (plan_shells<T>(c, 0, ep, ew, hp, hw, c->shellCells, err)) return set_error(c, VGPU_EINVAL, "%s", err.c_str());
   c->nEpcs = (int)ep.size(); c->nHpcs = (int)hp.size();
   c->nEwork = (int)ew.size(); c->nHwork = (int)hw.size();
   for (int q = 0; q < 2; ++q) {
@@ -894,10 +906,32 @@ template <typename T> int upload_pack(Ctx *c)
 {
   StepArgs<T> A{};
   fill_step_args<T>(c, A, nullptr);
+  // shells at SHELL_KC planes per block: a packed launch fills the device
+  // with many grids' blocks, where a small grid's own plan used one plane
+  std::vector<ShellPiece> ep, hp;
+  std::vector<ShellWork> ew, hw;
+  std::string err;
+  long long cells[2];
+  if (plan_shells<T>(c, SHELL_KC, ep, ew, hp, hw, cells, err)) return set_error(c, VGPU_EINVAL, "%s", err.c_str());
+  c->d_epcsPk = dev_alloc(c, ep.size() * sizeof(ShellPiece));
+  c->d_hpcsPk = dev_alloc(c, hp.size() * sizeof(ShellPiece));
+  c->d_eworkPk = dev_alloc(c, ew.size() * sizeof(ShellWork));
+  c->d_hworkPk = dev_alloc(c, hw.size() * sizeof(ShellWork));
+  if (!c->d_epcsPk || !c->d_hpcsPk || !c->d_eworkPk || !c->d_hworkPk)
+    return set_error(c, VGPU_ENOMEM, "group-run shells");
+  auto up = [](void *d, const void *h, size_t n) { return n == 0 || hipMemcpy(d, h, n, hipMemcpyHostToDevice) == hipSuccess; };
+  if (!up(c->d_epcsPk, ep.data(), ep.size() * sizeof(ShellPiece)) ||
+      !up(c->d_hpcsPk, hp.data(), hp.size() * sizeof(ShellPiece)) ||
+      !up(c->d_eworkPk, ew.data(), ew.size() * sizeof(ShellWork)) ||
+      !up(c->d_hworkPk, hw.data(), hw.size() * sizeof(ShellWork)))
+    return set_error(c, VGPU_EDEVICE, "group-run shells upload");
+  c->nEworkPk = (int)ew.size();
+  c->nHworkPk = (int)hw.size();
   PackArgs<T> p{};
   p.fe = A.fe[0];
   p.fh = A.fh[0];
-  p.epc = A.epc; p.hpc = A.hpc; p.ewk = A.ewk; p.hwk = A.hwk;
+  p.epc = P<ShellPiece>(c->d_epcsPk); p.hpc = P<ShellPiece>(c->d_hpcsPk);
+  p.ewk = P<ShellWork>(c->d_eworkPk); p.hwk = P<ShellWork>(c->d_hworkPk);
   p.ea = A.eap[0]; p.ha = A.hap[0];
   p.da = A.da[0];
   p.ce = A.ce[0][0];
@@ -949,8 +983,8 @@ template <typename T> int run_group_steps(Ctx *const *cs, int n, uint64_t nsteps
       return cs[i]->indexed == idx && A[i].egrid.x ? (long long)A[i].eg1.x : 0LL; });
     fastH[x] = plan([&](int i, long long &, long long &) {
       return cs[i]->indexed == idx && A[i].hgrid.x ? (long long)A[i].hg1.x : 0LL; });
-    shellE[x] = plan([&](int i, long long &, long long &) { return cs[i]->indexed == idx ? (long long)cs[i]->nEwork : 0LL; });
-    shellH[x] = plan([&](int i, long long &, long long &) { return cs[i]->indexed == idx ? (long long)cs[i]->nHwork : 0LL; });
+    shellE[x] = plan([&](int i, long long &, long long &) { return cs[i]->indexed == idx ? (long long)cs[i]->nEworkPk : 0LL; });
+    shellH[x] = plan([&](int i, long long &, long long &) { return cs[i]->indexed == idx ? (long long)cs[i]->nHworkPk : 0LL; });
   }
   unsigned headGx = 1;
   for (int i = 0; i < n; ++i)
