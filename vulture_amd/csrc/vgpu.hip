@@ -1456,9 +1456,35 @@ int vgpu_run_group(vgpu_ctx *const *ctxs, int n, const uint64_t *first, uint64_t
   for (int i = 0; i < n; ++i) key[i] = cs[i]->serial;
   const char *ge = getenv("VGPU_GRAPHS");
   const bool same = key == lead->groupKey && nsteps == lead->groupSteps;
+  // replaced graphs whose last launch has finished are freed; a graph still
+  // in flight is retired behind an event instead of being freed under it
+  for (size_t q = 0; q < lead->retired.size();) {
+    Ctx::RetiredGraph &g = lead->retired[q];
+    if (hipEventQuery(g.done) != hipSuccess) {
+      ++q;
+      continue;
+    }
+    hipGraphExecDestroy(g.exec);
+    hipGraphDestroy(g.graph);
+    hipEventDestroy(g.done);
+    g = lead->retired.back();
+    lead->retired.pop_back();
+  }
+  (void)hipGetLastError();
   if (!same) {
-    if (lead->groupExec) hipGraphExecDestroy(lead->groupExec);
-    if (lead->groupGraph) hipGraphDestroy(lead->groupGraph);
+    if (lead->groupExec) {
+      Ctx::RetiredGraph g{lead->groupGraph, lead->groupExec, nullptr};
+      if (hipEventCreateWithFlags(&g.done, hipEventDisableTiming) == hipSuccess &&
+          hipEventRecord(g.done, s) == hipSuccess) {
+        lead->retired.push_back(g);
+      } else {  // no event: wait for the stream before freeing the graph
+        if (g.done) hipEventDestroy(g.done);
+        const DeviceCalls guard(g_capture_mu);
+        hipStreamSynchronize(s);
+        hipGraphExecDestroy(g.exec);
+        hipGraphDestroy(g.graph);
+      }
+    }
     lead->groupExec = nullptr;
     lead->groupGraph = nullptr;
     lead->groupKey = key;
@@ -1714,6 +1740,11 @@ static void destroy_ctx(vgpu_ctx *c)
   if (c->graph) hipGraphDestroy(c->graph);
   if (c->groupExec) hipGraphExecDestroy(c->groupExec);
   if (c->groupGraph) hipGraphDestroy(c->groupGraph);
+  for (auto &g : c->retired) {  // the stream has drained (wait_last_run)
+    hipGraphExecDestroy(g.exec);
+    hipGraphDestroy(g.graph);
+    hipEventDestroy(g.done);
+  }
   for (auto e : c->evE) hipEventDestroy(e);
   for (auto e : c->evH) hipEventDestroy(e);
   if (c->ev0) hipEventDestroy(c->ev0);

@@ -625,6 +625,10 @@ struct Ctx {
   hipGraph_t groupGraph = nullptr;
   hipGraphExec_t groupExec = nullptr;
   bool groupTried = false;
+  // replaced group graphs, freed once the event recorded after their last
+  // launch has completed
+  struct RetiredGraph { hipGraph_t graph; hipGraphExec_t exec; hipEvent_t done; };
+  std::vector<RetiredGraph> retired;
   void *d_ocell = nullptr, *d_ofirst = nullptr, *d_olast = nullptr, *d_onf = nullptr, *d_ooff = nullptr;
   void *d_oomega = nullptr, *d_otd = nullptr, *d_odre = nullptr, *d_odim = nullptr;
   long long vnodes = 0;

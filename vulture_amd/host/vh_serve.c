@@ -260,7 +260,7 @@ static void drop_failed(vh_server *sv, request *r)
   pthread_mutex_unlock(&sv->mu);
 }
 
-/* Scheduler: admission in arrival order, chunk enqueueing, completion. */
+/* Scheduler: admission (least work first), completion, chunk enqueueing. */
 static void *scheduler(void *arg)
 {
   vh_server *sv = arg;
