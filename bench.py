@@ -27,6 +27,11 @@ per UPML cell, the Debye state), over ms_per_step.
 sources with its OpenMP option, built by oracle/Makefile) on a bounded number
 of steps of the same mesh on this host, when that build is present; the CPU
 restatement of the loop is timed beside it (`cpu_port`).
+
+Further records of the same run: `fp32_error` (the full 3000 steps of both
+forms against the FP64 oracle's series), `sibc_box` (the SIBC + Mur box),
+`packed` (16 small requests one at a time and as one packed group) and
+`serving` (the solve server on a seeded stream of the contract's shape).
 """
 import argparse
 import json
