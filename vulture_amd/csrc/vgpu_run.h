@@ -919,7 +919,9 @@ template <typename T> int upload_pack(Ctx *c)
   c->d_hworkPk = dev_alloc(c, hw.size() * sizeof(ShellWork));
   if (!c->d_epcsPk || !c->d_hpcsPk || !c->d_eworkPk || !c->d_hworkPk)
     return set_error(c, VGPU_ENOMEM, "group-run shells");
-  auto up = [](void *d, const void *h, size_t n) { return n == 0 || hipMemcpy(d, h, n, hipMemcpyHostToDevice) == hipSuccess; };
+  auto up = [](void *d, const void *h, size_t n) {
+    return n == 0 || hipMemcpy(d, h, n, hipMemcpyHostToDevice) == hipSuccess;
+  };
   if (!up(c->d_epcsPk, ep.data(), ep.size() * sizeof(ShellPiece)) ||
       !up(c->d_hpcsPk, hp.data(), hp.size() * sizeof(ShellPiece)) ||
       !up(c->d_eworkPk, ew.data(), ew.size() * sizeof(ShellWork)) ||
@@ -983,8 +985,10 @@ template <typename T> int run_group_steps(Ctx *const *cs, int n, uint64_t nsteps
       return cs[i]->indexed == idx && A[i].egrid.x ? (long long)A[i].eg1.x : 0LL; });
     fastH[x] = plan([&](int i, long long &, long long &) {
       return cs[i]->indexed == idx && A[i].hgrid.x ? (long long)A[i].hg1.x : 0LL; });
-    shellE[x] = plan([&](int i, long long &, long long &) { return cs[i]->indexed == idx ? (long long)cs[i]->nEworkPk : 0LL; });
-    shellH[x] = plan([&](int i, long long &, long long &) { return cs[i]->indexed == idx ? (long long)cs[i]->nHworkPk : 0LL; });
+    shellE[x] = plan([&](int i, long long &, long long &) {
+      return cs[i]->indexed == idx ? (long long)cs[i]->nEworkPk : 0LL; });
+    shellH[x] = plan([&](int i, long long &, long long &) {
+      return cs[i]->indexed == idx ? (long long)cs[i]->nHworkPk : 0LL; });
   }
   unsigned headGx = 1;
   for (int i = 0; i < n; ++i)
@@ -1018,6 +1022,7 @@ template <typename T> int run_group_steps(Ctx *const *cs, int n, uint64_t nsteps
       cnt = dl[i][j].second;
       return (cnt + 255) / 256; });
   const dim3 vblk = A[0].vblk;
+  auto blocks = [](const PackMap &m) { return dim3(m.start[m.n]); };  // a packed launch's 1-D grid
   for (uint64_t st = 0; st < nsteps; ++st) {
     const int soff = (int)st;
     if (head.n) {
@@ -1026,33 +1031,33 @@ template <typename T> int run_group_steps(Ctx *const *cs, int n, uint64_t nsteps
     }
     if (vol.n) {
       vol.soff = soff;
-      hipLaunchKernelGGL(k_observe_volume_pk<T>, dim3(vol.start[vol.n]), dim3(256), 0, s, vol);
+      hipLaunchKernelGGL(k_observe_volume_pk<T>, blocks(vol), dim3(256), 0, s, vol);
     }
     for (int i = 0; i < n; ++i)
       if (cs[i]->murmask) hipLaunchKernelGGL(k_mur_e<T>, A[i].murGrid, dim3(16, 16), 0, s, A[i].ma[0]);
-    if (fastE[0].n) hipLaunchKernelGGL((k_e_fast_pk<T, true>), dim3(fastE[0].start[fastE[0].n]), vblk, 0, s, fastE[0]);
-    if (fastE[1].n) hipLaunchKernelGGL((k_e_fast_pk<T, false>), dim3(fastE[1].start[fastE[1].n]), vblk, 0, s, fastE[1]);
-    if (shellE[0].n) hipLaunchKernelGGL((k_e_shell_pk<T, true>), dim3(shellE[0].start[shellE[0].n]), dim3(256), 0, s, shellE[0]);
-    if (shellE[1].n) hipLaunchKernelGGL((k_e_shell_pk<T, false>), dim3(shellE[1].start[shellE[1].n]), dim3(256), 0, s, shellE[1]);
-    for (const PackMap &m : debye) hipLaunchKernelGGL(k_debye_pk<T>, dim3(m.start[m.n]), dim3(256), 0, s, m);
+    if (fastE[0].n) hipLaunchKernelGGL((k_e_fast_pk<T, true>), blocks(fastE[0]), vblk, 0, s, fastE[0]);
+    if (fastE[1].n) hipLaunchKernelGGL((k_e_fast_pk<T, false>), blocks(fastE[1]), vblk, 0, s, fastE[1]);
+    if (shellE[0].n) hipLaunchKernelGGL((k_e_shell_pk<T, true>), blocks(shellE[0]), dim3(256), 0, s, shellE[0]);
+    if (shellE[1].n) hipLaunchKernelGGL((k_e_shell_pk<T, false>), blocks(shellE[1]), dim3(256), 0, s, shellE[1]);
+    for (const PackMap &m : debye) hipLaunchKernelGGL(k_debye_pk<T>, blocks(m), dim3(256), 0, s, m);
     for (int i = 0; i < n; ++i)
       if (cs[i]->sibcFaces)
         hipLaunchKernelGGL(k_sibc_e<T>, dim3((unsigned)cs[i]->sibcBlocks), dim3(128), 0, s, A[i].se[0]);
     if (corrE.n) {
       corrE.soff = soff;
-      hipLaunchKernelGGL(k_corr_e_pk<T>, dim3(corrE.start[corrE.n]), dim3(256), 0, s, corrE, pi);
+      hipLaunchKernelGGL(k_corr_e_pk<T>, blocks(corrE), dim3(256), 0, s, corrE, pi);
     }
     for (int i = 0; i < n; ++i) launch_ghosts<T>(cs[i], 1, 0, s);
-    if (fastH[0].n) hipLaunchKernelGGL((k_h_fast_pk<T, true>), dim3(fastH[0].start[fastH[0].n]), vblk, 0, s, fastH[0]);
-    if (fastH[1].n) hipLaunchKernelGGL((k_h_fast_pk<T, false>), dim3(fastH[1].start[fastH[1].n]), vblk, 0, s, fastH[1]);
-    if (shellH[0].n) hipLaunchKernelGGL((k_h_shell_pk<T, true>), dim3(shellH[0].start[shellH[0].n]), dim3(256), 0, s, shellH[0]);
-    if (shellH[1].n) hipLaunchKernelGGL((k_h_shell_pk<T, false>), dim3(shellH[1].start[shellH[1].n]), dim3(256), 0, s, shellH[1]);
+    if (fastH[0].n) hipLaunchKernelGGL((k_h_fast_pk<T, true>), blocks(fastH[0]), vblk, 0, s, fastH[0]);
+    if (fastH[1].n) hipLaunchKernelGGL((k_h_fast_pk<T, false>), blocks(fastH[1]), vblk, 0, s, fastH[1]);
+    if (shellH[0].n) hipLaunchKernelGGL((k_h_shell_pk<T, true>), blocks(shellH[0]), dim3(256), 0, s, shellH[0]);
+    if (shellH[1].n) hipLaunchKernelGGL((k_h_shell_pk<T, false>), blocks(shellH[1]), dim3(256), 0, s, shellH[1]);
     for (int i = 0; i < n; ++i)
       if (cs[i]->sibcFaces)
         hipLaunchKernelGGL(k_sibc_h<T>, dim3((unsigned)((A[i].sh[0].nrows + 255) / 256)), dim3(256), 0, s, A[i].sh[0]);
     if (corrH.n) {
       corrH.soff = soff;
-      hipLaunchKernelGGL(k_corr_h_pk<T>, dim3(corrH.start[corrH.n]), dim3(256), 0, s, corrH, pi);
+      hipLaunchKernelGGL(k_corr_h_pk<T>, blocks(corrH), dim3(256), 0, s, corrH, pi);
     }
     for (int i = 0; i < n; ++i) launch_ghosts<T>(cs[i], 0, 0, s);
   }
