@@ -84,3 +84,25 @@ def test_group_limits():
         vulture_amd.run_meshes_group([], [])
     with pytest.raises(ValueError):
         vulture_amd.run_meshes_group(["a"] * (vulture_amd.GROUP_MAX + 1), ["b"] * (vulture_amd.GROUP_MAX + 1))
+
+
+def test_fused_sweep_solves_are_not_grouped(tmp_path, monkeypatch):
+    """Contexts with the fused leapfrog sweep (VGPU_LEAP=1) have no packed
+    form: a group run of them is refused with a message, and the solve server
+    runs such small solves one by one, with the files of each run alone."""
+    from vulture_amd import serve
+    monkeypatch.setenv("VGPU_LEAP", "1")
+    meshes = [_stage(n, tmp_path / "in" / n) for n in ("freespace_pml", "dipole_vry", "softsrc_imdz")]
+    print("case group of fused-sweep contexts", flush=True)
+    with pytest.raises(vulture_amd.VgpuError, match="separate sweeps"):
+        vulture_amd.run_meshes_group(meshes, [str(tmp_path / f"g{i}") for i in range(3)], precision="fp32")
+    solo = [str(tmp_path / f"solo{i}") for i in range(3)]
+    for m, o in zip(meshes, solo):
+        vulture_amd.run_mesh(m, o, precision="fp32", chunk=100)
+    print("case server with fused-sweep contexts", flush=True)
+    with serve.Server(precision="fp32", max_active=4, workers=4, chunk=100) as sv:
+        rids = [sv.submit(m, str(tmp_path / f"srv{i}")) for i, m in enumerate(meshes)]
+        infos = [sv.wait(r) for r in rids]
+    assert all(i["status"] == 1 for i in infos), infos
+    for i in range(3):
+        assert not _same_files(solo[i], str(tmp_path / f"srv{i}")), i

@@ -226,6 +226,7 @@ struct vh_dsolve {
   unsigned long qfirst[2], qn[2];  /* runs in flight, oldest first */
   int qlen;
   int nbin;                    /* binary observers: rows streamed as runs complete */
+  int packable;                /* the context can take part in group runs (separate sweeps) */
   int *pid, *vid;
   uint64_t *vnodes;
   size_t volNodes;
@@ -273,12 +274,15 @@ vh_dsolve *vh_dsolve_open(vh_solve *s, int prec, int device, unsigned long chunk
     }
   for (int o = 0; o < s->numObs; o++) d->nbin += s->obs[o].format == OF_BINARY;
   s->binaryStreamed = d->nbin > 0;
+  vgpu_stats st;
+  d->packable = vgpu_get_stats(d->c, &st) == 0 && !st.leap && !st.debye_fused;
   return d;
 }
 
 vgpu_ctx *vh_dsolve_context(vh_dsolve *d) { return d->c; }
 unsigned long vh_dsolve_remaining(const vh_dsolve *d) { return d->NT - d->enq; }
 int vh_dsolve_done(const vh_dsolve *d) { return d->next >= d->NT; }
+int vh_dsolve_packable(const vh_dsolve *d) { return d->packable; }
 
 /* Enqueue the next chunk of steps on the context's stream (asynchronous),
  * keeping at most two chunks in flight, so the device runs one while the

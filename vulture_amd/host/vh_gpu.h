@@ -31,6 +31,9 @@ int vh_dsolve_ready(vh_dsolve *d);
 int vh_dsolve_complete(vh_dsolve *d);
 unsigned long vh_dsolve_remaining(const vh_dsolve *d);
 int vh_dsolve_done(const vh_dsolve *d);
+/* 1 when the solve can step in a group (vh_dsolve_enqueue_group): its
+ * context uses the separate sweeps, not the fused leapfrog or Debye forms. */
+int vh_dsolve_packable(const vh_dsolve *d);
 int vh_dsolve_collect(vh_dsolve *d, double ***td);
 void vh_dsolve_write(vh_dsolve *d, double **td);
 double vh_dsolve_loop_ms(const vh_dsolve *d);

@@ -347,7 +347,8 @@ static void *scheduler(void *arg)
         vh_dsolve *pk[VGPU_GROUP_MAX];
         int idx[VGPU_GROUP_MAX], m = 0;
         for (int u = q; u < n && m < VGPU_GROUP_MAX; u++)
-          if (live[u] && act[u]->info.cells < sv->packCells && vh_dsolve_remaining(act[u]->d) > 0) {
+          if (live[u] && act[u]->info.cells < sv->packCells && vh_dsolve_remaining(act[u]->d) > 0 &&
+              vh_dsolve_packable(act[u]->d)) {
             idx[m] = u;
             pk[m++] = act[u]->d;
           }
